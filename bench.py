@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the direct-summation force path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload r3_2m]
+
+A step is one force evaluation of all N rows against all N sources (N^2
+pair interactions, self pair included as in the reference) on particles that
+are already resident in HBM.  The default workload is the compactified R^3
+configuration at N = 2^21 (seed 1003): the FP64 path is the headline value;
+the mixed-precision path, its accuracy against FP64, and the CPU baseline
+(the reference-faithful OpenMP restatement on this host's cores, on a bounded
+sample of rows) are reported on the same JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import steps_amd  # noqa: E402
+from steps_amd import workloads as wl  # noqa: E402
+
+# Algorithmic FLOP per pair interaction on the dominant r >= beta branch, as
+# written in the reference (forces_cuda.cu:533-542, :482-484): beta 1,
+# d = xj - xi 3, r = sqrt(dx^2+dy^2+dz^2) 6, r^2, r^3, 1/r^3 3, *M[j] 1,
+# F += w*d 6  ->  20.  sqrt and divide count as 1, loop invariants excluded.
+FLOP_PER_PAIR_R3 = 20
+PEAK_FP64_VECTOR_TFLOPS = 78.6    # MI355X published FP64 vector peak
+PEAK_FP32_VECTOR_TFLOPS = 157.3   # MI355X published FP32 vector peak (packed)
+
+WORKLOADS = {
+    # name: (generator kwargs, description)
+    "r3_2m": dict(N=2 ** 21, seed=1003),
+    "r3_16k": dict(N=16384, seed=1000),
+}
+
+
+def cpu_baseline(w, target_s: float = 12.0) -> dict:
+    """Reference-faithful OpenMP CPU direct sum on a bounded row sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib  # the oracle: timed here as the CPU baseline only
+
+    lib = oracle_lib.lib()
+    threads = int(lib.oracle_max_threads())
+    p = oracle_lib.params(0, cosmology=w.cosmology, comoving=w.comoving,
+                          mass_in_unit_sphere=w.mass_in_unit_sphere)
+    N = w.N
+
+    def run(nrows, kind):
+        rows = (np.arange(nrows, dtype=np.int64) * N // nrows).astype(np.int32)
+        t0 = time.perf_counter()
+        oracle_lib.forces_rows(p, w.x, w.M, w.soft, rows, kind=kind)
+        return time.perf_counter() - t0
+
+    calib = max(16, threads * 2)
+    t = run(calib, "faithful")
+    nrows = int(min(N, max(calib, calib * target_s / max(t, 1e-3))))
+    t = run(nrows, "faithful")
+    rate = nrows * N / t
+    tf = run(nrows, "fast")
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": rate / 1e9,
+        "unit": "G pair-interactions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{nrows} strided target rows x all {N} sources (R3, FP64, omp dynamic + atomics, "
+                  f"-O3 -ffast-math), {t:.1f} s",
+        "cpu_model": cpu_model,
+        "nproc": os.cpu_count(),
+        "optimised_variant_value": nrows * N / tf / 1e9,
+    }
+
+
+def time_path(sim, steps: int, warmup: int):
+    for _ in range(warmup):
+        sim.forces()
+        sim.synchronize()
+    sim.synchronize()
+    pair_ms = []
+    total_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.forces()
+        a, b = sim.last_kernel_ms()   # HIP events on the sim's own stream
+        pair_ms.append(a)
+        total_ms.append(b)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, float(np.mean(pair_ms)), float(np.mean(total_ms))
+
+
+def load_traffic(name: str):
+    f = ROOT / "profiles" / f"{name}_traffic.json"
+    if f.exists():
+        try:
+            return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="r3_2m", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs the full N x N interaction on a single "
+              "MI355X; multi-GPU sharding is design only (see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    gen = WORKLOADS[args.workload]
+    w = wl.r3_compactified(**gen)
+    N = w.N
+    kw = dict(cosmology=w.cosmology, comoving=w.comoving, mass_in_unit_sphere=w.mass_in_unit_sphere)
+    d64, k64 = steps_amd.make_desc(steps_amd.TOPO_R3, precision=steps_amd.PREC_FP64, **kw)
+    sim = steps_amd.Sim(d64, N)
+    sim.set_state(w.x, w.M, w.soft, w.v)
+    wall, pair_ms, total_ms = time_path(sim, args.steps, args.warmup)
+    ms_per_step = wall * 1e3 / args.steps
+    pairs = float(N) * float(N)
+    value = pairs * args.steps / wall / 1e9
+    achieved = FLOP_PER_PAIR_R3 * pairs / (pair_ms * 1e-3) / 1e12
+    F64 = sim.get_forces()
+    sim.close()
+
+    out = {
+        "metric": "G pair-interactions/s + % vector peak at N=2^21; serving p50/p99 step latency",
+        "value": value,
+        "unit": "G pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"r3_compactified N={N} seed={gen['seed']} (open R^3, L=100 Mpc, "
+                               f"omega_max=2 atan 10, m~(1+r^2/L^2)^3, h=0.05 m^(1/3), comoving LCDM)",
+                   "N": N, "precision": "fp64", "pair_interactions_per_step": pairs},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VECTOR_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VECTOR_TFLOPS,
+                     "traffic": load_traffic(f"r3_fp64_n{N}"),
+                     "flop_per_pair": FLOP_PER_PAIR_R3, "kernel_ms": pair_ms, "eval_ms": total_ms},
+    }
+
+    if not args.no_mixed:
+        dmx, kmx = steps_amd.make_desc(steps_amd.TOPO_R3, precision=steps_amd.PREC_MIXED, **kw)
+        sim = steps_amd.Sim(dmx, N)
+        sim.set_state(w.x, w.M, w.soft, w.v)
+        wall_m, pair_m, total_m = time_path(sim, args.steps, args.warmup)
+        Fm = sim.get_forces()
+        sim.close()
+        err = np.linalg.norm(Fm - F64, axis=1) / np.linalg.norm(F64, axis=1)
+        ach_m = FLOP_PER_PAIR_R3 * pairs / (pair_m * 1e-3) / 1e12
+        out["mixed"] = {
+            "value": pairs * args.steps / wall_m / 1e9,
+            "ms_per_step": wall_m * 1e3 / args.steps,
+            "kernel_ms": pair_m,
+            "speedup_vs_fp64": (pairs / wall_m) / (pairs / wall),
+            "rel_err_vs_fp64_p50": float(np.median(err)),
+            "rel_err_vs_fp64_p99": float(np.percentile(err, 99)),
+            "roofline": {"bound": "valu", "achieved": ach_m, "peak": PEAK_FP32_VECTOR_TFLOPS,
+                         "unit": "TFLOP/s", "frac": ach_m / PEAK_FP32_VECTOR_TFLOPS},
+        }
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w)
+    out["host"] = platform.node()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

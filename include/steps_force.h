@@ -1,0 +1,154 @@
+/*
+ * steps_force.h -- C ABI of the MI355X direct-summation gravity force engine.
+ *
+ * The reference selects its force engine at link time: forces_cuda.o or
+ * forces.o exports forces()/forces_periodic()/forces_periodic_z() and
+ * recalculate_softening() with C++ linkage, reading every other input from
+ * globals (StePS/src/forces_cuda.cu:73-79, :169-173, :457-461, :868-875;
+ * StePS/src/global_variables.h:42-147).  This header is the plain C boundary
+ * underneath that: every input the reference reads from a global is a field
+ * of steps_force_desc, all buffers are plain pointers with explicit sizes, and
+ * every entry point returns a status code instead of setting ForceError or
+ * calling exit().  The C++ shim in steps_amd/csrc/shim/steps_forces_shim.cc
+ * maps the reference's globals onto these calls.
+ *
+ * Units are the reference's: Mpc, 1e11 Msun, G = 1.  Forces are accelerations
+ * before the a^-3 factor of the integrator (StePS/src/step.cc:135).
+ * Thread safety: one-shot calls serialise on an internal lock; a steps_sim is
+ * owned by one caller thread at a time; the server is thread safe.
+ */
+#ifndef STEPS_FORCE_H
+#define STEPS_FORCE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define STEPS_OK 0
+#define STEPS_ERR_ARG (-1)       /* null pointer, negative size, bad enum       */
+#define STEPS_ERR_RANGE (-2)     /* id_min/id_max outside [0, N)                 */
+#define STEPS_ERR_TABLE (-3)     /* a table the topology needs is missing        */
+#define STEPS_ERR_DEVICE (-4)    /* no HIP device, or a HIP runtime call failed  */
+#define STEPS_ERR_NOMEM (-5)     /* device or host allocation failed             */
+#define STEPS_ERR_ADMISSION (-6) /* server: request exceeds the memory budget    */
+#define STEPS_ERR_STATE (-7)     /* handle used in the wrong state               */
+#define STEPS_ERR_NONFINITE (-8) /* NaN/Inf in positions (input validation)      */
+
+/* ---- configuration ------------------------------------------------------- */
+enum steps_topology {
+    STEPS_TOPO_R3 = 0,   /* open compactified R^3 (no -DPERIODIC*)          */
+    STEPS_TOPO_T3 = 1,   /* periodic T^3 (-DPERIODIC)                       */
+    STEPS_TOPO_S1R2 = 2  /* periodic z axis, open x-y (-DPERIODIC_Z)        */
+};
+
+enum steps_precision {
+    STEPS_PREC_FP64 = 0, /* all pair arithmetic in FP64                      */
+    STEPS_PREC_MIXED = 1 /* FP32 pair arithmetic on tile-recentred positions,
+                            FP64 accumulation across tiles and corrections   */
+};
+
+/* Everything the reference force routines read from globals. */
+typedef struct steps_force_desc {
+    int32_t topology;            /* enum steps_topology                                  */
+    int32_t is_periodic;         /* IS_PERIODIC: T3 1 nearest image, >=2 Ewald;
+                                    S1R2 1 nearest z-image, >=2 Ewald / image sum          */
+    int32_t cosmology;           /* COSMOLOGY                                             */
+    int32_t comoving;            /* COMOVING_INTEGRATION                                  */
+    double L;                    /* L: T3 box size, S1R2 z period                         */
+    double Rsim;                 /* Rsim (S1R2 table extent 2.25*Rsim, radial table)      */
+    double mass_in_unit_sphere;  /* external-term constant (main.cc:1288, :1313)          */
+    double H0;                   /* code units; DE = H0^2*Omega_lambda (forces_cuda.cu:887) */
+    double Omega_lambda;
+    int32_t s1r2_lookup;         /* 1: S1R2 Ewald table (default build);
+                                    0: direct z-image sum (-DPERIODIC_Z_NOLOOKUP)          */
+    int32_t interp_order;        /* S1R2 EWALD_INTERPOLATION_ORDER: 0 NGP, 2 CIC, 4 TSC.
+                                    T3 is always tricubic Catmull-Rom (main.cc:501)       */
+    const double *t3_table;      /* T3_EWALD_FORCE_TABLE [ng^3*3], host memory            */
+    int32_t t3_ngrid;            /* N_EWALD_FORCE_GRID                                    */
+    const double *s1r2_table;    /* S1R2_EWALD_FORCE_TABLE [nrho*nz*2], host memory       */
+    int32_t s1r2_nrho, s1r2_nz;
+    const double *radial_table;  /* RADIAL_FORCE_TABLE [size], host memory                */
+    int32_t radial_table_size;
+    int32_t precision;           /* enum steps_precision                                  */
+    int32_t device;              /* HIP device ordinal                                    */
+} steps_force_desc;
+
+/* Fill *d with the defaults (R3, FP64, no cosmology, device 0). */
+void steps_force_desc_init(steps_force_desc *d);
+
+/* ---- library -------------------------------------------------------------- */
+int steps_device_count(void);
+const char *steps_last_error(void); /* thread-local text of the last failure */
+const char *steps_version(void);
+
+/* ---- one-shot force evaluation (the reference's force routine) ------------
+ * Replaces forces_cuda() (forces_cuda.cu:878-1117), forces_periodic_cuda()
+ * (:1121-1374) and forces_periodic_z_cuda() (:1378-1704) for one process and
+ * one device.  x is host AoS [3*N]; M and soft are host [N]; F is host AoS
+ * [3*(id_max-id_min+1)] and rows id_min..id_max (inclusive) are overwritten
+ * at offset 3*(i-id_min), as the GPU reference does.  Device buffers and
+ * tables are cached across calls instead of allocated per call. */
+int steps_force_eval(const steps_force_desc *d, int64_t N, const double *x, const double *M,
+                     const double *soft, double *F, int64_t id_min, int64_t id_max);
+
+/* ---- resident simulation (state kept in HBM between calls) ---------------- */
+typedef struct steps_sim steps_sim;
+
+int steps_sim_create(const steps_force_desc *d, int64_t N, steps_sim **out);
+int steps_sim_destroy(steps_sim *s);
+/* x, v AoS [3N] (v may be NULL: zero), M and soft [N]; host memory. */
+int steps_sim_set_state(steps_sim *s, const double *x, const double *v, const double *M,
+                        const double *soft);
+int steps_sim_set_positions(steps_sim *s, const double *x);
+int steps_sim_get_state(steps_sim *s, double *x, double *v);
+/* Queue one force evaluation for rows id_min..id_max on the sim's stream. */
+int steps_sim_forces(steps_sim *s, int64_t id_min, int64_t id_max);
+/* Copy the last evaluation's forces (AoS [3*(id_max-id_min+1)]) to host. */
+int steps_sim_get_forces(steps_sim *s, double *F);
+int steps_sim_synchronize(steps_sim *s);
+/* Duration of the last force evaluation's kernels, measured with HIP events
+ * on the sim's stream: pair kernel and whole evaluation (pair + reduction). */
+int steps_sim_last_kernel_ms(steps_sim *s, float *pair_ms, float *total_ms);
+
+/* One kick-drift-kick step with Hubble drag (StePS/src/step.cc:100-312),
+ * all on the device: A = G F a^-3 - 2 H v; v += A h/2, x += v h, periodic
+ * wrap (T3: x,y,z; S1R2: z), forces, A' with the updated a and H, v += A' h/2,
+ * errmax = max_i |A'_i| / cbrt(M_i 3/(4 pi rho_part)).  a_kick1/hubble1 are the
+ * scale factor and H(a) before the step, a_kick2/hubble2 after the host's
+ * Friedmann update (step.cc:233-243); G is +1.  The stored F must hold the
+ * forces of the current positions (after steps_sim_forces). */
+typedef struct steps_step_params {
+    double h;
+    double a_kick1, hubble1;
+    double a_kick2, hubble2;
+    double G;
+    double rho_part;
+} steps_step_params;
+int steps_sim_step(steps_sim *s, const steps_step_params *p, double *errmax);
+/* errmax of the stored forces without stepping (step.cc:35-98). */
+int steps_sim_errmax(steps_sim *s, double a, double hubble, double G, double rho_part,
+                     double *errmax);
+
+/* ---- tables (built on the device, returned in host memory) ----------------
+ * T3 Ewald correction, ewald_space.cc:288-383 with main.cc:422-448 params;
+ * call with table==NULL to get *ngrid, then with a [ngrid^3*3] buffer. */
+int steps_t3_table(int32_t is_periodic, double L, double *table, int32_t *ngrid);
+/* S1R2 Tornberg 1P correction, ewald_space.cc:754-798 with main.cc:569-602;
+ * table==NULL queries (*nrho, *nz). */
+int steps_s1r2_table(int32_t is_periodic, double L, double Rsim, double *table,
+                     int32_t *nrho, int32_t *nz);
+/* finite-cylinder radial force table, utils.cc:162-228 */
+int steps_radial_table(double R, double Lz, int32_t size, int32_t accuracy, double *table);
+
+/* ---- softening set-up (utils.cc:59-82) ------------------------------------ */
+int steps_softening_lengths(const double *M, int64_t N, double particle_radius, double *soft,
+                            double *M_min, double *rho_part);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,151 @@
+/*
+ * steps_oracle.h -- CPU oracle for the StePS direct-summation force path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the steps_amd library,
+ * its C ABI or the C++ drop-in shim) links, loads or calls this code.  Only
+ * tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py use it,
+ * as the checker and as the timed CPU baseline.
+ *
+ * Every function restates the reference algorithm from its source; the
+ * comment above each definition in steps_oracle.c cites the file:line it
+ * follows (paths relative to the StePS repository root).
+ *
+ * Pinning status (how each function is checked independently of itself):
+ *   force_softening        closed form + enclosed-mass integral of the
+ *                          Gadget-2 spline density (tests/test_oracle_softening.py)
+ *   forces R3              long-double brute force, Newton's third law,
+ *                          shell theorem, exact external term
+ *   t3 table + forces      antisymmetry, O_h symmetry, independent Ewald sum
+ *                          with another alpha, perfect-lattice zero force,
+ *                          Catmull-Rom node reproduction
+ *   s1r2 table + forces    direct real-space z-image sum, interpolation
+ *                          weights, node reproduction
+ *   radial table           independent adaptive quadrature of the integrand,
+ *                          long-cylinder limit
+ *   softening setup, kdk   closed forms, two-body orbit
+ *   fast_erfcx branch of the S1R2 table (ewald_space.cc:539-561): parity
+ *                          unpinned beyond the direct-image-sum bound, because
+ *                          the reference's own A&S 7.1.26 fit is only
+ *                          accurate to ~1e-7 and no reference output exists.
+ */
+#ifndef STEPS_ORACLE_H
+#define STEPS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_R3 = 0, ORC_T3 = 1, ORC_S1R2 = 2 };
+
+/* Everything the reference force routines read from globals
+ * (global_variables.h:42-147, main.cc:37-147). */
+typedef struct oracle_params {
+    int topology;              /* ORC_R3 / ORC_T3 / ORC_S1R2 (build flag PERIODIC / PERIODIC_Z) */
+    int is_periodic;           /* IS_PERIODIC */
+    int cosmology;             /* COSMOLOGY */
+    int comoving;              /* COMOVING_INTEGRATION */
+    double L;                  /* L: T3 box size, S1R2 z period */
+    double Rsim;               /* Rsim */
+    double mass_in_unit_sphere;
+    double H0;                 /* code units (km/s/Mpc / UNIT_V) */
+    double Omega_lambda;
+    int s1r2_lookup;           /* 1: Ewald table (default build); 0: PERIODIC_Z_NOLOOKUP */
+    int interp_order;          /* T3: 4 Catmull-Rom (GPU), 2 trilinear (CPU default);
+                                  S1R2: 0 NGP, 2 CIC, 4 TSC (EWALD_INTERPOLATION_ORDER) */
+    int cut_inclusive;         /* S1R2 no-lookup image cut: 1 '<=' (forces.cc:1274), 0 '<' (forces_cuda.cu:730) */
+    const double *t3_table;    /* T3_EWALD_FORCE_TABLE [Ng^3*3] */
+    int t3_ngrid;              /* N_EWALD_FORCE_GRID */
+    const double *s1r2_table;  /* S1R2_EWALD_FORCE_TABLE [Nrho*Nz*2] */
+    int s1r2_nrho, s1r2_nz;
+    const double *radial_table;/* RADIAL_FORCE_TABLE */
+    int radial_table_size;
+} oracle_params;
+
+/* softening law */
+double oracle_force_softening(double r, double beta);
+double oracle_force_softening_gpu(double r, double beta);
+
+/* direct sums: F[3*(i-id_min)+k] overwritten for rows id_min..id_max (inclusive).
+ * Deterministic: rows in parallel, j summed in order 0..N-1 per row.
+ * Returns 0, or a negative code for bad arguments. */
+int oracle_forces(const oracle_params *p, int N, const double *x, const double *M,
+                  const double *h, double *F, int id_min, int id_max);
+/* same over an explicit row list (F row k belongs to rows[k]) */
+int oracle_forces_rows(const oracle_params *p, int N, const double *x, const double *M,
+                       const double *h, double *F, const int *rows, int nrows);
+
+/* reference-faithful OpenMP baseline (schedule(dynamic,chunk), pow, three
+ * omp atomic adds per pair, -O3 -ffast-math); used only for timing. */
+int oracle_forces_faithful_rows(const oracle_params *p, int N, const double *x,
+                                const double *M, const double *h, double *F,
+                                const int *rows, int nrows);
+/* optimised CPU variant of the same mathematics (no atomics, register
+ * accumulation); context row for the CPU baseline. */
+int oracle_forces_fast_rows(const oracle_params *p, int N, const double *x,
+                            const double *M, const double *h, double *F,
+                            const int *rows, int nrows);
+int oracle_max_threads(void);
+
+/* long-double brute force, R3 only (small N) */
+void oracle_forces_r3_longdouble(const oracle_params *p, int N, const double *x,
+                                 const double *M, const double *h, long double *F);
+
+/* T3 Ewald table and interpolation */
+int  oracle_ewald_space_count(double R);
+int  oracle_ewald_space(double R, int *idx /* [count*4] */);
+void oracle_t3_ewald_correction(const double r[3], double L, double alpha,
+                                double rel_cut, double rec_cut, double D[3]);
+void oracle_t3_ewald_table(int Ngrid, double L, double alpha, double rel_cut,
+                           double rec_cut, double *table);
+void oracle_t3_params(int is_periodic, double L, double *alpha, double *rel_cut,
+                      double *rec_cut, int *ngrid);
+void oracle_t3_interpolate(int Ngrid, double L, const double *table, double dx,
+                           double dy, double dz, double D[3], int order);
+
+/* S1R2 Ewald table and interpolation */
+double oracle_fast_erfcx(double x);
+void oracle_s1r2_force_pair(double dx, double dy, double dz, double Lz, double alpha,
+                            int nmax, int mmax, int rspace, double F[3]);
+void oracle_s1r2_table(int Nrho, int Nz, double rho_max, double Lz, double alpha,
+                       int nmax, int mmax, int rspace, double *table);
+void oracle_s1r2_params(int is_periodic, double L, double Rsim, int rspace,
+                        double *alpha, int *rel_cut, int *rec_cut, int *nz, int *nrho);
+void oracle_s1r2_interpolate(const double *T, int Nrho, int Nz, double rho_max,
+                             double Lz, double dx, double dy, double dz, int order,
+                             double D[3]);
+
+/* cylindrical radial force table and its linear interpolation */
+void oracle_cylindrical_force_table(double *table, double R, double Lz, int size,
+                                    int accuracy);
+double oracle_cylindrical_force_correction(double r, double R, const double *table,
+                                           int size);
+
+/* softening set-up: h_i = cbrt(M_i*3/(4 pi rho_part)) */
+void oracle_softening_lengths(const double *M, int N, double particle_radius,
+                              double *h, double *M_min, double *rho_part);
+
+/* cosmology (COSMOPARAM 0) */
+typedef struct oracle_cosmo {
+    double H0, Omega_m, Omega_lambda, Omega_r, Omega_k;
+} oracle_cosmo;
+double oracle_friedmann_step(const oracle_cosmo *c, double a0, double h);
+double oracle_hubble(const oracle_cosmo *c, double a);
+
+/* one KDK step (rank-0 view, one process): returns errmax.  On entry F holds
+ * the force at x; on exit x, v, F, *a are advanced. */
+typedef struct oracle_step_state {
+    double a, hubble, h;          /* scale factor, H(a), step size */
+    double rho_part;
+    double G;                     /* +1 (GLASS_MAKING would be -1) */
+} oracle_step_state;
+double oracle_kdk_step(const oracle_params *p, const oracle_cosmo *c,
+                       oracle_step_state *s, int N, double *x, double *v,
+                       double *F, const double *M, const double *hsoft);
+double oracle_initial_errmax(const oracle_params *p, oracle_step_state *s, int N,
+                             double *x, const double *v, const double *F,
+                             const double *M);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

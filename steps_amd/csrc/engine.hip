@@ -1,0 +1,594 @@
+// engine.hip -- host side of the C ABI (include/steps_force.h).
+//
+// Owns device memory, streams and table copies.  Unlike the reference host
+// wrappers (forces_cuda.cu:878-1117, which cudaMalloc/copy/free six buffers and
+// the whole Ewald table on every call), device buffers are grown once and
+// reused, and tables are uploaded once per content.
+#include "internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace steps {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+int fail(int code, const std::string &msg) {
+    set_error(msg);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    set_error(m);
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? STEPS_ERR_NOMEM
+                                                                        : STEPS_ERR_DEVICE;
+}
+
+namespace {
+
+#define STEPS_HIP(call, what)                                  \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return hip_fail(e_, (what));     \
+    } while (0)
+
+int cu_count(int device) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0)
+        n = 256;
+    return n;
+}
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    int reserve(size_t bytes, const char *what) {
+        if (bytes <= cap && ptr) return STEPS_OK;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return hip_fail(e, what);
+        }
+        cap = want;
+        return STEPS_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Cheap content fingerprint of a host table (size + strided sample), so a
+// table that changes between calls is re-uploaded while an unchanged one is not.
+uint64_t fingerprint(const double *h, size_t n) {
+    uint64_t f = 1469598103934665603ull ^ (uint64_t)n;
+    const size_t step = std::max<size_t>(1, n / 4096);
+    for (size_t i = 0; i < n; i += step) {
+        uint64_t b;
+        std::memcpy(&b, &h[i], sizeof b);
+        f = (f ^ b) * 1099511628211ull;
+    }
+    if (n) {
+        uint64_t b;
+        std::memcpy(&b, &h[n - 1], sizeof b);
+        f = (f ^ b) * 1099511628211ull;
+    }
+    return f;
+}
+
+struct TableCopy {
+    const double *host = nullptr;
+    size_t n = 0;
+    uint64_t fp = 0;
+    DevBuf dev;
+    int sync(const double *h, size_t count, hipStream_t st, const char *what) {
+        if (!h || count == 0) return STEPS_OK;
+        const uint64_t f = fingerprint(h, count);
+        if (h == host && count == n && f == fp && dev.ptr) return STEPS_OK;
+        int rc = dev.reserve(count * sizeof(double), what);
+        if (rc) return rc;
+        STEPS_HIP(hipMemcpyAsync(dev.ptr, h, count * sizeof(double), hipMemcpyHostToDevice, st), what);
+        STEPS_HIP(hipStreamSynchronize(st), what);
+        host = h;
+        n = count;
+        fp = f;
+        return STEPS_OK;
+    }
+    const double *ptr() const { return dev.as<const double>(); }
+};
+
+// Build the launch plan of one evaluation and validate the descriptor.
+int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_max, EvalPlan &p) {
+    if (N < 0) return fail(STEPS_ERR_ARG, "N < 0");
+    if (N > (int64_t)INT32_MAX * 4) return fail(STEPS_ERR_ARG, "N too large");
+    if (id_max < id_min - 1) return fail(STEPS_ERR_RANGE, "id_max < id_min - 1");
+    if (id_max >= id_min && (id_min < 0 || id_max >= N))
+        return fail(STEPS_ERR_RANGE, "row range outside [0, N)");
+    if (d.precision != STEPS_PREC_FP64 && d.precision != STEPS_PREC_MIXED)
+        return fail(STEPS_ERR_ARG, "unknown precision");
+    p = EvalPlan{};
+    p.N = N;
+    p.id_min = id_min;
+    p.nrows = std::max<int64_t>(0, id_max - id_min + 1);
+    p.L = d.L;
+    p.Rsim = d.Rsim;
+    p.is_periodic = d.is_periodic;
+    p.cosmology = d.cosmology;
+    p.comoving = d.comoving;
+    p.precision = d.precision;
+    const double DE = d.H0 * d.H0 * d.Omega_lambda;  // forces_cuda.cu:887
+    switch (d.topology) {
+    case STEPS_TOPO_R3:
+        p.variant = PV_R3;
+        if (d.cosmology == 1) {
+            p.ext_mode = 1;
+            p.ext_coeff = (d.comoving == 1) ? d.mass_in_unit_sphere : DE;
+        }
+        break;
+    case STEPS_TOPO_T3:
+        if (d.is_periodic < 1) return fail(STEPS_ERR_ARG, "T3 needs IS_PERIODIC >= 1 (main.cc:405)");
+        if (!(d.L > 0)) return fail(STEPS_ERR_ARG, "T3 needs L > 0");
+        if (d.is_periodic == 1) {
+            p.variant = PV_T3_NEAREST;
+        } else {
+            if (!d.t3_table || d.t3_ngrid < 1) return fail(STEPS_ERR_TABLE, "T3 Ewald table missing");
+            p.variant = PV_T3_EWALD;
+            p.t3_ng = d.t3_ngrid;
+        }
+        break;
+    case STEPS_TOPO_S1R2: {
+        if (d.is_periodic < 1) return fail(STEPS_ERR_ARG, "S1R2 needs IS_PERIODIC >= 1 (main.cc:536)");
+        if (!(d.L > 0)) return fail(STEPS_ERR_ARG, "S1R2 needs L > 0");
+        const bool radial_used = (d.is_periodic == 1 || !d.s1r2_lookup);
+        if (d.is_periodic == 1) {
+            p.variant = PV_S1R2_NEAREST;
+        } else if (d.s1r2_lookup) {
+            if (!d.s1r2_table || d.s1r2_nrho < 1 || d.s1r2_nz < 1)
+                return fail(STEPS_ERR_TABLE, "S1R2 Ewald table missing");
+            p.variant = d.interp_order == 0 ? PV_S1R2_NGP
+                        : d.interp_order == 2 ? PV_S1R2_CIC
+                                              : PV_S1R2_TSC;  // forces_cuda.cu:449-451 fallback
+            p.s1r2_nrho = d.s1r2_nrho;
+            p.s1r2_nz = d.s1r2_nz;
+        } else {
+            p.variant = PV_S1R2_IMAGES;
+            p.images = d.is_periodic + 1;  // forces_cuda.cu:659-660
+        }
+        if (d.cosmology == 1 && d.comoving == 1) {
+            p.ext_coeff = d.mass_in_unit_sphere;
+            p.ext_mode = radial_used ? 3 : 2;
+            if (radial_used) {
+                if (!d.radial_table || d.radial_table_size < 2)
+                    return fail(STEPS_ERR_TABLE, "S1R2 radial force table missing");
+                if (!(d.Rsim > 0)) return fail(STEPS_ERR_ARG, "S1R2 needs Rsim > 0");
+                p.radial_size = d.radial_table_size;
+            }
+        } else if (d.cosmology == 1) {
+            p.ext_mode = 2;
+            p.ext_coeff = DE;
+        }
+        if (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC && !(d.Rsim > 0))
+            return fail(STEPS_ERR_ARG, "S1R2 table needs Rsim > 0");
+        break;
+    }
+    default:
+        return fail(STEPS_ERR_ARG, "unknown topology");
+    }
+    if (p.precision == STEPS_PREC_MIXED && p.variant != PV_R3)
+        return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3 only");
+
+    // split the source range so that (row blocks) x (splits) fills the chip
+    const int ni = (p.variant == PV_T3_EWALD || (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC))
+                       ? 1
+                       : (p.precision == STEPS_PREC_MIXED ? kRowsPerLaneMixed : kRowsPerLane);
+    const int64_t rows_per_block = (int64_t)kBlock * ni;
+    const int64_t nib = std::max<int64_t>(1, (p.nrows + rows_per_block - 1) / rows_per_block);
+    const int64_t target = (int64_t)cu_count(d.device) * 8;
+    int64_t splits = (target + nib - 1) / nib;
+    splits = std::max<int64_t>(1, std::min<int64_t>(splits, std::max<int64_t>(1, N / 256)));
+    int64_t jchunk = std::max<int64_t>(1, (N + splits - 1) / splits);
+    if (p.precision == STEPS_PREC_MIXED)
+        jchunk = ((jchunk + kMixedTileRows - 1) / kMixedTileRows) * kMixedTileRows;
+    splits = std::max<int64_t>(1, (N + jchunk - 1) / jchunk);
+    p.splits = (int)splits;
+    p.jchunk = jchunk;
+    return STEPS_OK;
+}
+
+size_t scratch_bytes(const EvalPlan &p) {
+    size_t b = (partial_bytes(p) + 255) & ~(size_t)255;
+    if (p.precision == STEPS_PREC_MIXED) b += mixed_extra_bytes(p.N) + 256;
+    return b;
+}
+
+// State of the one-shot entry point (one device at a time, under a lock).
+struct OneShot {
+    std::mutex mu;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    DevBuf x, M, soft, jrec, jsoft, part, F;
+    TableCopy t3, s1r2, radial;
+};
+OneShot g_oneshot;
+
+int set_device(int device) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return fail(STEPS_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(STEPS_ERR_DEVICE, "device ordinal out of range");
+    STEPS_HIP(hipSetDevice(device), "hipSetDevice");
+    return STEPS_OK;
+}
+
+int check_finite(const double *x, int64_t n) {
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(x[i])) return fail(STEPS_ERR_NONFINITE, "non-finite input value");
+    return STEPS_OK;
+}
+
+}  // namespace
+}  // namespace steps
+
+// ---------------------------------------------------------------------------
+// resident simulation handle
+// ---------------------------------------------------------------------------
+struct steps_sim {
+    int device = 0;
+    steps_force_desc desc{};
+    int64_t N = 0;
+    steps::DeviceParticles dp;
+    steps::DevBuf jrec, jsoft, v, F, part, stage, blockmax;
+    steps::TableCopy t3, s1r2, radial;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    int64_t last_nrows = 0;
+    bool evaluated = false;
+};
+
+using namespace steps;
+
+extern "C" {
+
+void steps_force_desc_init(steps_force_desc *d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof *d);
+    d->topology = STEPS_TOPO_R3;
+    d->interp_order = 2;
+    d->s1r2_lookup = 1;
+    d->precision = STEPS_PREC_FP64;
+}
+
+int steps_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char *steps_last_error(void) { return g_error.c_str(); }
+
+const char *steps_version(void) { return "steps_amd 0.1 (gfx950)"; }
+
+int steps_force_eval(const steps_force_desc *d, int64_t N, const double *x, const double *M,
+                     const double *soft, double *F, int64_t id_min, int64_t id_max) {
+    if (!d) return fail(STEPS_ERR_ARG, "null descriptor");
+    EvalPlan p;
+    int rc = make_plan(*d, N, id_min, id_max, p);
+    if (rc) return rc;
+    if (p.nrows == 0) return STEPS_OK;
+    if (!x || !M || !soft || !F) return fail(STEPS_ERR_ARG, "null buffer");
+    if ((rc = check_finite(x, 3 * N))) return rc;
+    std::lock_guard<std::mutex> lock(g_oneshot.mu);
+    OneShot &o = g_oneshot;
+    if ((rc = set_device(d->device))) return rc;
+    if (o.device != d->device) {
+        // buffers belong to the previous device: drop them
+        o.x.release(); o.M.release(); o.soft.release(); o.jrec.release(); o.jsoft.release();
+        o.part.release(); o.F.release();
+        o.t3 = TableCopy{}; o.s1r2 = TableCopy{}; o.radial = TableCopy{};
+        if (o.stream) (void)hipStreamDestroy(o.stream);
+        o.stream = nullptr;
+        STEPS_HIP(hipStreamCreateWithFlags(&o.stream, hipStreamNonBlocking), "hipStreamCreate");
+        o.device = d->device;
+    }
+    const size_t nb = (size_t)N * sizeof(double);
+    if ((rc = o.x.reserve(3 * nb, "alloc x")) || (rc = o.M.reserve(nb, "alloc M")) ||
+        (rc = o.soft.reserve(nb, "alloc soft")) || (rc = o.jrec.reserve((size_t)N * sizeof(double4), "alloc jrec")) ||
+        (rc = o.jsoft.reserve(nb, "alloc jsoft")) || (rc = o.part.reserve(scratch_bytes(p), "alloc partials")) ||
+        (rc = o.F.reserve(3 * (size_t)p.nrows * sizeof(double), "alloc F")))
+        return rc;
+    hipStream_t st = o.stream;
+    if (p.variant == PV_T3_EWALD &&
+        (rc = o.t3.sync(d->t3_table, (size_t)d->t3_ngrid * d->t3_ngrid * d->t3_ngrid * 3, st, "T3 table")))
+        return rc;
+    if (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC &&
+        (rc = o.s1r2.sync(d->s1r2_table, (size_t)d->s1r2_nrho * d->s1r2_nz * 2, st, "S1R2 table")))
+        return rc;
+    if (p.ext_mode == 3 && (rc = o.radial.sync(d->radial_table, (size_t)d->radial_table_size, st, "radial table")))
+        return rc;
+    p.t3 = o.t3.ptr();
+    p.s1r2 = o.s1r2.ptr();
+    p.radial = o.radial.ptr();
+    STEPS_HIP(hipMemcpyAsync(o.x.ptr, x, 3 * nb, hipMemcpyHostToDevice, st), "copy x");
+    STEPS_HIP(hipMemcpyAsync(o.M.ptr, M, nb, hipMemcpyHostToDevice, st), "copy M");
+    STEPS_HIP(hipMemcpyAsync(o.soft.ptr, soft, nb, hipMemcpyHostToDevice, st), "copy soft");
+    DeviceParticles dp{o.jrec.as<double4>(), o.jsoft.as<double>(), N};
+    STEPS_HIP(launch_pack(o.x.as<double>(), o.M.as<double>(), o.soft.as<double>(), N, dp, st), "pack");
+    STEPS_HIP(launch_forces(p, dp, o.part.as<double>(), o.F.as<double>(), st, nullptr, nullptr, nullptr),
+              "force kernels");
+    STEPS_HIP(hipMemcpyAsync(F, o.F.ptr, 3 * (size_t)p.nrows * sizeof(double), hipMemcpyDeviceToHost, st),
+              "copy F");
+    STEPS_HIP(hipStreamSynchronize(st), "force evaluation");
+    return STEPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+
+int steps_sim_create(const steps_force_desc *d, int64_t N, steps_sim **out) {
+    if (!d || !out) return fail(STEPS_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (N <= 0) return fail(STEPS_ERR_ARG, "N must be positive");
+    EvalPlan p;
+    int rc = make_plan(*d, N, 0, N - 1, p);
+    if (rc) return rc;
+    if ((rc = set_device(d->device))) return rc;
+    steps_sim *s = new (std::nothrow) steps_sim();
+    if (!s) return fail(STEPS_ERR_NOMEM, "host allocation");
+    s->device = d->device;
+    s->desc = *d;
+    s->N = N;
+    const size_t nb = (size_t)N * sizeof(double);
+    auto cleanup = [&](int code) {
+        steps_sim_destroy(s);
+        return code;
+    };
+    if ((rc = s->jrec.reserve((size_t)N * sizeof(double4), "alloc jrec")) ||
+        (rc = s->jsoft.reserve(nb, "alloc soft")) || (rc = s->v.reserve(3 * nb, "alloc v")) ||
+        (rc = s->F.reserve(3 * nb, "alloc F")) || (rc = s->part.reserve(scratch_bytes(p), "alloc partials")) ||
+        (rc = s->stage.reserve(std::max(3 * nb, (size_t)1024), "alloc stage")) ||
+        (rc = s->blockmax.reserve((size_t)(kdk_reduce_blocks(N) + 1) * sizeof(double), "alloc blockmax")))
+        return cleanup(rc);
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess ||
+        (e = hipEventCreate(&s->ev2)) != hipSuccess)
+        return cleanup(hip_fail(e, "stream/event create"));
+    if (p.variant == PV_T3_EWALD &&
+        (rc = s->t3.sync(d->t3_table, (size_t)d->t3_ngrid * d->t3_ngrid * d->t3_ngrid * 3, s->stream, "T3 table")))
+        return cleanup(rc);
+    if (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC &&
+        (rc = s->s1r2.sync(d->s1r2_table, (size_t)d->s1r2_nrho * d->s1r2_nz * 2, s->stream, "S1R2 table")))
+        return cleanup(rc);
+    if (p.ext_mode == 3 &&
+        (rc = s->radial.sync(d->radial_table, (size_t)d->radial_table_size, s->stream, "radial table")))
+        return cleanup(rc);
+    // the sim owns device copies: host table pointers are not kept
+    s->desc.t3_table = nullptr;
+    s->desc.s1r2_table = nullptr;
+    s->desc.radial_table = nullptr;
+    s->dp = DeviceParticles{s->jrec.as<double4>(), s->jsoft.as<double>(), N};
+    if ((e = hipMemsetAsync(s->v.ptr, 0, 3 * nb, s->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(s->F.ptr, 0, 3 * nb, s->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(s->stream)) != hipSuccess)
+        return cleanup(hip_fail(e, "initialise sim"));
+    *out = s;
+    return STEPS_OK;
+}
+
+int steps_sim_destroy(steps_sim *s) {
+    if (!s) return STEPS_OK;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->jrec.release(); s->jsoft.release(); s->v.release(); s->F.release(); s->part.release();
+    s->stage.release(); s->blockmax.release();
+    s->t3.dev.release(); s->s1r2.dev.release(); s->radial.dev.release();
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->ev2) (void)hipEventDestroy(s->ev2);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return STEPS_OK;
+}
+
+int steps_sim_set_state(steps_sim *s, const double *x, const double *v, const double *M,
+                        const double *soft) {
+    if (!s || !x || !M || !soft) return fail(STEPS_ERR_ARG, "null argument");
+    int rc = check_finite(x, 3 * s->N);
+    if (rc) return rc;
+    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    const size_t nb = (size_t)s->N * sizeof(double);
+    // stage: x (3N) then M, soft in the F buffer's space (F is recomputed anyway)
+    double *xs = s->stage.as<double>();
+    double *ms = s->F.as<double>();
+    double *hs = ms + s->N;
+    STEPS_HIP(hipMemcpyAsync(xs, x, 3 * nb, hipMemcpyHostToDevice, s->stream), "copy x");
+    STEPS_HIP(hipMemcpyAsync(ms, M, nb, hipMemcpyHostToDevice, s->stream), "copy M");
+    STEPS_HIP(hipMemcpyAsync(hs, soft, nb, hipMemcpyHostToDevice, s->stream), "copy soft");
+    STEPS_HIP(launch_pack(xs, ms, hs, s->N, s->dp, s->stream), "pack");
+    if (v) STEPS_HIP(hipMemcpyAsync(s->v.ptr, v, 3 * nb, hipMemcpyHostToDevice, s->stream), "copy v");
+    else STEPS_HIP(hipMemsetAsync(s->v.ptr, 0, 3 * nb, s->stream), "zero v");
+    STEPS_HIP(hipMemsetAsync(s->F.ptr, 0, 3 * nb, s->stream), "zero F");
+    STEPS_HIP(hipStreamSynchronize(s->stream), "set state");
+    s->evaluated = false;
+    return STEPS_OK;
+}
+
+int steps_sim_set_positions(steps_sim *s, const double *x) {
+    if (!s || !x) return fail(STEPS_ERR_ARG, "null argument");
+    int rc = check_finite(x, 3 * s->N);
+    if (rc) return rc;
+    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    const size_t nb = (size_t)s->N * sizeof(double);
+    STEPS_HIP(hipMemcpyAsync(s->stage.ptr, x, 3 * nb, hipMemcpyHostToDevice, s->stream), "copy x");
+    STEPS_HIP(launch_pack_positions(s->stage.as<double>(), s->N, s->dp, s->stream), "pack positions");
+    STEPS_HIP(hipStreamSynchronize(s->stream), "set positions");
+    s->evaluated = false;
+    return STEPS_OK;
+}
+
+int steps_sim_get_state(steps_sim *s, double *x, double *v) {
+    if (!s) return fail(STEPS_ERR_ARG, "null sim");
+    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    const size_t nb = (size_t)s->N * sizeof(double);
+    if (x) {
+        STEPS_HIP(launch_unpack_positions(s->dp, s->stage.as<double>(), s->stream), "unpack");
+        STEPS_HIP(hipMemcpyAsync(x, s->stage.ptr, 3 * nb, hipMemcpyDeviceToHost, s->stream), "copy x");
+    }
+    if (v) STEPS_HIP(hipMemcpyAsync(v, s->v.ptr, 3 * nb, hipMemcpyDeviceToHost, s->stream), "copy v");
+    STEPS_HIP(hipStreamSynchronize(s->stream), "get state");
+    return STEPS_OK;
+}
+
+static int sim_plan(steps_sim *s, int64_t id_min, int64_t id_max, EvalPlan &p) {
+    steps_force_desc d = s->desc;  // tables live on the device; the checks only need non-null
+    d.t3_table = s->t3.ptr();
+    d.s1r2_table = s->s1r2.ptr();
+    d.radial_table = s->radial.ptr();
+    int rc = make_plan(d, s->N, id_min, id_max, p);
+    if (rc) return rc;
+    p.t3 = s->t3.ptr();
+    p.s1r2 = s->s1r2.ptr();
+    p.radial = s->radial.ptr();
+    return STEPS_OK;
+}
+
+// Second kick (optional) and errmax = max_i |A_i| / cbrt(M_i*3/(4 pi rho_part))
+// on the device (step.cc:256-269, :70-80); one double comes back to the host.
+static int sim_kick_errmax(steps_sim *s, double a, double hubble, double G, double h,
+                           double rho_part, double *errmax, bool kick) {
+    if (!(rho_part > 0)) return fail(STEPS_ERR_ARG, "rho_part must be positive");
+    const double const_beta = 3.0 / rho_part / (4.0 * kPi);
+    const int nb = kdk_reduce_blocks(s->N);
+    double *bm = s->blockmax.as<double>();
+    STEPS_HIP(launch_kick_errmax(s->dp, s->v.as<double>(), s->F.as<double>(), G, std::pow(a, -3.0), hubble,
+                                 h, const_beta, bm, nb, bm + nb, s->stream, kick),
+              "kick/errmax");
+    double e = 0;
+    STEPS_HIP(hipMemcpyAsync(&e, bm + nb, sizeof(double), hipMemcpyDeviceToHost, s->stream), "copy errmax");
+    STEPS_HIP(hipStreamSynchronize(s->stream), "step");
+    if (errmax) *errmax = e;
+    return STEPS_OK;
+}
+
+int steps_sim_forces(steps_sim *s, int64_t id_min, int64_t id_max) {
+    if (!s) return fail(STEPS_ERR_ARG, "null sim");
+    EvalPlan p;
+    int rc = sim_plan(s, id_min, id_max, p);
+    if (rc) return rc;
+    if ((size_t)scratch_bytes(p) > s->part.cap) return fail(STEPS_ERR_STATE, "scratch too small");
+    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    STEPS_HIP(launch_forces(p, s->dp, s->part.as<double>(), s->F.as<double>(), s->stream, s->ev0, s->ev1,
+                            s->ev2),
+              "force kernels");
+    s->last_nrows = p.nrows;
+    s->evaluated = true;
+    return STEPS_OK;
+}
+
+int steps_sim_get_forces(steps_sim *s, double *F) {
+    if (!s || !F) return fail(STEPS_ERR_ARG, "null argument");
+    if (!s->evaluated) return fail(STEPS_ERR_STATE, "no force evaluation queued");
+    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    STEPS_HIP(hipMemcpyAsync(F, s->F.ptr, 3 * (size_t)s->last_nrows * sizeof(double), hipMemcpyDeviceToHost,
+                             s->stream),
+              "copy F");
+    STEPS_HIP(hipStreamSynchronize(s->stream), "get forces");
+    return STEPS_OK;
+}
+
+int steps_sim_synchronize(steps_sim *s) {
+    if (!s) return fail(STEPS_ERR_ARG, "null sim");
+    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    STEPS_HIP(hipStreamSynchronize(s->stream), "synchronize");
+    return STEPS_OK;
+}
+
+int steps_sim_last_kernel_ms(steps_sim *s, float *pair_ms, float *total_ms) {
+    if (!s) return fail(STEPS_ERR_ARG, "null sim");
+    if (!s->evaluated) return fail(STEPS_ERR_STATE, "no force evaluation queued");
+    STEPS_HIP(hipEventSynchronize(s->ev2), "event sync");
+    float a = 0, b = 0;
+    STEPS_HIP(hipEventElapsedTime(&a, s->ev0, s->ev1), "elapsed");
+    STEPS_HIP(hipEventElapsedTime(&b, s->ev0, s->ev2), "elapsed");
+    if (pair_ms) *pair_ms = a;
+    if (total_ms) *total_ms = b;
+    return STEPS_OK;
+}
+
+int steps_sim_step(steps_sim *s, const steps_step_params *sp, double *errmax) {
+    if (!s || !sp) return fail(STEPS_ERR_ARG, "null argument");
+    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    EvalPlan p;
+    int rc = sim_plan(s, 0, s->N - 1, p);
+    if (rc) return rc;
+    const double ainv3a = std::pow(sp->a_kick1, -3.0);
+    STEPS_HIP(launch_kick_drift(s->dp, s->v.as<double>(), s->F.as<double>(), sp->G, ainv3a, sp->hubble1, sp->h,
+                                s->desc.topology, s->desc.L, s->stream),
+              "kick-drift");
+    STEPS_HIP(launch_forces(p, s->dp, s->part.as<double>(), s->F.as<double>(), s->stream, s->ev0, s->ev1, s->ev2),
+              "force kernels");
+    s->last_nrows = p.nrows;
+    s->evaluated = true;
+    return sim_kick_errmax(s, sp->a_kick2, sp->hubble2, sp->G, sp->h, sp->rho_part, errmax, true);
+}
+
+int steps_sim_errmax(steps_sim *s, double a, double hubble, double G, double rho_part, double *errmax) {
+    if (!s) return fail(STEPS_ERR_ARG, "null sim");
+    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    return sim_kick_errmax(s, a, hubble, G, 0.0, rho_part, errmax, false);
+}
+
+int steps_softening_lengths(const double *M, int64_t N, double particle_radius, double *soft,
+                            double *M_min, double *rho_part) {
+    // utils.cc:59-82
+    if (!M || !soft || N <= 0) return fail(STEPS_ERR_ARG, "bad softening arguments");
+    if (!(particle_radius > 0)) return fail(STEPS_ERR_ARG, "particle radius must be positive");
+    double mmin = M[0];
+    for (int64_t i = 0; i < N; ++i)
+        if (mmin > M[i]) mmin = M[i];
+    const double rp = mmin / (4.0 * kPi * std::pow(particle_radius, 3.0) / 3.0);
+    const double cb = 3.0 / rp / (4.0 * kPi);
+    for (int64_t i = 0; i < N; ++i) soft[i] = std::cbrt(M[i] * cb);
+    if (M_min) *M_min = mmin;
+    if (rho_part) *rho_part = rp;
+    return STEPS_OK;
+}
+
+int steps_t3_table(int32_t is_periodic, double L, double *table, int32_t *ngrid) {
+    int ng = 0;
+    int rc = build_t3_table(is_periodic, L, table, &ng);
+    if (ngrid) *ngrid = ng;
+    return rc;
+}
+
+int steps_s1r2_table(int32_t is_periodic, double L, double Rsim, double *table, int32_t *nrho,
+                     int32_t *nz) {
+    int a = 0, b = 0;
+    int rc = build_s1r2_table(is_periodic, L, Rsim, table, &a, &b);
+    if (nrho) *nrho = a;
+    if (nz) *nz = b;
+    return rc;
+}
+
+int steps_radial_table(double R, double Lz, int32_t size, int32_t accuracy, double *table) {
+    return build_radial_table(R, Lz, size, accuracy, table);
+}
+
+}  // extern "C"

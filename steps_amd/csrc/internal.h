@@ -1,0 +1,105 @@
+// internal.h -- declarations shared by the steps_amd HIP translation units.
+// Not part of the public ABI (that is include/steps_force.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "steps_force.h"
+
+namespace steps {
+
+constexpr double kPi = 3.14159265358979323846264338327950288419716939937510;  // global_variables.h:16
+constexpr double kRadialExtent = 2.25;                                          // global_variables.h:39
+constexpr int kBlock = 256;      // threads per workgroup of the pair kernels (4 wave64)
+constexpr int kRowsPerLane = 2;  // i-particles held in registers per lane (FP64 path)
+constexpr int kRowsPerLaneMixed = 2;
+
+// Pair-term variants: one kernel instantiation each.
+enum PairVariant : int {
+    PV_R3 = 0,            // forces_cuda.cu:522-563
+    PV_T3_NEAREST = 1,    // forces_cuda.cu:576-608   (IS_PERIODIC == 1)
+    PV_T3_EWALD = 2,      // forces_cuda.cu:609-643   (IS_PERIODIC >= 2)
+    PV_S1R2_NEAREST = 3,  // forces_cuda.cu:772-818   (IS_PERIODIC == 1)
+    PV_S1R2_NGP = 4,      // forces_cuda.cu:819-863, EWALD_INTERPOLATION_ORDER 0
+    PV_S1R2_CIC = 5,      //   ... order 2
+    PV_S1R2_TSC = 6,      //   ... order 4
+    PV_S1R2_IMAGES = 7,   // forces_cuda.cu:711-759   (PERIODIC_Z_NOLOOKUP, IS_PERIODIC >= 2)
+};
+
+// Device-resident particle set in the layout the pair kernels read:
+// jrec[i] = (x, y, z, m) for wave-uniform scalar loads of the j-particle,
+// soft[i] = h.  xoff holds FP32 tile-recentred positions for the mixed path.
+struct DeviceParticles {
+    double4 *jrec = nullptr;
+    double *soft = nullptr;
+    int64_t n = 0;
+};
+
+// Everything one force evaluation needs on the device.
+struct EvalPlan {
+    int variant = PV_R3;
+    int precision = STEPS_PREC_FP64;
+    int64_t N = 0;
+    int64_t id_min = 0, nrows = 0;
+    int splits = 1;        // j-range splits (partials reduced in fixed order)
+    int64_t jchunk = 0;    // j per split
+    // topology scalars
+    double L = 0, Rsim = 0;
+    int is_periodic = 0;
+    int cosmology = 0, comoving = 0;
+    double ext_coeff = 0;  // mass_in_unit_sphere (comoving) or DE (non-comoving)
+    int ext_mode = 0;      // 0 none, 1 xyz, 2 xy, 3 xy * radial correction
+    // tables (device)
+    const double *t3 = nullptr; int t3_ng = 0;
+    const double *s1r2 = nullptr; int s1r2_nrho = 0, s1r2_nz = 0;
+    const double *radial = nullptr; int radial_size = 0;
+    int images = 0;        // S1R2 image-sum: ewald_max
+};
+
+// Scratch of one evaluation: partial sums [splits][3][nrows] (FP64), followed
+// (256-byte aligned) by the mixed path's FP32 tile offsets when precision is
+// STEPS_PREC_MIXED.
+size_t partial_bytes(const EvalPlan &p);
+size_t mixed_extra_bytes(int64_t N);
+constexpr int kMixedTileRows = 64;  // sources per recentred FP32 tile
+
+// Launch pair kernels + fixed-order reduction; F is device AoS [3*nrows].
+// If ev_pair_begin/ev_pair_end/ev_end are non-null they are recorded around
+// the pair kernel and after the reduction (on `stream`).
+hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *partials,
+                         double *F, hipStream_t stream, hipEvent_t ev_begin,
+                         hipEvent_t ev_pair_end, hipEvent_t ev_end);
+
+// AoS x [3n] + M [n] + soft [n] (device) -> DeviceParticles layout.
+hipError_t launch_pack(const double *x, const double *M, const double *soft, int64_t n,
+                       DeviceParticles &dp, hipStream_t stream);
+// Replace positions only (AoS x [3n], device) keeping masses.
+hipError_t launch_pack_positions(const double *x, int64_t n, DeviceParticles &dp,
+                                 hipStream_t stream);
+// DeviceParticles positions -> AoS x [3n] (device).
+hipError_t launch_unpack_positions(const DeviceParticles &dp, double *x, hipStream_t stream);
+
+// KDK pieces (kdk_kernels.hip)
+hipError_t launch_kick_drift(DeviceParticles &dp, double *v, const double *F, double G,
+                             double ainv3, double hubble, double h, int topology, double L,
+                             hipStream_t stream);
+hipError_t launch_kick_errmax(const DeviceParticles &dp, double *v, const double *F, double G,
+                              double ainv3, double hubble, double h, double const_beta,
+                              double *block_max, int nblocks_max, double *errmax_dev,
+                              hipStream_t stream, bool kick);
+int kdk_reduce_blocks(int64_t n);
+
+// Table builders (table_kernels.hip); all buffers host memory.
+int build_t3_table(int is_periodic, double L, double *table, int *ngrid);
+int build_s1r2_table(int is_periodic, double L, double Rsim, double *table, int *nrho, int *nz);
+int build_radial_table(double R, double Lz, int size, int accuracy, double *table);
+
+// error text (engine.hip)
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+}  // namespace steps

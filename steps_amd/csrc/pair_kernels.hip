@@ -1,0 +1,681 @@
+// pair_kernels.hip -- direct O(N^2) pair-force kernels for gfx950 (MI355X).
+//
+// Work decomposition (one force evaluation = rows id_min..id_max against all
+// N sources, self pair included as in forces_cuda.cu:531 and forces.cc:538):
+//   * a workgroup of 256 lanes (4 wave64) owns kBlockRows = 256*NI target rows,
+//     NI rows per lane held in VGPRs together with their FP64 accumulators;
+//   * blockIdx.y selects one of `splits` contiguous source ranges, so small N
+//     still fills 256 CUs; every (row-block, split) writes its FP64 partial
+//     sums to partials[split][3][nrows] and a second kernel adds the splits in
+//     fixed order (deterministic, no atomics) and adds the external term;
+//   * the source particle j is wave-uniform, so its record (x,y,z,m) and h are
+//     fetched with scalar loads into SGPRs and reach all 64 lanes as VALU
+//     scalar operands -- the broadcast costs no VALU or LDS instruction;
+//   * 1/r^3 uses v_rsq_f64 (2^-24 accurate on gfx950) plus one second-order
+//     Newton correction; the cubic-spline softening branch (r < h_i + h_j,
+//     forces_cuda.cu:466-519) is evaluated exactly, with the reference's own
+//     formula, only when some lane of the wave needs it.
+#include "internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace steps {
+
+namespace {
+
+struct KScalars {
+    int64_t N;
+    int64_t id_min;
+    int64_t nrows;
+    int64_t jchunk;
+    double L, halfL;
+    // T3 table
+    int ng;
+    // S1R2 table
+    int nrho, nz;
+    double rho_max;
+    // S1R2 image sum
+    int images;
+    double image_cut;
+    // Newton-correction constants, passed in so that they live in registers
+    // for the whole loop instead of being rematerialised per pair
+    double c15, c1875;
+};
+
+// ---------------------------------------------------------------------------
+// FP64 building blocks
+// ---------------------------------------------------------------------------
+
+// r2^(-3/2): v_rsq_f64 seed y (relative error e0 <= 2^-24), e = 1 - r2*y^2,
+// y^3 * (1 - e)^(-3/2) ~= y^3 * (1 + 1.5 e + 1.875 e^2); truncation error
+// O(e^3) ~ 1e-21.  r2 == 0 gives NaN: the caller's softening branch owns it.
+__device__ __forceinline__ double inv_r3(double r2, double c15, double c1875) {
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double t = y * y;
+    const double e = fma(-r2, t, 1.0);
+    const double y3 = t * y;
+    const double c = fma(e, c1875, c15);
+    return fma(y3 * e, c, y3);
+}
+
+// forces_cuda.cu:466-519, written out as the reference does (beta powers and
+// divisions); only reached for r < beta, so its cost does not matter.
+__device__ __forceinline__ double softening_exact(double r, double beta) {
+    const double half_beta = beta * 0.5;
+    const double r2 = r * r;
+    const double r3 = r2 * r;
+    const double inv_r3v = 1.0 / r3;
+    if (r >= beta) return inv_r3v;
+    const double b2 = beta * beta, b3 = b2 * beta, b4 = b2 * b2, b5 = b4 * beta, b6 = b3 * b3;
+    if (r > half_beta) {
+        const double c0 = -32.0 / (3.0 * b6), c1 = 38.4 / b5, c2 = -48.0 / b4;
+        const double c3 = 64.0 / (3.0 * b3), c4 = -1.0 / 15.0;
+        return c0 * r3 + c1 * r2 + c2 * r + c3 + c4 * inv_r3v;
+    }
+    const double c0 = 32.0 / b6, c1 = -38.4 / b5, c2 = (32.0 / 3.0) / b3;
+    return c0 * r3 + c1 * r2 + c2;
+}
+
+// Softened coefficient w(r, beta) for a pair with squared distance r2.
+__device__ __forceinline__ double pair_w(double r2, double beta, const KScalars &k) {
+    double w = inv_r3(r2, k.c15, k.c1875);
+    if (r2 < beta * beta) w = softening_exact(sqrt(r2), beta);
+    return w;
+}
+
+__device__ __forceinline__ int imodp(int i, int n) {
+    int r = i % n;  // forces_cuda.cu:87-92
+    return (r < 0) ? (r + n) : r;
+}
+
+// forces_cuda.cu:93-167: tricubic Catmull-Rom interpolation of the T3 table
+// (cell-centred grid, periodic index wrap), table layout [(ix*N+iy)*N+iz]*3+c.
+__device__ __forceinline__ void t3_interp(const double *__restrict__ T, int Ng, double L,
+                                          double dx, double dy, double dz, double &Dx,
+                                          double &Dy, double &Dz) {
+    const double gs = L / (double)Ng;
+    int ix[4], iy[4], iz[4];
+    double wx[4], wy[4], wz[4];
+    const double r3v[3] = {dx, dy, dz};
+    int *idx[3] = {ix, iy, iz};
+    double *wt[3] = {wx, wy, wz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double u = (r3v[a] + L * 0.5) / gs - 0.5;
+        const double uf = floor(u);
+        const int i0 = imodp((int)uf, Ng);
+        const double t = u - uf;
+        const double t2 = t * t, t3 = t * t2;
+        wt[a][0] = -0.5 * t3 + t2 - 0.5 * t;
+        wt[a][1] = 1.5 * t3 - 2.5 * t2 + 1.0;
+        wt[a][2] = -1.5 * t3 + 2.0 * t2 + 0.5 * t;
+        wt[a][3] = 0.5 * t3 - 0.5 * t2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) idx[a][q] = imodp(i0 - 1 + q, Ng);
+    }
+    double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double wxy = wx[i] * wy[j];
+            const double *row = T + (size_t)((ix[i] * Ng + iy[j]) * Ng) * 3u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double w = wxy * wz[k];
+                const double *p = row + (size_t)iz[k] * 3u;
+                s0 += w * p[0];
+                s1 += w * p[1];
+                s2 += w * p[2];
+            }
+        }
+    }
+    Dx = s0;
+    Dy = s1;
+    Dz = s2;
+}
+
+__device__ __forceinline__ double s1r2_at(const double *__restrict__ T, int Nz, int ir, int iz,
+                                          int c) {
+    return T[((size_t)ir * (size_t)Nz + (size_t)iz) * 2u + (size_t)c];
+}
+
+// forces_cuda.cu:188-455: NGP / CIC / TSC interpolation of (D_rho, D_z) at
+// (rho, z), rho clamped to [0, rho_max], z periodic; back to Cartesian.
+template <int ORDER>
+__device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, int Nrho, int Nz,
+                                            double rho_max, double Lz, double dx, double dy,
+                                            double dz, double &Dx, double &Dy, double &Dz) {
+    const double rho = sqrt(dx * dx + dy * dy);
+    const double half = 0.5 * Lz;
+    const double drho = rho_max / (double)max(1, Nrho - 1);
+    const double dzg = Lz / (double)Nz;
+    double Dr, Dzv;
+    if (ORDER == 0) {
+        double rc = rho;
+        if (rho < 0.0) rc = 0.0;
+        if (rho > rho_max) rc = rho_max;
+        const double ur = (drho > 0.0) ? (rc / drho) : 0.0;
+        const double uz = (dz + half) / dzg - 0.5;
+        int ir = (int)floor(ur + 0.5);
+        int iz = (int)floor(uz + 0.5);
+        ir = min(max(ir, 0), Nrho - 1);
+        iz = imodp(iz, Nz);
+        Dr = s1r2_at(T, Nz, ir, iz, 0);
+        Dzv = s1r2_at(T, Nz, ir, iz, 1);
+    } else if (ORDER == 2) {
+        double ur;
+        if (rho < 0.0) ur = 0.0;
+        else if (rho > rho_max) ur = (double)(Nrho - 1);
+        else ur = rho / drho;
+        int ir0 = (int)floor(ur);
+        double fr = ur - (double)ir0;
+        if (ir0 < 0) { ir0 = 0; fr = 0.0; }
+        if (ir0 > Nrho - 2) { ir0 = max(0, Nrho - 2); fr = 1.0; }
+        const int ir1 = ir0 + 1;
+        const double uz = (dz + half) / dzg - 0.5;
+        int iz0 = (int)floor(uz);
+        const double fz = uz - (double)iz0;
+        iz0 = imodp(iz0, Nz);
+        const int iz1 = imodp(iz0 + 1, Nz);
+        double w00 = 1.0 - fr; w00 *= (1.0 - fz);
+        double w10 = fr; w10 *= (1.0 - fz);
+        double w01 = 1.0 - fr; w01 *= fz;
+        const double w11 = fr * fz;
+        Dr = w00 * s1r2_at(T, Nz, ir0, iz0, 0) + w10 * s1r2_at(T, Nz, ir1, iz0, 0) +
+             w01 * s1r2_at(T, Nz, ir0, iz1, 0) + w11 * s1r2_at(T, Nz, ir1, iz1, 0);
+        Dzv = w00 * s1r2_at(T, Nz, ir0, iz0, 1) + w10 * s1r2_at(T, Nz, ir1, iz0, 1) +
+              w01 * s1r2_at(T, Nz, ir0, iz1, 1) + w11 * s1r2_at(T, Nz, ir1, iz1, 1);
+    } else {
+        double ur;
+        if (rho < 0.0) ur = 0.0;
+        else if (rho > rho_max) ur = (double)(Nrho - 1);
+        else ur = rho / drho;
+        const double uz = (dz + half) / dzg - 0.5;
+        const int jr = (int)floor(ur + 0.5), jz = (int)floor(uz + 0.5);
+        const double sr = ur - (double)jr, sz = uz - (double)jz;
+        const double wr[3] = {0.5 * (0.5 - sr) * (0.5 - sr), 0.75 - sr * sr,
+                              0.5 * (0.5 + sr) * (0.5 + sr)};
+        const double wz[3] = {0.5 * (0.5 - sz) * (0.5 - sz), 0.75 - sz * sz,
+                              0.5 * (0.5 + sz) * (0.5 + sz)};
+        const int irs[3] = {max(jr - 1, 0), min(max(jr, 0), Nrho - 1), min(jr + 1, Nrho - 1)};
+        const int izs[3] = {imodp(jz - 1, Nz), imodp(jz, Nz), imodp(jz + 1, Nz)};
+        Dr = 0.0;
+        Dzv = 0.0;
+#pragma unroll
+        for (int row = 0; row < 3; ++row) {
+            const double w0 = wr[0] * wz[row], w1 = wr[1] * wz[row], w2 = wr[2] * wz[row];
+            Dr += w0 * s1r2_at(T, Nz, irs[0], izs[row], 0) + w1 * s1r2_at(T, Nz, irs[1], izs[row], 0) +
+                  w2 * s1r2_at(T, Nz, irs[2], izs[row], 0);
+            Dzv += w0 * s1r2_at(T, Nz, irs[0], izs[row], 1) + w1 * s1r2_at(T, Nz, irs[1], izs[row], 1) +
+                   w2 * s1r2_at(T, Nz, irs[2], izs[row], 1);
+        }
+    }
+    const double ex = (rho > 0.0) ? dx / rho : 0.0;
+    const double ey = (rho > 0.0) ? dy / rho : 0.0;
+    Dx = Dr * ex;
+    Dy = Dr * ey;
+    Dz = Dzv;
+}
+
+// One pair (i = lane row, j = uniform source) for variant V, FP64.
+template <int V>
+__device__ __forceinline__ void pair_f64(const KScalars &k, const double *__restrict__ table,
+                                         const double4 pj, const double hj, const double xi,
+                                         const double yi, const double zi, const double hi,
+                                         double &fx, double &fy, double &fz) {
+    double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+    const double beta = hi + hj;
+    if (V == PV_R3) {
+        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        const double s = pj.w * pair_w(r2, beta, k);
+        fx = fma(s, dx, fx);
+        fy = fma(s, dy, fy);
+        fz = fma(s, dz, fz);
+    } else if (V == PV_T3_NEAREST || V == PV_T3_EWALD) {
+        // forces_cuda.cu:622-627: d - L*d/|d| when |d| > L/2 (d/|d| is exactly +-1)
+        if (fabs(dx) > k.halfL) dx -= copysign(k.L, dx);
+        if (fabs(dy) > k.halfL) dy -= copysign(k.L, dy);
+        if (fabs(dz) > k.halfL) dz -= copysign(k.L, dz);
+        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        const double w = pair_w(r2, beta, k);
+        if (V == PV_T3_NEAREST) {
+            const double s = pj.w * w;
+            fx = fma(s, dx, fx);
+            fy = fma(s, dy, fy);
+            fz = fma(s, dz, fz);
+        } else {
+            double Dx, Dy, Dz;
+            t3_interp(table, k.ng, k.L, dx, dy, dz, Dx, Dy, Dz);
+            fx = fma(pj.w, fma(w, dx, -Dx), fx);
+            fy = fma(pj.w, fma(w, dy, -Dy), fy);
+            fz = fma(pj.w, fma(w, dz, -Dz), fz);
+        }
+    } else if (V == PV_S1R2_IMAGES) {
+        // forces_cuda.cu:723-737: raw dz, images m in [-(P+1), P+1], cut |dz+mL| < (P+0.6)L
+        const double rxy2 = fma(dx, dx, dy * dy);
+        double sx = 0.0, sz = 0.0;
+        for (int m = -k.images; m <= k.images; ++m) {
+            const double dze = dz + (double)m * k.L;
+            if (fabs(dze) < k.image_cut) {
+                const double r2 = fma(dze, dze, rxy2);
+                const double s = pj.w * pair_w(r2, beta, k);
+                sx += s;
+                sz = fma(s, dze, sz);
+            }
+        }
+        fx = fma(sx, dx, fx);
+        fy = fma(sx, dy, fy);
+        fz += sz;
+    } else {
+        // forces_cuda.cu:833-836: nearest z-image only
+        if (dz > k.halfL) dz -= k.L;
+        else if (dz < -k.halfL) dz += k.L;
+        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        const double w = pair_w(r2, beta, k);
+        if (V == PV_S1R2_NEAREST) {
+            const double s = pj.w * w;
+            fx = fma(s, dx, fx);
+            fy = fma(s, dy, fy);
+            fz = fma(s, dz, fz);
+        } else {
+            double Dx, Dy, Dz;
+            if (V == PV_S1R2_NGP)
+                s1r2_interp<0>(table, k.nrho, k.nz, k.rho_max, k.L, dx, dy, dz, Dx, Dy, Dz);
+            else if (V == PV_S1R2_CIC)
+                s1r2_interp<2>(table, k.nrho, k.nz, k.rho_max, k.L, dx, dy, dz, Dx, Dy, Dz);
+            else
+                s1r2_interp<4>(table, k.nrho, k.nz, k.rho_max, k.L, dx, dy, dz, Dx, Dy, Dz);
+            fx = fma(pj.w, fma(w, dx, -Dx), fx);
+            fy = fma(pj.w, fma(w, dy, -Dy), fy);
+            fz = fma(pj.w, fma(w, dz, -Dz), fz);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 pair kernel
+// ---------------------------------------------------------------------------
+template <int V, int NI, int UNR>
+__global__ __launch_bounds__(kBlock) void pair_kernel_f64(const double4 *__restrict__ jrec,
+                                                          const double *__restrict__ soft,
+                                                          const double *__restrict__ table,
+                                                          double *__restrict__ part,
+                                                          const KScalars k) {
+    const int tid = threadIdx.x;
+    const int64_t row_base = (int64_t)blockIdx.x * (kBlock * NI);
+    const int split = blockIdx.y;
+    const int64_t j0 = (int64_t)split * k.jchunk;
+    const int64_t j1 = min(k.N, j0 + k.jchunk);
+
+    double xi[NI], yi[NI], zi[NI], hi[NI], fx[NI], fy[NI], fz[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int64_t r = min(row_base + q * kBlock + tid, k.nrows - 1);
+        const double4 p = jrec[k.id_min + r];
+        xi[q] = p.x;
+        yi[q] = p.y;
+        zi[q] = p.z;
+        hi[q] = soft[k.id_min + r];
+        fx[q] = fy[q] = fz[q] = 0.0;
+    }
+
+    int64_t j = j0;
+    for (; j + UNR <= j1; j += UNR) {
+        double4 pj[UNR];
+        double hj[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            pj[u] = jrec[j + u];
+            hj[u] = soft[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+#pragma unroll
+            for (int q = 0; q < NI; ++q)
+                pair_f64<V>(k, table, pj[u], hj[u], xi[q], yi[q], zi[q], hi[q], fx[q], fy[q], fz[q]);
+    }
+    for (; j < j1; ++j) {
+        const double4 pj = jrec[j];
+        const double hj = soft[j];
+#pragma unroll
+        for (int q = 0; q < NI; ++q)
+            pair_f64<V>(k, table, pj, hj, xi[q], yi[q], zi[q], hi[q], fx[q], fy[q], fz[q]);
+    }
+
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int64_t r = row_base + q * kBlock + tid;
+        if (r < k.nrows) {
+            double *out = part + (size_t)split * 3u * (size_t)k.nrows;
+            out[r] = fx[q];
+            out[k.nrows + r] = fy[q];
+            out[2 * k.nrows + r] = fz[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mixed-precision pair kernel (R3): FP32 pair arithmetic on positions
+// recentred on each 64-source tile, FP32 sums inside a tile, FP64 across tiles;
+// pairs inside the softening radius are recomputed from the FP64 records.
+// ---------------------------------------------------------------------------
+constexpr int kMixedTile = 64;
+
+template <int NI>
+__global__ __launch_bounds__(kBlock) void pair_kernel_mixed_r3(
+    const double4 *__restrict__ jrec, const double *__restrict__ soft,
+    const float4 *__restrict__ jrel, const float *__restrict__ soft32,
+    const double4 *__restrict__ centres, double *__restrict__ part, const KScalars k) {
+    const int tid = threadIdx.x;
+    const int64_t row_base = (int64_t)blockIdx.x * (kBlock * NI);
+    const int split = blockIdx.y;
+    const int64_t j0 = (int64_t)split * k.jchunk;  // multiple of kMixedTile
+    const int64_t j1 = min(k.N, j0 + k.jchunk);
+
+    double xi[NI], yi[NI], zi[NI], hi[NI], fx[NI], fy[NI], fz[NI];
+    float hi32[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int64_t r = min(row_base + q * kBlock + tid, k.nrows - 1);
+        const double4 p = jrec[k.id_min + r];
+        xi[q] = p.x;
+        yi[q] = p.y;
+        zi[q] = p.z;
+        hi[q] = soft[k.id_min + r];
+        hi32[q] = (float)hi[q];
+        fx[q] = fy[q] = fz[q] = 0.0;
+    }
+
+    for (int64_t t0 = j0; t0 < j1; t0 += kMixedTile) {
+        const double4 c = centres[t0 / kMixedTile];
+        float xr[NI], yr[NI], zr[NI], ax[NI], ay[NI], az[NI];
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            xr[q] = (float)(xi[q] - c.x);
+            yr[q] = (float)(yi[q] - c.y);
+            zr[q] = (float)(zi[q] - c.z);
+            ax[q] = ay[q] = az[q] = 0.0f;
+        }
+        const int64_t tend = min(j1, t0 + kMixedTile);
+        for (int64_t j = t0; j < tend; ++j) {
+            const float4 pj = jrel[j];
+            const float hj = soft32[j];
+#pragma unroll
+            for (int q = 0; q < NI; ++q) {
+                const float dx = pj.x - xr[q], dy = pj.y - yr[q], dz = pj.z - zr[q];
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const float beta = hi32[q] + hj;
+                if (r2 < beta * beta) {
+                    // close pair: exact FP64 term from the FP64 records
+                    const double4 pd = jrec[j];
+                    const double ddx = pd.x - xi[q], ddy = pd.y - yi[q], ddz = pd.z - zi[q];
+                    const double rr2 = fma(ddx, ddx, fma(ddy, ddy, ddz * ddz));
+                    const double s = pd.w * pair_w(rr2, hi[q] + soft[j], k);
+                    fx[q] = fma(s, ddx, fx[q]);
+                    fy[q] = fma(s, ddy, fy[q]);
+                    fz[q] = fma(s, ddz, fz[q]);
+                } else {
+                    const float y = __builtin_amdgcn_rsqf(r2);
+                    const float s = (y * y) * (y * pj.w);
+                    ax[q] = fmaf(s, dx, ax[q]);
+                    ay[q] = fmaf(s, dy, ay[q]);
+                    az[q] = fmaf(s, dz, az[q]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            fx[q] += (double)ax[q];
+            fy[q] += (double)ay[q];
+            fz[q] += (double)az[q];
+        }
+    }
+
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int64_t r = row_base + q * kBlock + tid;
+        if (r < k.nrows) {
+            double *out = part + (size_t)split * 3u * (size_t)k.nrows;
+            out[r] = fx[q];
+            out[k.nrows + r] = fy[q];
+            out[2 * k.nrows + r] = fz[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fixed-order reduction of the splits + external term, AoS output
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double radial_correction(double r, double R, const double *T,
+                                                    int size) {
+    // forces_cuda.cu:41-70 (linear interpolation; nodes assumed at R/size*i)
+    const double step = R / (double)size;
+    const int i = (int)floor(r / R * (size - 1));
+    double c = T[size - 1];
+    if (i < size - 1) {
+        const double X1 = step * i, Y1 = T[i], X2 = step * (i + 1), Y2 = T[i + 1];
+        const double A = (Y2 - Y1) / (X2 - X1);
+        const double B = Y1 - A * X1;
+        c = A * r + B;
+    }
+    return c;
+}
+
+__global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ part,
+                                                     const double4 *__restrict__ jrec,
+                                                     const double *__restrict__ radial,
+                                                     double *__restrict__ F, int64_t nrows,
+                                                     int64_t id_min, int splits, int ext_mode,
+                                                     double ext_coeff, double Rsim,
+                                                     int radial_size) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int s = 0; s < splits; ++s) {
+        const double *p = part + (size_t)s * 3u * (size_t)nrows;
+        f[0] += p[r];
+        f[1] += p[nrows + r];
+        f[2] += p[2 * nrows + r];
+    }
+    if (ext_mode == 3) {
+        // forces_cuda.cu:698-701: mass_in_unit_sphere * x * correction(r_xy), x and y only
+        const double4 x = jrec[id_min + r];
+        const double corr = radial_correction(sqrt(x.x * x.x + x.y * x.y), Rsim, radial, radial_size);
+        f[0] += ext_coeff * x.x * corr;
+        f[1] += ext_coeff * x.y * corr;
+    } else if (ext_mode != 0) {
+        // forces_cuda.cu:545-556 (mode 1, xyz) and :848-857 (mode 2, xy)
+        const double4 x = jrec[id_min + r];
+        f[0] += ext_coeff * x.x;
+        f[1] += ext_coeff * x.y;
+        if (ext_mode == 1) f[2] += ext_coeff * x.z;
+    }
+    F[3 * r + 0] = f[0];
+    F[3 * r + 1] = f[1];
+    F[3 * r + 2] = f[2];
+}
+
+// ---------------------------------------------------------------------------
+// layout kernels
+// ---------------------------------------------------------------------------
+__global__ void pack_kernel(const double *__restrict__ x, const double *__restrict__ M,
+                            const double *__restrict__ h, double4 *__restrict__ jrec,
+                            double *__restrict__ soft, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    jrec[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], M[i]);
+    soft[i] = h[i];
+}
+
+__global__ void pack_positions_kernel(const double *__restrict__ x, double4 *__restrict__ jrec,
+                                      int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = jrec[i].w;
+    jrec[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], m);
+}
+
+__global__ void unpack_positions_kernel(const double4 *__restrict__ jrec, double *__restrict__ x,
+                                        int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = jrec[i];
+    x[3 * i] = p.x;
+    x[3 * i + 1] = p.y;
+    x[3 * i + 2] = p.z;
+}
+
+// Mixed path: per-tile centre (bounding-box middle, FP64) and FP32 offsets.
+__global__ __launch_bounds__(kMixedTile) void mixed_tiles_kernel(
+    const double4 *__restrict__ jrec, const double *__restrict__ soft, float4 *__restrict__ jrel,
+    float *__restrict__ soft32, double4 *__restrict__ centres, int64_t n) {
+    __shared__ double red[6][kMixedTile];
+    const int64_t t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t i = t * kMixedTile + lane;
+    const bool ok = i < n;
+    const double4 p = jrec[ok ? i : (n - 1)];
+    red[0][lane] = p.x; red[1][lane] = p.y; red[2][lane] = p.z;
+    red[3][lane] = p.x; red[4][lane] = p.y; red[5][lane] = p.z;
+    __syncthreads();
+    for (int s = kMixedTile / 2; s > 0; s >>= 1) {
+        if (lane < s) {
+            for (int a = 0; a < 3; ++a) {
+                red[a][lane] = fmin(red[a][lane], red[a][lane + s]);
+                red[3 + a][lane] = fmax(red[3 + a][lane], red[3 + a][lane + s]);
+            }
+        }
+        __syncthreads();
+    }
+    const double cx = 0.5 * (red[0][0] + red[3][0]);
+    const double cy = 0.5 * (red[1][0] + red[4][0]);
+    const double cz = 0.5 * (red[2][0] + red[5][0]);
+    if (lane == 0) centres[t] = make_double4(cx, cy, cz, 0.0);
+    if (ok) {
+        jrel[i] = make_float4((float)(p.x - cx), (float)(p.y - cy), (float)(p.z - cz), (float)p.w);
+        soft32[i] = (float)soft[i];
+    }
+}
+
+// Table variants gather 8..192 table words per pair: one row per lane keeps
+// them out of spill territory; the arithmetic variants hold kRowsPerLane rows.
+constexpr int rows_per_lane(int V) {
+    return (V == PV_T3_EWALD || V == PV_S1R2_NGP || V == PV_S1R2_CIC || V == PV_S1R2_TSC)
+               ? 1
+               : kRowsPerLane;
+}
+
+template <int V>
+hipError_t launch_variant(const EvalPlan &p, const DeviceParticles &dp, double *part,
+                          hipStream_t st, const KScalars &k, const double *table) {
+    constexpr int NI = rows_per_lane(V);
+    constexpr int UNR = (V == PV_R3 || V == PV_T3_NEAREST || V == PV_S1R2_NEAREST) ? 4 : 1;
+    const int64_t rows_per_block = (int64_t)kBlock * NI;
+    dim3 grid((unsigned)((p.nrows + rows_per_block - 1) / rows_per_block), (unsigned)p.splits);
+    hipLaunchKernelGGL((pair_kernel_f64<V, NI, UNR>), grid, dim3(kBlock), 0, st, dp.jrec, dp.soft,
+                       table, part, k);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t partial_bytes(const EvalPlan &p) {
+    return (size_t)p.splits * 3u * (size_t)std::max<int64_t>(p.nrows, 1) * sizeof(double);
+}
+
+// Mixed-path scratch lives after the partial sums in the same buffer:
+// jrel [N] float4, soft32 [N] float, centres [ceil(N/64)] double4.
+size_t mixed_extra_bytes(int64_t N) {
+    const int64_t tiles = (N + kMixedTile - 1) / kMixedTile;
+    return (size_t)N * sizeof(float4) + (size_t)N * sizeof(float) + (size_t)tiles * sizeof(double4) + 512;
+}
+
+hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *partials,
+                         double *F, hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_pair_end,
+                         hipEvent_t ev_end) {
+    hipError_t e;
+    if (ev_begin && (e = hipEventRecord(ev_begin, st)) != hipSuccess) return e;
+    if (p.nrows > 0 && p.N > 0) {
+        KScalars k;
+        k.N = p.N;
+        k.id_min = p.id_min;
+        k.nrows = p.nrows;
+        k.jchunk = p.jchunk;
+        k.L = p.L;
+        k.halfL = 0.5 * p.L;
+        k.ng = p.t3_ng;
+        k.nrho = p.s1r2_nrho;
+        k.nz = p.s1r2_nz;
+        k.rho_max = kRadialExtent * p.Rsim;
+        k.images = p.images;
+        k.image_cut = (((double)p.images) - 0.4) * p.L;
+        k.c15 = 1.5;
+        k.c1875 = 1.875;
+        if (p.precision == STEPS_PREC_MIXED && p.variant == PV_R3) {
+            char *extra = (char *)partials + ((partial_bytes(p) + 255) & ~(size_t)255);
+            float4 *jrel = (float4 *)extra;
+            float *soft32 = (float *)(jrel + p.N);
+            double4 *centres = (double4 *)(((uintptr_t)(soft32 + p.N) + 255) & ~(uintptr_t)255);
+            const int64_t tiles = (p.N + kMixedTile - 1) / kMixedTile;
+            hipLaunchKernelGGL(mixed_tiles_kernel, dim3((unsigned)tiles), dim3(kMixedTile), 0, st,
+                               dp.jrec, dp.soft, jrel, soft32, centres, p.N);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            constexpr int NI = kRowsPerLaneMixed;
+            const int64_t rows_per_block = (int64_t)kBlock * NI;
+            dim3 grid((unsigned)((p.nrows + rows_per_block - 1) / rows_per_block), (unsigned)p.splits);
+            hipLaunchKernelGGL((pair_kernel_mixed_r3<NI>), grid, dim3(kBlock), 0, st, dp.jrec,
+                               dp.soft, jrel, soft32, centres, partials, k);
+            e = hipGetLastError();
+        } else {
+            switch (p.variant) {
+            case PV_R3: e = launch_variant<PV_R3>(p, dp, partials, st, k, nullptr); break;
+            case PV_T3_NEAREST: e = launch_variant<PV_T3_NEAREST>(p, dp, partials, st, k, nullptr); break;
+            case PV_T3_EWALD: e = launch_variant<PV_T3_EWALD>(p, dp, partials, st, k, p.t3); break;
+            case PV_S1R2_NEAREST: e = launch_variant<PV_S1R2_NEAREST>(p, dp, partials, st, k, nullptr); break;
+            case PV_S1R2_NGP: e = launch_variant<PV_S1R2_NGP>(p, dp, partials, st, k, p.s1r2); break;
+            case PV_S1R2_CIC: e = launch_variant<PV_S1R2_CIC>(p, dp, partials, st, k, p.s1r2); break;
+            case PV_S1R2_TSC: e = launch_variant<PV_S1R2_TSC>(p, dp, partials, st, k, p.s1r2); break;
+            case PV_S1R2_IMAGES: e = launch_variant<PV_S1R2_IMAGES>(p, dp, partials, st, k, nullptr); break;
+            default: return hipErrorInvalidValue;
+            }
+        }
+        if (e != hipSuccess) return e;
+    }
+    if (ev_pair_end && (e = hipEventRecord(ev_pair_end, st)) != hipSuccess) return e;
+    if (p.nrows > 0) {
+        const int splits = (p.N > 0) ? p.splits : 0;
+        hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)((p.nrows + 255) / 256)), dim3(256), 0, st,
+                           partials, dp.jrec, p.radial, F, p.nrows, p.id_min, splits, p.ext_mode,
+                           p.ext_coeff, p.Rsim, p.radial_size);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (ev_end && (e = hipEventRecord(ev_end, st)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+hipError_t launch_pack(const double *x, const double *M, const double *soft, int64_t n,
+                       DeviceParticles &dp, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, M,
+                       soft, dp.jrec, dp.soft, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_positions(const double *x, int64_t n, DeviceParticles &dp,
+                                 hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_positions_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       x, dp.jrec, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_positions(const DeviceParticles &dp, double *x, hipStream_t st) {
+    if (dp.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(unpack_positions_kernel, dim3((unsigned)((dp.n + 255) / 256)), dim3(256), 0,
+                       st, dp.jrec, x, dp.n);
+    return hipGetLastError();
+}
+
+}  // namespace steps
