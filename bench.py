@@ -107,6 +107,22 @@ def time_path(sim, steps: int, warmup: int):
     return wall, float(np.mean(pair_ms)), float(np.mean(total_ms))
 
 
+def copy_calibration() -> int:
+    """One elementwise copy of 2^27 doubles (1 GiB read, 1 GiB written): under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE its counter values per byte scale
+    the pair kernel's counters to HBM bytes."""
+    import torch
+
+    n = 1 << 27
+    a = torch.full((n,), 1.0, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "check": float(b[n - 1].item())}))
+    return 0
+
+
 def load_traffic(name: str):
     f = ROOT / "profiles" / f"{name}_traffic.json"
     if f.exists():
@@ -125,7 +141,11 @@ def main() -> int:
     ap.add_argument("--workload", default="r3_2m", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run one plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the full N x N interaction on a single "
               "MI355X; multi-GPU sharding is design only (see DESIGN.md)", file=sys.stderr)
@@ -176,6 +196,18 @@ def main() -> int:
         Fm = sim.get_forces()
         sim.close()
         err = np.linalg.norm(Fm - F64, axis=1) / np.linalg.norm(F64, axis=1)
+        # the comoving external term dominates |F| for this mass normalisation,
+        # so the error of the pair sum alone (cosmology off) is reported too
+        errs = {}
+        for prec in (steps_amd.PREC_FP64, steps_amd.PREC_MIXED):
+            d0, k0 = steps_amd.make_desc(steps_amd.TOPO_R3, precision=prec)
+            s0 = steps_amd.Sim(d0, N)
+            s0.set_state(w.x, w.M, w.soft, w.v)
+            s0.forces()
+            errs[prec] = s0.get_forces()
+            s0.close()
+        e0 = (np.linalg.norm(errs[steps_amd.PREC_MIXED] - errs[steps_amd.PREC_FP64], axis=1) /
+              np.linalg.norm(errs[steps_amd.PREC_FP64], axis=1))
         ach_m = FLOP_PER_PAIR_R3 * pairs / (pair_m * 1e-3) / 1e12
         out["mixed"] = {
             "value": pairs * args.steps / wall_m / 1e9,
@@ -184,6 +216,9 @@ def main() -> int:
             "speedup_vs_fp64": (pairs / wall_m) / (pairs / wall),
             "rel_err_vs_fp64_p50": float(np.median(err)),
             "rel_err_vs_fp64_p99": float(np.percentile(err, 99)),
+            "rel_err_pairsum_vs_fp64_p50": float(np.median(e0)),
+            "rel_err_pairsum_vs_fp64_p99": float(np.percentile(e0, 99)),
+            "rel_err_pairsum_vs_fp64_max": float(np.max(e0)),
             "roofline": {"bound": "valu", "achieved": ach_m, "peak": PEAK_FP32_VECTOR_TFLOPS,
                          "unit": "TFLOP/s", "frac": ach_m / PEAK_FP32_VECTOR_TFLOPS},
         }

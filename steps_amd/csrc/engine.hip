@@ -205,8 +205,7 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
     int64_t splits = (target + nib - 1) / nib;
     splits = std::max<int64_t>(1, std::min<int64_t>(splits, std::max<int64_t>(1, N / 256)));
     int64_t jchunk = std::max<int64_t>(1, (N + splits - 1) / splits);
-    if (p.precision == STEPS_PREC_MIXED)
-        jchunk = ((jchunk + kMixedTileRows - 1) / kMixedTileRows) * kMixedTileRows;
+    jchunk = ((jchunk + kTile - 1) / kTile) * kTile;  // splits start on tile boundaries
     splits = std::max<int64_t>(1, (N + jchunk - 1) / jchunk);
     p.splits = (int)splits;
     p.jchunk = jchunk;
@@ -214,9 +213,7 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
 }
 
 size_t scratch_bytes(const EvalPlan &p) {
-    size_t b = (partial_bytes(p) + 255) & ~(size_t)255;
-    if (p.precision == STEPS_PREC_MIXED) b += mixed_extra_bytes(p.N) + 256;
-    return b;
+    return ((partial_bytes(p) + 255) & ~(size_t)255) + tile_extra_bytes(p.N, p.precision);
 }
 
 // State of the one-shot entry point (one device at a time, under a lock).

@@ -31,7 +31,7 @@ enum PairVariant : int {
 
 // Device-resident particle set in the layout the pair kernels read:
 // jrec[i] = (x, y, z, m) for wave-uniform scalar loads of the j-particle,
-// soft[i] = h.  xoff holds FP32 tile-recentred positions for the mixed path.
+// soft[i] = h.
 struct DeviceParticles {
     double4 *jrec = nullptr;
     double *soft = nullptr;
@@ -60,14 +60,14 @@ struct EvalPlan {
 };
 
 // Scratch of one evaluation: partial sums [splits][3][nrows] (FP64), followed
-// (256-byte aligned) by the mixed path's FP32 tile offsets when precision is
-// STEPS_PREC_MIXED.
+// (256-byte aligned) by the source-tile table and, for STEPS_PREC_MIXED, the
+// FP32 tile-recentred source records.
 size_t partial_bytes(const EvalPlan &p);
-size_t mixed_extra_bytes(int64_t N);
-constexpr int kMixedTileRows = 64;  // sources per recentred FP32 tile
+size_t tile_extra_bytes(int64_t N, int precision);
+constexpr int kTile = 64;  // sources per tile (one wave64 prepares one tile)
 
-// Launch pair kernels + fixed-order reduction; F is device AoS [3*nrows].
-// If ev_pair_begin/ev_pair_end/ev_end are non-null they are recorded around
+// Launch the tile pass, the pair kernel and the fixed-order reduction; F is
+// device AoS [3*nrows].  Non-null events are recorded right before and after
 // the pair kernel and after the reduction (on `stream`).
 hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *partials,
                          double *F, hipStream_t stream, hipEvent_t ev_begin,

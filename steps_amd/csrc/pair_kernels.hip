@@ -8,13 +8,16 @@
 //     still fills 256 CUs; every (row-block, split) writes its FP64 partial
 //     sums to partials[split][3][nrows] and a second kernel adds the splits in
 //     fixed order (deterministic, no atomics) and adds the external term;
-//   * the source particle j is wave-uniform, so its record (x,y,z,m) and h are
-//     fetched with scalar loads into SGPRs and reach all 64 lanes as VALU
-//     scalar operands -- the broadcast costs no VALU or LDS instruction;
+//   * the source particle j is wave-uniform, so its record (x,y,z,m) is
+//     fetched with scalar loads into SGPRs and reaches all 64 lanes as a VALU
+//     scalar operand -- the broadcast costs no VALU or LDS instruction;
+//   * sources are processed in tiles of 64; a tiny pre-pass stores each tile's
+//     largest softening length, so one v_cmp against (h_i + h_max,tile)^2
+//     replaces the per-pair h_i + h_j test, and the exact cubic-spline branch
+//     (r < h_i + h_j, forces_cuda.cu:466-519, the reference's own formula)
+//     runs only when some lane of the wave is that close;
 //   * 1/r^3 uses v_rsq_f64 (2^-24 accurate on gfx950) plus one second-order
-//     Newton correction; the cubic-spline softening branch (r < h_i + h_j,
-//     forces_cuda.cu:466-519) is evaluated exactly, with the reference's own
-//     formula, only when some lane of the wave needs it.
+//     Newton correction.
 #include "internal.h"
 
 #include <algorithm>
@@ -47,6 +50,15 @@ struct KScalars {
 // FP64 building blocks
 // ---------------------------------------------------------------------------
 
+// Copy a wave-uniform constant into a VGPR once, so that an FMA taking it and
+// another SGPR constant (VOP3 reads at most one SGPR on gfx9) does not
+// re-materialise it with a v_mov in every pair.
+__device__ __forceinline__ double in_vgpr(double x) {
+    double r;
+    asm("v_mov_b64 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+
 // r2^(-3/2): v_rsq_f64 seed y (relative error e0 <= 2^-24), e = 1 - r2*y^2,
 // y^3 * (1 - e)^(-3/2) ~= y^3 * (1 + 1.5 e + 1.875 e^2); truncation error
 // O(e^3) ~ 1e-21.  r2 == 0 gives NaN: the caller's softening branch owns it.
@@ -75,13 +87,6 @@ __device__ __forceinline__ double softening_exact(double r, double beta) {
     }
     const double c0 = 32.0 / b6, c1 = -38.4 / b5, c2 = (32.0 / 3.0) / b3;
     return c0 * r3 + c1 * r2 + c2;
-}
-
-// Softened coefficient w(r, beta) for a pair with squared distance r2.
-__device__ __forceinline__ double pair_w(double r2, double beta, const KScalars &k) {
-    double w = inv_r3(r2, k.c15, k.c1875);
-    if (r2 < beta * beta) w = softening_exact(sqrt(r2), beta);
-    return w;
 }
 
 __device__ __forceinline__ int imodp(int i, int n) {
@@ -219,17 +224,31 @@ __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, int Nr
     Dz = Dzv;
 }
 
+// Softened coefficient for a pair, with the tile bound B = (h_i + h_max,tile)^2:
+// only pairs with r2 < B can be inside h_i + h_j, so the exact test (and the
+// softening_exact evaluation) runs in a branch that almost no wave takes.
+__device__ __forceinline__ double pair_w_tile(double r2, double B, double hi,
+                                              const double *__restrict__ soft, int64_t j,
+                                              const KScalars &k) {
+    double w = inv_r3(r2, k.c15, k.c1875);
+    if (r2 < B) {
+        const double beta = hi + soft[j];
+        if (r2 < beta * beta) w = softening_exact(sqrt(r2), beta);
+    }
+    return w;
+}
+
 // One pair (i = lane row, j = uniform source) for variant V, FP64.
 template <int V>
 __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__restrict__ table,
-                                         const double4 pj, const double hj, const double xi,
-                                         const double yi, const double zi, const double hi,
+                                         const double *__restrict__ soft, const int64_t j,
+                                         const double4 pj, const double xi, const double yi,
+                                         const double zi, const double hi, const double B,
                                          double &fx, double &fy, double &fz) {
     double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-    const double beta = hi + hj;
     if (V == PV_R3) {
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double s = pj.w * pair_w(r2, beta, k);
+        const double s = pj.w * pair_w_tile(r2, B, hi, soft, j, k);
         fx = fma(s, dx, fx);
         fy = fma(s, dy, fy);
         fz = fma(s, dz, fz);
@@ -239,7 +258,7 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
         if (fabs(dy) > k.halfL) dy -= copysign(k.L, dy);
         if (fabs(dz) > k.halfL) dz -= copysign(k.L, dz);
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double w = pair_w(r2, beta, k);
+        const double w = pair_w_tile(r2, B, hi, soft, j, k);
         if (V == PV_T3_NEAREST) {
             const double s = pj.w * w;
             fx = fma(s, dx, fx);
@@ -260,7 +279,7 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
             const double dze = dz + (double)m * k.L;
             if (fabs(dze) < k.image_cut) {
                 const double r2 = fma(dze, dze, rxy2);
-                const double s = pj.w * pair_w(r2, beta, k);
+                const double s = pj.w * pair_w_tile(r2, B, hi, soft, j, k);
                 sx += s;
                 sz = fma(s, dze, sz);
             }
@@ -273,7 +292,7 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
         if (dz > k.halfL) dz -= k.L;
         else if (dz < -k.halfL) dz += k.L;
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double w = pair_w(r2, beta, k);
+        const double w = pair_w_tile(r2, B, hi, soft, j, k);
         if (V == PV_S1R2_NEAREST) {
             const double s = pj.w * w;
             fx = fma(s, dx, fx);
@@ -295,18 +314,23 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
 }
 
 // ---------------------------------------------------------------------------
-// FP64 pair kernel
+// FP64 pair kernel: sources in tiles of kTile (64); per tile each lane forms
+// B = (h_i + h_max,tile)^2 once, then UNR sources at a time come in through
+// scalar loads and are applied to the NI rows of the lane.
 // ---------------------------------------------------------------------------
 template <int V, int NI, int UNR>
 __global__ __launch_bounds__(kBlock) void pair_kernel_f64(const double4 *__restrict__ jrec,
                                                           const double *__restrict__ soft,
+                                                          const double4 *__restrict__ tiles,
                                                           const double *__restrict__ table,
                                                           double *__restrict__ part,
-                                                          const KScalars k) {
+                                                          const KScalars k0) {
+    KScalars k = k0;
+    k.c15 = in_vgpr(k0.c15);
     const int tid = threadIdx.x;
     const int64_t row_base = (int64_t)blockIdx.x * (kBlock * NI);
     const int split = blockIdx.y;
-    const int64_t j0 = (int64_t)split * k.jchunk;
+    const int64_t j0 = (int64_t)split * k.jchunk;  // multiple of kTile
     const int64_t j1 = min(k.N, j0 + k.jchunk);
 
     double xi[NI], yi[NI], zi[NI], hi[NI], fx[NI], fy[NI], fz[NI];
@@ -321,27 +345,33 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(const double4 *__restr
         fx[q] = fy[q] = fz[q] = 0.0;
     }
 
-    int64_t j = j0;
-    for (; j + UNR <= j1; j += UNR) {
-        double4 pj[UNR];
-        double hj[UNR];
+    for (int64_t t0 = j0; t0 < j1; t0 += kTile) {
+        const double hmax = tiles[t0 / kTile].w;
+        double B[NI];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            pj[u] = jrec[j + u];
-            hj[u] = soft[j + u];
+        for (int q = 0; q < NI; ++q) {
+            const double b = hi[q] + hmax;
+            B[q] = b * b;
         }
+        const int64_t tend = min(j1, t0 + kTile);
+        int64_t j = t0;
+        for (; j + UNR <= tend; j += UNR) {
+            double4 pj[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u)
+            for (int u = 0; u < UNR; ++u) pj[u] = jrec[j + u];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int q = 0; q < NI; ++q)
+                    pair_f64<V>(k, table, soft, j + u, pj[u], xi[q], yi[q], zi[q], hi[q], B[q], fx[q],
+                                fy[q], fz[q]);
+        }
+        for (; j < tend; ++j) {
+            const double4 pj = jrec[j];
 #pragma unroll
             for (int q = 0; q < NI; ++q)
-                pair_f64<V>(k, table, pj[u], hj[u], xi[q], yi[q], zi[q], hi[q], fx[q], fy[q], fz[q]);
-    }
-    for (; j < j1; ++j) {
-        const double4 pj = jrec[j];
-        const double hj = soft[j];
-#pragma unroll
-        for (int q = 0; q < NI; ++q)
-            pair_f64<V>(k, table, pj, hj, xi[q], yi[q], zi[q], hi[q], fx[q], fy[q], fz[q]);
+                pair_f64<V>(k, table, soft, j, pj, xi[q], yi[q], zi[q], hi[q], B[q], fx[q], fy[q], fz[q]);
+        }
     }
 
 #pragma unroll
@@ -358,24 +388,41 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(const double4 *__restr
 
 // ---------------------------------------------------------------------------
 // mixed-precision pair kernel (R3): FP32 pair arithmetic on positions
-// recentred on each 64-source tile, FP32 sums inside a tile, FP64 across tiles;
-// pairs inside the softening radius are recomputed from the FP64 records.
+// recentred on each 64-source tile (offsets from the tile's bounding-box
+// centre, prepared once per evaluation), FP32 sums inside a tile, FP64 across
+// tiles; pairs inside the softening radius are recomputed in FP64 from the
+// FP64 records, so close-pair forces keep full precision.
 // ---------------------------------------------------------------------------
-constexpr int kMixedTile = 64;
+__device__ __forceinline__ void mixed_pair(const float4 pj, const float xr, const float yr,
+                                           const float zr, const float B, bool &near, float &ax,
+                                           float &ay, float &az) {
+    const float dx = pj.x - xr, dy = pj.y - yr, dz = pj.z - zr;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float y = __builtin_amdgcn_rsqf(r2);
+    const float s = (y * y) * (y * pj.w);
+    if (r2 < B) {
+        near = true;
+    } else {
+        ax = fmaf(s, dx, ax);
+        ay = fmaf(s, dy, ay);
+        az = fmaf(s, dz, az);
+    }
+}
 
-template <int NI>
+template <int NI, int UNR>
 __global__ __launch_bounds__(kBlock) void pair_kernel_mixed_r3(
     const double4 *__restrict__ jrec, const double *__restrict__ soft,
-    const float4 *__restrict__ jrel, const float *__restrict__ soft32,
-    const double4 *__restrict__ centres, double *__restrict__ part, const KScalars k) {
+    const float4 *__restrict__ jrel, const double4 *__restrict__ tiles,
+    double *__restrict__ part, const KScalars k0) {
+    KScalars k = k0;
+    k.c15 = in_vgpr(k0.c15);
     const int tid = threadIdx.x;
     const int64_t row_base = (int64_t)blockIdx.x * (kBlock * NI);
     const int split = blockIdx.y;
-    const int64_t j0 = (int64_t)split * k.jchunk;  // multiple of kMixedTile
+    const int64_t j0 = (int64_t)split * k.jchunk;  // multiple of kTile
     const int64_t j1 = min(k.N, j0 + k.jchunk);
 
     double xi[NI], yi[NI], zi[NI], hi[NI], fx[NI], fy[NI], fz[NI];
-    float hi32[NI];
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
         const int64_t r = min(row_base + q * kBlock + tid, k.nrows - 1);
@@ -384,44 +431,60 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_mixed_r3(
         yi[q] = p.y;
         zi[q] = p.z;
         hi[q] = soft[k.id_min + r];
-        hi32[q] = (float)hi[q];
         fx[q] = fy[q] = fz[q] = 0.0;
     }
 
-    for (int64_t t0 = j0; t0 < j1; t0 += kMixedTile) {
-        const double4 c = centres[t0 / kMixedTile];
-        float xr[NI], yr[NI], zr[NI], ax[NI], ay[NI], az[NI];
+    for (int64_t t0 = j0; t0 < j1; t0 += kTile) {
+        const double4 c = tiles[t0 / kTile];
+        float xr[NI], yr[NI], zr[NI], B[NI], ax[NI], ay[NI], az[NI];
 #pragma unroll
         for (int q = 0; q < NI; ++q) {
             xr[q] = (float)(xi[q] - c.x);
             yr[q] = (float)(yi[q] - c.y);
             zr[q] = (float)(zi[q] - c.z);
+            // conservative FP32 bound: (h_i + h_max)^2 widened by 1e-5 so that
+            // FP32 rounding of r2 cannot hide a pair inside the radius
+            const float b = (float)((hi[q] + c.w) * (1.0 + 1e-5));
+            B[q] = b * b;
             ax[q] = ay[q] = az[q] = 0.0f;
         }
-        const int64_t tend = min(j1, t0 + kMixedTile);
-        for (int64_t j = t0; j < tend; ++j) {
-            const float4 pj = jrel[j];
-            const float hj = soft32[j];
+        const int64_t tend = min(j1, t0 + kTile);
+        bool near = false;
+        int64_t j = t0;
+        for (; j + UNR <= tend; j += UNR) {
+            float4 pj[UNR];
 #pragma unroll
-            for (int q = 0; q < NI; ++q) {
-                const float dx = pj.x - xr[q], dy = pj.y - yr[q], dz = pj.z - zr[q];
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const float beta = hi32[q] + hj;
-                if (r2 < beta * beta) {
-                    // close pair: exact FP64 term from the FP64 records
-                    const double4 pd = jrec[j];
-                    const double ddx = pd.x - xi[q], ddy = pd.y - yi[q], ddz = pd.z - zi[q];
-                    const double rr2 = fma(ddx, ddx, fma(ddy, ddy, ddz * ddz));
-                    const double s = pd.w * pair_w(rr2, hi[q] + soft[j], k);
-                    fx[q] = fma(s, ddx, fx[q]);
-                    fy[q] = fma(s, ddy, fy[q]);
-                    fz[q] = fma(s, ddz, fz[q]);
-                } else {
-                    const float y = __builtin_amdgcn_rsqf(r2);
-                    const float s = (y * y) * (y * pj.w);
-                    ax[q] = fmaf(s, dx, ax[q]);
-                    ay[q] = fmaf(s, dy, ay[q]);
-                    az[q] = fmaf(s, dz, az[q]);
+            for (int u = 0; u < UNR; ++u) pj[u] = jrel[j + u];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int q = 0; q < NI; ++q) mixed_pair(pj[u], xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q]);
+        }
+        for (; j < tend; ++j) {
+            const float4 pj = jrel[j];
+#pragma unroll
+            for (int q = 0; q < NI; ++q) mixed_pair(pj, xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q]);
+        }
+        // rare: some lane had a pair within its bound -> redo those pairs in FP64
+        if (__builtin_expect(near, 0)) {
+            for (int64_t jj = t0; jj < tend; ++jj) {
+                const float4 pr = jrel[jj];
+                const double4 pd = jrec[jj];
+#pragma unroll
+                for (int q = 0; q < NI; ++q) {
+                    const float dx = pr.x - xr[q], dy = pr.y - yr[q], dz = pr.z - zr[q];
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    if (r2 < B[q]) {
+                        const double ddx = pd.x - xi[q], ddy = pd.y - yi[q], ddz = pd.z - zi[q];
+                        const double rr2 = fma(ddx, ddx, fma(ddy, ddy, ddz * ddz));
+                        double w = inv_r3(rr2, k.c15, k.c1875);
+                        const double beta = hi[q] + soft[jj];
+                        if (rr2 < beta * beta) w = softening_exact(sqrt(rr2), beta);
+                        const double s = pd.w * w;
+                        fx[q] = fma(s, ddx, fx[q]);
+                        fy[q] = fma(s, ddy, fy[q]);
+                        fz[q] = fma(s, ddz, fz[q]);
+                    }
                 }
             }
         }
@@ -527,37 +590,41 @@ __global__ void unpack_positions_kernel(const double4 *__restrict__ jrec, double
     x[3 * i + 2] = p.z;
 }
 
-// Mixed path: per-tile centre (bounding-box middle, FP64) and FP32 offsets.
-__global__ __launch_bounds__(kMixedTile) void mixed_tiles_kernel(
-    const double4 *__restrict__ jrec, const double *__restrict__ soft, float4 *__restrict__ jrel,
-    float *__restrict__ soft32, double4 *__restrict__ centres, int64_t n) {
-    __shared__ double red[6][kMixedTile];
+// One wave64 per source tile: the tile's bounding-box centre and largest
+// softening length (double4 (cx, cy, cz, h_max)); with MIXED also the FP32
+// offsets of each source from that centre plus its mass.
+template <bool MIXED>
+__global__ __launch_bounds__(kTile) void tiles_kernel(const double4 *__restrict__ jrec,
+                                                     const double *__restrict__ soft,
+                                                     double4 *__restrict__ tiles,
+                                                     float4 *__restrict__ jrel, int64_t n) {
     const int64_t t = blockIdx.x;
     const int lane = threadIdx.x;
-    const int64_t i = t * kMixedTile + lane;
+    const int64_t i = t * kTile + lane;
     const bool ok = i < n;
-    const double4 p = jrec[ok ? i : (n - 1)];
-    red[0][lane] = p.x; red[1][lane] = p.y; red[2][lane] = p.z;
-    red[3][lane] = p.x; red[4][lane] = p.y; red[5][lane] = p.z;
-    __syncthreads();
-    for (int s = kMixedTile / 2; s > 0; s >>= 1) {
-        if (lane < s) {
-            for (int a = 0; a < 3; ++a) {
-                red[a][lane] = fmin(red[a][lane], red[a][lane + s]);
-                red[3 + a][lane] = fmax(red[3 + a][lane], red[3 + a][lane + s]);
-            }
+    const int64_t src = ok ? i : t * kTile;  // t*kTile < n: padding lanes repeat the first source
+    const double4 p = jrec[src];
+    double lo[3] = {p.x, p.y, p.z}, hi[3] = {p.x, p.y, p.z};
+    double hm = soft[src];
+#pragma unroll
+    for (int m = kTile / 2; m > 0; m >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fmin(lo[a], __shfl_xor(lo[a], m, kTile));
+            hi[a] = fmax(hi[a], __shfl_xor(hi[a], m, kTile));
         }
-        __syncthreads();
+        hm = fmax(hm, __shfl_xor(hm, m, kTile));
     }
-    const double cx = 0.5 * (red[0][0] + red[3][0]);
-    const double cy = 0.5 * (red[1][0] + red[4][0]);
-    const double cz = 0.5 * (red[2][0] + red[5][0]);
-    if (lane == 0) centres[t] = make_double4(cx, cy, cz, 0.0);
-    if (ok) {
+    const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+    if (lane == 0) tiles[t] = make_double4(cx, cy, cz, hm);
+    if (MIXED && ok)
         jrel[i] = make_float4((float)(p.x - cx), (float)(p.y - cy), (float)(p.z - cz), (float)p.w);
-        soft32[i] = (float)soft[i];
-    }
 }
+
+template <int V>
+hipError_t launch_variant(const EvalPlan &p, const DeviceParticles &dp, double *part,
+                          const double4 *tiles, hipStream_t st, const KScalars &k,
+                          const double *table);
 
 // Table variants gather 8..192 table words per pair: one row per lane keeps
 // them out of spill territory; the arithmetic variants hold kRowsPerLane rows.
@@ -569,13 +636,14 @@ constexpr int rows_per_lane(int V) {
 
 template <int V>
 hipError_t launch_variant(const EvalPlan &p, const DeviceParticles &dp, double *part,
-                          hipStream_t st, const KScalars &k, const double *table) {
+                          const double4 *tiles, hipStream_t st, const KScalars &k,
+                          const double *table) {
     constexpr int NI = rows_per_lane(V);
     constexpr int UNR = (V == PV_R3 || V == PV_T3_NEAREST || V == PV_S1R2_NEAREST) ? 4 : 1;
     const int64_t rows_per_block = (int64_t)kBlock * NI;
     dim3 grid((unsigned)((p.nrows + rows_per_block - 1) / rows_per_block), (unsigned)p.splits);
     hipLaunchKernelGGL((pair_kernel_f64<V, NI, UNR>), grid, dim3(kBlock), 0, st, dp.jrec, dp.soft,
-                       table, part, k);
+                       tiles, table, part, k);
     return hipGetLastError();
 }
 
@@ -585,18 +653,22 @@ size_t partial_bytes(const EvalPlan &p) {
     return (size_t)p.splits * 3u * (size_t)std::max<int64_t>(p.nrows, 1) * sizeof(double);
 }
 
-// Mixed-path scratch lives after the partial sums in the same buffer:
-// jrel [N] float4, soft32 [N] float, centres [ceil(N/64)] double4.
-size_t mixed_extra_bytes(int64_t N) {
-    const int64_t tiles = (N + kMixedTile - 1) / kMixedTile;
-    return (size_t)N * sizeof(float4) + (size_t)N * sizeof(float) + (size_t)tiles * sizeof(double4) + 512;
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Scratch after the partial sums (same buffer, 256-byte aligned pieces):
+// tiles [ceil(N/64)] double4, then for the mixed path jrel [N] float4.
+size_t tile_extra_bytes(int64_t N, int precision) {
+    const int64_t nt = (N + kTile - 1) / kTile;
+    size_t b = align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4));
+    if (precision == STEPS_PREC_MIXED) b += align256((size_t)std::max<int64_t>(N, 1) * sizeof(float4));
+    return b + 256;
 }
 
 hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *partials,
                          double *F, hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_pair_end,
                          hipEvent_t ev_end) {
     hipError_t e;
-    if (ev_begin && (e = hipEventRecord(ev_begin, st)) != hipSuccess) return e;
+    if (!(p.nrows > 0 && p.N > 0) && ev_begin && (e = hipEventRecord(ev_begin, st)) != hipSuccess) return e;
     if (p.nrows > 0 && p.N > 0) {
         KScalars k;
         k.N = p.N;
@@ -613,31 +685,35 @@ hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *p
         k.image_cut = (((double)p.images) - 0.4) * p.L;
         k.c15 = 1.5;
         k.c1875 = 1.875;
-        if (p.precision == STEPS_PREC_MIXED && p.variant == PV_R3) {
-            char *extra = (char *)partials + ((partial_bytes(p) + 255) & ~(size_t)255);
-            float4 *jrel = (float4 *)extra;
-            float *soft32 = (float *)(jrel + p.N);
-            double4 *centres = (double4 *)(((uintptr_t)(soft32 + p.N) + 255) & ~(uintptr_t)255);
-            const int64_t tiles = (p.N + kMixedTile - 1) / kMixedTile;
-            hipLaunchKernelGGL(mixed_tiles_kernel, dim3((unsigned)tiles), dim3(kMixedTile), 0, st,
-                               dp.jrec, dp.soft, jrel, soft32, centres, p.N);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
+        const int64_t nt = (p.N + kTile - 1) / kTile;
+        const bool mixed = (p.precision == STEPS_PREC_MIXED && p.variant == PV_R3);
+        double4 *tiles = (double4 *)((char *)partials + align256(partial_bytes(p)));
+        float4 *jrel = (float4 *)((char *)tiles + align256((size_t)nt * sizeof(double4)));
+        if (mixed)
+            hipLaunchKernelGGL((tiles_kernel<true>), dim3((unsigned)nt), dim3(kTile), 0, st, dp.jrec, dp.soft,
+                               tiles, jrel, p.N);
+        else
+            hipLaunchKernelGGL((tiles_kernel<false>), dim3((unsigned)nt), dim3(kTile), 0, st, dp.jrec, dp.soft,
+                               tiles, jrel, p.N);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (ev_begin && (e = hipEventRecord(ev_begin, st)) != hipSuccess) return e;
+        if (mixed) {
             constexpr int NI = kRowsPerLaneMixed;
             const int64_t rows_per_block = (int64_t)kBlock * NI;
             dim3 grid((unsigned)((p.nrows + rows_per_block - 1) / rows_per_block), (unsigned)p.splits);
-            hipLaunchKernelGGL((pair_kernel_mixed_r3<NI>), grid, dim3(kBlock), 0, st, dp.jrec,
-                               dp.soft, jrel, soft32, centres, partials, k);
+            hipLaunchKernelGGL((pair_kernel_mixed_r3<NI, 8>), grid, dim3(kBlock), 0, st, dp.jrec, dp.soft, jrel,
+                               tiles, partials, k);
             e = hipGetLastError();
         } else {
             switch (p.variant) {
-            case PV_R3: e = launch_variant<PV_R3>(p, dp, partials, st, k, nullptr); break;
-            case PV_T3_NEAREST: e = launch_variant<PV_T3_NEAREST>(p, dp, partials, st, k, nullptr); break;
-            case PV_T3_EWALD: e = launch_variant<PV_T3_EWALD>(p, dp, partials, st, k, p.t3); break;
-            case PV_S1R2_NEAREST: e = launch_variant<PV_S1R2_NEAREST>(p, dp, partials, st, k, nullptr); break;
-            case PV_S1R2_NGP: e = launch_variant<PV_S1R2_NGP>(p, dp, partials, st, k, p.s1r2); break;
-            case PV_S1R2_CIC: e = launch_variant<PV_S1R2_CIC>(p, dp, partials, st, k, p.s1r2); break;
-            case PV_S1R2_TSC: e = launch_variant<PV_S1R2_TSC>(p, dp, partials, st, k, p.s1r2); break;
-            case PV_S1R2_IMAGES: e = launch_variant<PV_S1R2_IMAGES>(p, dp, partials, st, k, nullptr); break;
+            case PV_R3: e = launch_variant<PV_R3>(p, dp, partials, tiles, st, k, nullptr); break;
+            case PV_T3_NEAREST: e = launch_variant<PV_T3_NEAREST>(p, dp, partials, tiles, st, k, nullptr); break;
+            case PV_T3_EWALD: e = launch_variant<PV_T3_EWALD>(p, dp, partials, tiles, st, k, p.t3); break;
+            case PV_S1R2_NEAREST: e = launch_variant<PV_S1R2_NEAREST>(p, dp, partials, tiles, st, k, nullptr); break;
+            case PV_S1R2_NGP: e = launch_variant<PV_S1R2_NGP>(p, dp, partials, tiles, st, k, p.s1r2); break;
+            case PV_S1R2_CIC: e = launch_variant<PV_S1R2_CIC>(p, dp, partials, tiles, st, k, p.s1r2); break;
+            case PV_S1R2_TSC: e = launch_variant<PV_S1R2_TSC>(p, dp, partials, tiles, st, k, p.s1r2); break;
+            case PV_S1R2_IMAGES: e = launch_variant<PV_S1R2_IMAGES>(p, dp, partials, tiles, st, k, nullptr); break;
             default: return hipErrorInvalidValue;
             }
         }

@@ -11,6 +11,12 @@
 #include <cstring>
 #include <vector>
 
+// No FMA contraction in this file: the reference builds its tables on the host
+// without fused multiply-adds, and the grid positions must come out the same --
+// fused, (31 + 0.5)*(L/63) - L/2 is ~1e-14 instead of exactly 0 and the T3
+// correction at that node becomes the difference of two ~1e28 forces.
+#pragma clang fp contract(off)
+
 namespace steps {
 
 namespace {
@@ -21,35 +27,21 @@ constexpr double kSqrtPi = 1.7724538509055160272981674833411;  // ewald_space.cc
 // T3
 // --------------------------------------------------------------------------
 
-// The 24 proper rotations of the cube group O as (permutation, signs), listed
-// in the order ewald_space.cc:63-114 builds them (even permutations with an
-// even number of sign flips, then odd permutations with an odd number); the
-// fill order matters only where a node is its own image.
-struct Rot {
-    int perm[3];
-    int sign[3];
-};
-__constant__ Rot c_rot[24];
-
-std::vector<Rot> o_rotations() {
-    static const int pe[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
-    static const int po[3][3] = {{0, 2, 1}, {2, 1, 0}, {1, 0, 2}};
-    static const int sp[4][3] = {{1, 1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
-    static const int sn[4][3] = {{-1, -1, -1}, {-1, 1, 1}, {1, -1, 1}, {1, 1, -1}};
-    std::vector<Rot> r;
-    for (int p = 0; p < 3; ++p)
-        for (int s = 0; s < 4; ++s) {
-            Rot q;
-            for (int k = 0; k < 3; ++k) { q.perm[k] = pe[p][k]; q.sign[k] = sp[s][k]; }
-            r.push_back(q);
-        }
-    for (int p = 0; p < 3; ++p)
-        for (int s = 0; s < 4; ++s) {
-            Rot q;
-            for (int k = 0; k < 3; ++k) { q.perm[k] = po[p][k]; q.sign[k] = sn[s][k]; }
-            r.push_back(q);
-        }
-    return r;
+// The 24 proper rotations of the cube group O as (permutation, signs), in the
+// order ewald_space.cc:63-114 builds them (even permutations with an even
+// number of sign flips, then odd permutations with an odd number); the fill
+// order matters only where a node is its own image.  Computed from q in
+// registers, on host and device alike.
+__host__ __device__ inline void o_rotation(int q, int perm[3], int sign[3]) {
+    const int pe[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+    const int po[3][3] = {{0, 2, 1}, {2, 1, 0}, {1, 0, 2}};
+    const int sp[4][3] = {{1, 1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
+    const int sn[4][3] = {{-1, -1, -1}, {-1, 1, 1}, {1, -1, 1}, {1, 1, -1}};
+    const int p = (q % 12) / 4, s = q % 4;
+    for (int k = 0; k < 3; ++k) {
+        perm[k] = q < 12 ? pe[p][k] : po[p][k];
+        sign[k] = q < 12 ? sp[s][k] : sn[s][k];
+    }
 }
 
 // Integer image vectors n with |n_c| < R and |n|^2 < R^2 in the order of
@@ -73,21 +65,14 @@ std::vector<int4> image_vectors(double R) {
     return v;
 }
 
-__device__ __forceinline__ size_t t3_off(int Ng, int ix, int iy, int iz, int c) {
+__host__ __device__ inline size_t t3_off(int Ng, int ix, int iy, int iz, int c) {
     return ((size_t)((ix * Ng + iy) * Ng + iz) * 3u) + (size_t)c;
 }
 
-// ewald_space.cc:210-286 at one wedge node, then the O_h fill of :322-378.
-__global__ void t3_wedge_kernel(const int4 *__restrict__ wedge, int nw, const int4 *__restrict__ ridx,
-                                int nreal, const int4 *__restrict__ kidx, int nrec, double L,
-                                double alpha, double rel_cut, double rec_cut, int Ng,
-                                double *__restrict__ T) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nw) return;
-    const int4 w = wedge[t];
-    const double gs = L / (double)Ng;
-    const double r[3] = {((double)w.x + 0.5) * gs - L / 2.0, ((double)w.y + 0.5) * gs - L / 2.0,
-                         ((double)w.z + 0.5) * gs - L / 2.0};
+// ewald_space.cc:210-286: D(r) = (F_real + F_rec) - F_newton at one separation.
+__host__ __device__ inline void t3_node_correction(const double r[3], const int4 *ridx, int nreal,
+                                                   const int4 *kidx, int nrec, double L, double alpha,
+                                                   double rel_cut, double rec_cut, double D[3]) {
     double Fr[3] = {0, 0, 0}, Fk[3] = {0, 0, 0};
     const double V = L * L * L;
     const double two_alpha_over_sqrtpi = 2.0 * alpha / sqrt(kPi);
@@ -121,24 +106,40 @@ __global__ void t3_wedge_kernel(const int4 *__restrict__ wedge, int nw, const in
         Fk[1] += -kv[1] * coeff * s;
         Fk[2] += -kv[2] * coeff * s;
     }
-    double D[3] = {0, 0, 0};
+    D[0] = D[1] = D[2] = 0.0;
     const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
     if (r2 > 0.0) {
         const double rn = sqrt(r2);
         const double invr3 = 1.0 / (rn * rn * rn);
         for (int c = 0; c < 3; ++c) D[c] = (Fr[c] + Fk[c]) - (-r[c] * invr3);
     }
+}
+
+// One wedge node per thread, then the O_h fill of ewald_space.cc:322-378.
+__global__ void t3_wedge_kernel(const int4 *__restrict__ wedge, int nw, const int4 *__restrict__ ridx,
+                                int nreal, const int4 *__restrict__ kidx, int nrec, double L,
+                                double alpha, double rel_cut, double rec_cut, int Ng,
+                                double *__restrict__ T) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nw) return;
+    const int4 w = wedge[t];
+    const double gs = L / (double)Ng;
+    const double r[3] = {((double)w.x + 0.5) * gs - L / 2.0, ((double)w.y + 0.5) * gs - L / 2.0,
+                         ((double)w.z + 0.5) * gs - L / 2.0};
+    double D[3];
+    t3_node_correction(r, ridx, nreal, kidx, nrec, L, alpha, rel_cut, rec_cut, D);
     for (int c = 0; c < 3; ++c) T[t3_off(Ng, w.x, w.y, w.z, c)] = D[c];
     const int src[3] = {w.x, w.y, w.z};
     for (int q = 0; q < 24; ++q) {
+        int perm[3], sign[3];
+        o_rotation(q, perm, sign);
         int dst[3];
         double Dr[3];
         for (int d = 0; d < 3; ++d) {
-            const int sc = c_rot[q].perm[d];
-            int v = src[sc];
-            if (c_rot[q].sign[d] < 0) v = (Ng - 1) - v;
+            int v = src[perm[d]];
+            if (sign[d] < 0) v = (Ng - 1) - v;
             dst[d] = v;
-            Dr[d] = c_rot[q].sign[d] * D[sc];
+            Dr[d] = sign[d] * D[perm[d]];
         }
         for (int c = 0; c < 3; ++c) T[t3_off(Ng, dst[0], dst[1], dst[2], c)] = Dr[c];
         const int inv[3] = {(Ng - 1) - dst[0], (Ng - 1) - dst[1], (Ng - 1) - dst[2]};
@@ -322,7 +323,6 @@ int build_t3_table(int is_periodic, double L, double *table, int *ngrid) {
     for (int i = Ng / 2; i < Ng; ++i)
         for (int j = Ng / 2; j <= i; ++j)
             for (int k = Ng / 2; k <= j; ++k) wedge.push_back(make_int4(i, j, k, 0));
-    const std::vector<Rot> rot = o_rotations();
     const size_t nT = (size_t)Ng * Ng * Ng * 3;
     DevMem dT, dr, dk, dw;
     hipError_t e;
@@ -331,8 +331,7 @@ int build_t3_table(int is_periodic, double L, double *table, int *ngrid) {
         (e = hipMalloc(&dk.p, kidx.size() * sizeof(int4))) != hipSuccess ||
         (e = hipMalloc(&dw.p, wedge.size() * sizeof(int4))) != hipSuccess)
         return hip_fail(e, "T3 table alloc");
-    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_rot), rot.data(), sizeof(Rot) * 24)) != hipSuccess ||
-        (e = hipMemcpy(dr.p, ridx.data(), ridx.size() * sizeof(int4), hipMemcpyHostToDevice)) != hipSuccess ||
+    if ((e = hipMemcpy(dr.p, ridx.data(), ridx.size() * sizeof(int4), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(dk.p, kidx.data(), kidx.size() * sizeof(int4), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(dw.p, wedge.data(), wedge.size() * sizeof(int4), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemset(dT.p, 0, nT * sizeof(double))) != hipSuccess)

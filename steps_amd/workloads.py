@@ -126,7 +126,7 @@ def _gaussian_field(n: int, seed: int, stream: int, box: float) -> np.ndarray:
     amp = np.zeros_like(kk)
     ok = (kk > 0) & (kk <= knyq)
     amp[ok] = 1.0 / kk[ok]
-    f = np.fft.irfftn(wk * amp, s=(n, n, n))
+    f = np.fft.irfftn(wk * amp, s=(n, n, n), axes=(0, 1, 2))
     return f / np.sqrt(np.mean(f * f))
 
 
@@ -153,12 +153,18 @@ def t3_lattice(n_side: int = 64, seed: int = 1001, L: float = 100.0 / 0.6766,
 
 
 def s1r2_cylinder(n_omega: int = 128, n_phi: int = 64, n_z: int = 64, seed: int = 1002,
-                  Lz: float = 200.0, Lc: float = 100.0, omega_max: float = 2.0 * math.atan(10.0),
+                  Lz: float = 200.0, Lc: float = 100.0, rsim_over_lc: float = 10.0,
                   jitter: float = 0.1, soft_coeff: float = 0.05, is_periodic: int = 2) -> Workload:
-    """S^1 x R^2: shells R_a = Lc tan(omega_a/2), golden-angle phi offsets per
-    shell, z planes; isotropic Gaussian jitter of rms jitter*local spacing; z
-    wrapped; m ~ (1 + R^2/Lc^2)^2 normalised to Omega_m rho_c pi Rsim^2 Lz."""
-    Rsim = Lc * math.tan(0.5 * omega_max)
+    """S^1 x R^2: shells R_a = Lc tan(omega_a/2) with omega up to
+    2 atan(Rsim/Lc), golden-angle phi offsets per shell, z planes; isotropic
+    Gaussian jitter of rms jitter*local spacing; z wrapped;
+    m ~ (1 + R^2/Lc^2)^2 normalised to Omega_m rho_c pi Rsim^2 Lz.
+
+    Rsim is set exactly (Lc * rsim_over_lc): the reference's radial-table
+    trapezoid (utils.cc:174-205) evaluates sqrt(R^2 - Y^2) at Y = acc*(R/acc),
+    which is NaN whenever that product rounds above R."""
+    Rsim = Lc * rsim_over_lc
+    omega_max = 2.0 * math.atan(rsim_over_lc)
     a = np.arange(n_omega, dtype=np.float64)
     R_a = Lc * np.tan(0.5 * omega_max * (a + 0.5) / n_omega)
     R_lo = Lc * np.tan(0.5 * omega_max * a / n_omega)

@@ -1,0 +1,59 @@
+"""GPU kick-drift-kick steps (step.cc:100-312) against the oracle's step."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from steps_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("topology", ["r3", "t3"])
+def test_kdk_steps_match_oracle(steps, oracle, topology):
+    print(f"case kdk {topology}", flush=True)
+    if topology == "r3":
+        w = wl.r3_compactified(N=2048, seed=31)
+        kw = dict(cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere)
+        desc, keep = steps.make_desc(steps.TOPO_R3, **kw)
+        p = oracle.params(0, **kw)
+    else:
+        w = wl.t3_lattice(n_side=12, seed=5, is_periodic=1)
+        w.v[:] = np.random.default_rng(2).normal(size=w.v.shape) * 0.5
+        kw = dict(is_periodic=1, L=w.L, cosmology=1, comoving=1)
+        desc, keep = steps.make_desc(steps.TOPO_T3, **kw)
+        p = oracle.params(1, **kw)
+    H0 = wl.H0
+    cosmo = oracle.OracleCosmo(H0, wl.OMEGA_M, wl.OMEGA_L, 0.0, 1.0 - wl.OMEGA_M - wl.OMEGA_L)
+    lib = oracle.lib()
+    a = w.a_start
+    hub = lib.oracle_hubble(C.byref(cosmo), a)
+    h = 2e-4
+    # oracle state
+    xo = w.x.reshape(-1).copy()
+    vo = w.v.reshape(-1).copy()
+    Fo = oracle.forces(p, w.x, w.M, w.soft).reshape(-1).copy()
+    st = oracle.OracleStepState(a, hub, h, w.rho_part, 1.0)
+    # device state
+    sim = steps.Sim(desc, w.N)
+    sim.set_state(w.x, w.M, w.soft, w.v)
+    sim.forces()
+    e0 = sim.errmax(a, hub, w.rho_part)
+    e0o = lib.oracle_initial_errmax(C.byref(p), C.byref(st), w.N, oracle.dp(xo.copy()), oracle.dp(vo),
+                                    oracle.dp(Fo), oracle.dp(w.M))
+    assert e0 == pytest.approx(e0o, rel=1e-12)
+    for k in range(3):
+        a1, h1 = st.a, st.hubble
+        eo = lib.oracle_kdk_step(C.byref(p), C.byref(cosmo), C.byref(st), w.N, oracle.dp(xo), oracle.dp(vo),
+                                 oracle.dp(Fo), oracle.dp(w.M), oracle.dp(w.soft))
+        # the host advances a and H exactly as the oracle did, the device does the rest
+        eg = sim.step(h, a1, h1, st.a, st.hubble, w.rho_part)
+        x, v = sim.get_state()
+        sim.forces()
+        F = sim.get_forces()
+        assert np.allclose(x.reshape(-1), xo, rtol=0, atol=1e-12 * np.abs(xo).max()), k
+        assert np.allclose(v.reshape(-1), vo, rtol=1e-9, atol=1e-12 * np.abs(vo).max()), k
+        assert eg == pytest.approx(eo, rel=1e-9), k
+        rel = np.linalg.norm(F - Fo.reshape(-1, 3), axis=1) / np.linalg.norm(Fo.reshape(-1, 3), axis=1)
+        assert np.median(rel) <= 1e-12
+    sim.close()

@@ -1,0 +1,152 @@
+"""GPU parity of the periodic force variants and of the device-built tables.
+
+T3 (forces_cuda.cu:567-645) and S1R2 (:652-864) forces are compared with the
+oracle on the same inputs and the same tables (median relative error <= 1e-12);
+the tables the device builds are compared with the oracle's tables.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err, report
+from steps_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+L_T3 = 100.0 / 0.6766
+
+
+@pytest.fixture(scope="module")
+def t3_tab(oracle):
+    return oracle.t3_table(2, L_T3)
+
+
+def test_device_t3_table_matches_oracle(steps, t3_tab):
+    print("case t3 table is_periodic=2", flush=True)
+    T = steps.t3_table(2, L_T3)
+    assert T.shape == t3_tab.shape
+    scale = np.abs(t3_tab).max()
+    d = np.abs(T - t3_tab).max() / scale
+    print(f"[T3 table device vs oracle] max |dT|/max|T| = {d:.3e}", flush=True)
+    assert d <= 1e-12
+
+
+def test_device_s1r2_table_matches_oracle(steps, oracle):
+    print("case s1r2 table is_periodic=2 Lz=200 Rsim=1000", flush=True)
+    T, shape = steps.s1r2_table(2, 200.0, 1000.0)
+    To, shape_o = oracle.s1r2_table(2, 200.0, 1000.0)
+    assert shape == shape_o == (1440, 128)
+    d = np.abs(T - To).max() / np.abs(To).max()
+    print(f"[S1R2 table device vs oracle] max |dT|/max|T| = {d:.3e}", flush=True)
+    # device and host libm (erfc/exp/sin/cos) differ by ulps; the table is a
+    # difference of near-equal forces, which amplifies that to ~1e-12
+    assert d <= 1e-10
+
+
+def test_device_radial_table_matches_oracle(steps, oracle):
+    for (R, Lz, size, acc) in [(1000.0, 100.0, 1000, 1000), (500.0, 260.0, 500, 7500)]:
+        T = steps.radial_table(R, Lz, size, acc)
+        To = oracle.radial_table(R, Lz, size, acc)
+        assert np.allclose(T, To, rtol=1e-12, atol=0), np.abs(T / To - 1).max()
+
+
+@pytest.mark.parametrize("is_periodic", [2, 1])
+def test_t3_forces(steps, oracle, t3_tab, is_periodic):
+    print(f"case t3 forces n_side=16 is_periodic={is_periodic}", flush=True)
+    w = wl.t3_lattice(n_side=16, seed=1001, is_periodic=is_periodic)
+    tab = t3_tab if is_periodic >= 2 else None
+    d, keep = steps.make_desc(steps.TOPO_T3, is_periodic=is_periodic, L=w.L, cosmology=1, comoving=1,
+                              t3_table=tab)
+    F = steps.force_eval(d, w.x, w.M, w.soft)
+    p = oracle.params(1, is_periodic=is_periodic, L=w.L, cosmology=1, comoving=1, t3_table=tab, interp_order=4)
+    Fo = oracle.forces(p, w.x, w.M, w.soft)
+    e = rel_err(F, Fo)
+    report(f"T3 is_periodic={is_periodic} N={w.N}", e)
+    assert np.median(e) <= 1e-12
+    assert np.percentile(e, 99) <= 1e-9
+
+
+def test_t3_full_size_sampled_rows(steps, oracle, t3_tab):
+    """The 64^3 T3 configuration at full size, checked on 96 sampled rows."""
+    print("case t3 forces n_side=64 (N=262144) sampled rows", flush=True)
+    w = wl.t3_lattice(n_side=64, seed=1001)
+    d, keep = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=w.L, t3_table=t3_tab)
+    sim = steps.Sim(d, w.N)
+    sim.set_state(w.x, w.M, w.soft)
+    sim.forces()
+    F = sim.get_forces()
+    ms = sim.last_kernel_ms()
+    print(f"[T3 N=262144] pair kernel {ms[0]:.1f} ms -> {w.N ** 2 / ms[0] / 1e6:.1f} G pair/s", flush=True)
+    sim.close()
+    rows = np.linspace(0, w.N - 1, 96).astype(np.int32)
+    p = oracle.params(1, is_periodic=2, L=w.L, t3_table=t3_tab, interp_order=4)
+    Fo = oracle.forces_rows(p, w.x, w.M, w.soft, rows)
+    e = rel_err(F[rows], Fo)
+    report("T3 N=262144 sampled", e)
+    assert np.median(e) <= 1e-12
+
+
+@pytest.fixture(scope="module")
+def cyl(oracle):
+    w = wl.s1r2_cylinder(n_omega=32, n_phi=8, n_z=16, seed=1002)
+    T, shape = oracle.s1r2_table(2, w.L, w.Rsim)
+    return w, T, shape
+
+
+@pytest.mark.parametrize("order", [0, 2, 4])
+def test_s1r2_lookup_forces(steps, oracle, cyl, order):
+    print(f"case s1r2 lookup order={order}", flush=True)
+    w, T, shape = cyl
+    kw = dict(is_periodic=2, L=w.L, Rsim=w.Rsim, cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere,
+              s1r2_lookup=1, interp_order=order, s1r2_table=T, s1r2_shape=shape)
+    d, keep = steps.make_desc(steps.TOPO_S1R2, **kw)
+    F = steps.force_eval(d, w.x, w.M, w.soft)
+    Fo = oracle.forces(oracle.params(2, **kw), w.x, w.M, w.soft)
+    e = rel_err(F, Fo)
+    report(f"S1R2 lookup order={order} N={w.N}", e)
+    assert np.median(e) <= 1e-12
+    assert np.percentile(e, 99) <= 1e-9
+
+
+@pytest.mark.parametrize("mode", ["nearest", "images"])
+def test_s1r2_radial_corrected_forces(steps, oracle, cyl, mode):
+    print(f"case s1r2 {mode}", flush=True)
+    w, _, _ = cyl
+    if mode == "nearest":
+        is_p, lookup = 1, 1
+        radial = oracle.radial_table(w.Rsim, 0.5 * w.L, 1000, 1000)    # main.cc:1302
+    else:
+        is_p, lookup = 2, 0
+        radial = oracle.radial_table(w.Rsim, w.L * (is_p + 1 - 0.4), 1000, 1000)  # main.cc:1282
+    kw = dict(is_periodic=is_p, L=w.L, Rsim=w.Rsim, cosmology=1, comoving=1,
+              mass_in_unit_sphere=w.mass_in_unit_sphere, s1r2_lookup=lookup, radial_table=radial)
+    d, keep = steps.make_desc(steps.TOPO_S1R2, **kw)
+    F = steps.force_eval(d, w.x, w.M, w.soft)
+    Fo = oracle.forces(oracle.params(2, cut_inclusive=0, **kw), w.x, w.M, w.soft)
+    e = rel_err(F, Fo)
+    report(f"S1R2 {mode} N={w.N}", e)
+    assert np.median(e) <= 1e-12
+
+
+def test_s1r2_full_size_sampled_rows(steps, oracle):
+    """The 524288-particle S1R2 configuration (CIC and TSC), 64 sampled rows."""
+    w = wl.s1r2_cylinder(seed=1002)
+    T, shape = steps.s1r2_table(2, w.L, w.Rsim)
+    rows = np.linspace(0, w.N - 1, 64).astype(np.int32)
+    for order in (2, 4):
+        print(f"case s1r2 N={w.N} order={order} sampled rows", flush=True)
+        kw = dict(is_periodic=2, L=w.L, Rsim=w.Rsim, cosmology=1, comoving=1,
+                  mass_in_unit_sphere=w.mass_in_unit_sphere, s1r2_lookup=1, interp_order=order, s1r2_table=T,
+                  s1r2_shape=shape)
+        d, keep = steps.make_desc(steps.TOPO_S1R2, **kw)
+        sim = steps.Sim(d, w.N)
+        sim.set_state(w.x, w.M, w.soft)
+        sim.forces()
+        F = sim.get_forces()
+        ms = sim.last_kernel_ms()
+        print(f"[S1R2 N={w.N} order={order}] pair kernel {ms[0]:.1f} ms -> {w.N ** 2 / ms[0] / 1e6:.1f} G pair/s",
+              flush=True)
+        sim.close()
+        Fo = oracle.forces_rows(oracle.params(2, **kw), w.x, w.M, w.soft, rows)
+        e = rel_err(F[rows], Fo)
+        report(f"S1R2 N={w.N} order={order} sampled", e)
+        assert np.median(e) <= 1e-12
