@@ -22,7 +22,10 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 OBJDIR = ROOT / "_build" / "steps_amd"
 LIBNAME = "libsteps_amd.so"
-SHIMNAME = "libsteps_shim_test.so"
+# the drop-in shim built once per reference build variant, linked with the
+# test harness that plays the role of StePS's globals (tests/shim)
+SHIM_VARIANTS = {"r3": [], "t3": ["-DPERIODIC"], "s1r2": ["-DPERIODIC_Z", "-DEWALD_INTERPOLATION_ORDER=4"],
+                 "s1r2_nolookup": ["-DPERIODIC_Z", "-DPERIODIC_Z_NOLOOKUP"]}
 
 HIP_SOURCES = ["pair_kernels.hip", "engine.hip", "kdk_kernels.hip", "table_kernels.hip"]
 
@@ -74,16 +77,27 @@ def build(verbose: bool = False) -> Path:
     tmp = LIBDIR / (LIBNAME + ".tmp")
     _run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)], verbose)
     os.replace(tmp, out)
-    # drop-in shim + the globals a reference main.cc would define (test harness)
-    shim = CSRC / "shim" / "steps_forces_shim.cc"
-    harness = CSRC / "shim" / "reference_globals_harness.cc"
-    if shim.exists() and harness.exists():
-        sout = LIBDIR / SHIMNAME
-        stmp = LIBDIR / (SHIMNAME + ".tmp")
-        _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", *inc, str(shim), str(harness),
-              f"-L{LIBDIR}", "-lsteps_amd", "-Wl,-rpath,$ORIGIN", "-o", str(stmp)], verbose)
-        os.replace(stmp, sout)
+    build_shims(verbose)
     return out
+
+
+def build_shims(verbose: bool = False) -> None:
+    """The C++ drop-in shim (host code, g++) for each reference build variant,
+    linked with the globals harness of tests/shim into libsteps_shim_<v>.so."""
+    shim = CSRC / "shim" / "steps_forces_shim.cc"
+    harness = ROOT / "tests" / "shim" / "reference_globals_harness.cc"
+    jobs = []
+    for name, flags in SHIM_VARIANTS.items():
+        out = LIBDIR / f"libsteps_shim_{name}.so"
+        tmp = LIBDIR / f"libsteps_shim_{name}.so.tmp"
+        jobs.append((["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", *flags,
+                      f"-I{ROOT / 'include'}", str(shim), str(harness), f"-L{LIBDIR}", "-lsteps_amd",
+                      "-Wl,-rpath,$ORIGIN", "-o", str(tmp)], tmp, out))
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        for f in [ex.submit(_run, j[0], verbose) for j in jobs]:
+            f.result()
+    for _, tmp, out in jobs:
+        os.replace(tmp, out)
 
 
 if __name__ == "__main__":
