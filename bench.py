@@ -133,6 +133,32 @@ def load_traffic(name: str):
     return None
 
 
+def serving_leg(nreq: int) -> dict:
+    """64 resident simulations (R^3 and S^1 x R^2 alternating, N log-uniform in
+    [8192, 524288]) receive a seeded Poisson stream (200 req/s) of KDK step
+    requests, half FP64 and half mixed where the topology has a mixed path;
+    served by the continuous-batching server and, for comparison, one
+    request at a time on resident simulations.  The final (x, v) of every
+    simulation must be bitwise equal between the two."""
+    from steps_amd import serving as sv
+    sims = sv.make_ensemble()
+    reqs = sv.make_requests(sims, nreq=nreq, mixed_ok=sv.r3_only_mixed)
+    print(f"[bench] serving: {len(sims)} sims, {nreq} requests", file=sys.stderr, flush=True)
+    srv, fin_s = sv.run_server(sims, reqs)
+    print(f"[bench] serving: server done {srv}", file=sys.stderr, flush=True)
+    seq, fin_q = sv.run_sequential(sims, reqs)
+    print(f"[bench] serving: sequential done {seq}", file=sys.stderr, flush=True)
+    same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(fin_s, fin_q))
+    pairs = float(sum(sims[r.sim][0].N ** 2 for r in reqs))
+    return {"workload": "64 sims R3/S1R2, N log-uniform [8192, 524288], Poisson 200 req/s seed 1004, "
+                        "KDK steps, FP64 / mixed 50/50 on R3",
+            "requests": nreq, "pair_interactions": pairs,
+            "steps_per_s": srv["requests_per_s"], "p50_ms": srv["p50_ms"], "p99_ms": srv["p99_ms"],
+            "server": srv, "sequential": seq,
+            "speedup_vs_sequential": srv["requests_per_s"] / seq["requests_per_s"],
+            "results_identical": bool(same)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +167,9 @@ def main() -> int:
     ap.add_argument("--workload", default="r3_2m", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serving-requests", type=int, default=160,
+                    help="length of the seeded Poisson request stream of the serving leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -222,6 +251,9 @@ def main() -> int:
             "roofline": {"bound": "valu", "achieved": ach_m, "peak": PEAK_FP32_VECTOR_TFLOPS,
                          "unit": "TFLOP/s", "frac": ach_m / PEAK_FP32_VECTOR_TFLOPS},
         }
+
+    if not args.no_serving:
+        out["serving"] = serving_leg(args.serving_requests)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)

@@ -100,6 +100,11 @@ typedef struct steps_sim steps_sim;
 
 int steps_sim_create(const steps_force_desc *d, int64_t N, steps_sim **out);
 int steps_sim_destroy(steps_sim *s);
+/* Switch the precision of later evaluations (STEPS_PREC_FP64 / _MIXED).  No
+ * counterpart in the reference, which fixes the precision at compile time
+ * (USE_SINGLE_PRECISION, global_variables.h:26-32); scratch of an R3 sim is
+ * sized for both paths, so switching allocates nothing. */
+int steps_sim_set_precision(steps_sim *s, int32_t precision);
 /* x, v AoS [3N] (v may be NULL: zero), M and soft [N]; host memory. */
 int steps_sim_set_state(steps_sim *s, const double *x, const double *v, const double *M,
                         const double *soft);
@@ -143,6 +148,65 @@ int steps_s1r2_table(int32_t is_periodic, double L, double Rsim, double *table,
                      int32_t *nrho, int32_t *nz);
 /* finite-cylinder radial force table, utils.cc:162-228 */
 int steps_radial_table(double R, double Lz, int32_t size, int32_t accuracy, double *table);
+
+/* ---- serving loop ----------------------------------------------------------
+ * No counterpart in the reference (one simulation per MPI job there).  A
+ * server keeps an ensemble of independent simulations, takes force and step
+ * requests from any thread, and runs them on one device with continuous
+ * batching: simulations are admitted into HBM by memory budget (idle ones are
+ * evicted LRU to host memory and re-admitted with their state), up to
+ * `streams` requests run concurrently (one per simulation at a time, in
+ * submission order per simulation), and each simulation's force/step sequence
+ * is replayed from a captured hipGraph.  Results do not depend on the mix. */
+typedef struct steps_server steps_server;
+
+typedef struct steps_server_config {
+    int32_t device;
+    int32_t streams;           /* concurrent requests on the device, 1..16 (default 4) */
+    int32_t use_graphs;        /* 1: replay captured per-simulation graphs (default)    */
+    int64_t hbm_budget_bytes;  /* resident simulations' state + scratch (default 200 GiB) */
+} steps_server_config;
+
+enum steps_request_kind { STEPS_REQ_FORCE = 0, STEPS_REQ_STEP = 1 };
+
+typedef struct steps_request_result {
+    int32_t status;            /* STEPS_OK or an error code                        */
+    int32_t kind;
+    int64_t sim;
+    int64_t ticket;
+    double errmax;             /* STEP requests: max_i |A_i| / h_i after the step  */
+    double t_submit, t_start, t_end;  /* seconds on steps_server_clock()           */
+} steps_request_result;
+
+typedef struct steps_server_stats {
+    int64_t submitted, completed, failed;
+    int64_t admissions, evictions, graph_captures;
+    int64_t resident_sims, bytes_resident, max_backlog;
+} steps_server_stats;
+
+void steps_server_config_init(steps_server_config *c);
+int steps_server_create(const steps_server_config *c, steps_server **out);
+/* Abandons pending requests (they finish with STEPS_ERR_STATE), waits for the
+ * ones on the device, frees everything. */
+int steps_server_destroy(steps_server *s);
+/* Registers a simulation (host arrays are copied; v may be NULL).  It becomes
+ * resident when it first has work.  Fails with STEPS_ERR_ADMISSION if it
+ * could never fit the budget. */
+int steps_server_add_sim(steps_server *s, const steps_force_desc *d, int64_t N, const double *x,
+                         const double *v, const double *M, const double *soft, int64_t *sim_id);
+int steps_server_remove_sim(steps_server *s, int64_t sim_id);
+/* Device bytes the simulation takes while resident (state + scratch + tables). */
+int64_t steps_server_sim_bytes(steps_server *s, int64_t sim_id);
+/* kind: STEPS_REQ_FORCE (all rows) or STEPS_REQ_STEP (p required); precision
+ * per request (enum steps_precision). */
+int steps_server_submit(steps_server *s, int64_t sim_id, int32_t kind, int32_t precision,
+                        const steps_step_params *p, int64_t *ticket);
+int steps_server_wait(steps_server *s, int64_t ticket, steps_request_result *out);
+int steps_server_drain(steps_server *s);
+/* After the simulation's requests finished: x, v, F (AoS [3N], any may be NULL). */
+int steps_server_get_state(steps_server *s, int64_t sim_id, double *x, double *v, double *F);
+int steps_server_get_stats(steps_server *s, steps_server_stats *out);
+double steps_server_clock(steps_server *s);
 
 /* ---- softening set-up (utils.cc:59-82) ------------------------------------ */
 int steps_softening_lengths(const double *M, int64_t N, double particle_radius, double *soft,

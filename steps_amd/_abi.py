@@ -29,11 +29,16 @@ PREC_FP64, PREC_MIXED = 0, 1
 # every symbol include/steps_force.h declares (checked by tests/test_abi.py)
 EXPORTED = [
     "steps_force_desc_init", "steps_device_count", "steps_last_error", "steps_version",
-    "steps_force_eval", "steps_sim_create", "steps_sim_destroy", "steps_sim_set_state",
+    "steps_force_eval", "steps_sim_create", "steps_sim_destroy", "steps_sim_set_precision", "steps_sim_set_state",
     "steps_sim_set_positions", "steps_sim_get_state", "steps_sim_forces", "steps_sim_get_forces",
     "steps_sim_synchronize", "steps_sim_last_kernel_ms", "steps_sim_step", "steps_sim_errmax",
     "steps_t3_table", "steps_s1r2_table", "steps_radial_table", "steps_softening_lengths",
+    "steps_server_config_init", "steps_server_create", "steps_server_destroy", "steps_server_add_sim",
+    "steps_server_remove_sim", "steps_server_sim_bytes", "steps_server_submit", "steps_server_wait", "steps_server_drain",
+    "steps_server_get_state", "steps_server_get_stats", "steps_server_clock",
 ]
+
+REQ_FORCE, REQ_STEP = 0, 1
 
 
 class StepsError(RuntimeError):
@@ -79,6 +84,22 @@ class StepParams(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("streams", C.c_int32), ("use_graphs", C.c_int32),
+                ("hbm_budget_bytes", C.c_int64)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("kind", C.c_int32), ("sim", C.c_int64), ("ticket", C.c_int64),
+                ("errmax", C.c_double), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_end", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("submitted", "completed", "failed", "admissions", "evictions",
+                                        "graph_captures", "resident_sims", "bytes_resident", "max_backlog")]
+
+
 _lib = None
 _DP = C.POINTER(C.c_double)
 
@@ -100,6 +121,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "steps_version": (C.c_char_p, []),
         "steps_force_eval": (C.c_int, [C.POINTER(ForceDesc), i64, _DP, _DP, _DP, _DP, i64, i64]),
         "steps_sim_create": (C.c_int, [C.POINTER(ForceDesc), i64, C.POINTER(vp)]),
+        "steps_sim_set_precision": (C.c_int, [vp, C.c_int32]),
         "steps_sim_destroy": (C.c_int, [vp]),
         "steps_sim_set_state": (C.c_int, [vp, _DP, _DP, _DP, _DP]),
         "steps_sim_set_positions": (C.c_int, [vp, _DP]),
@@ -114,6 +136,18 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "steps_s1r2_table": (C.c_int, [i32, C.c_double, C.c_double, _DP, C.POINTER(i32), C.POINTER(i32)]),
         "steps_radial_table": (C.c_int, [C.c_double, C.c_double, i32, i32, _DP]),
         "steps_softening_lengths": (C.c_int, [_DP, i64, C.c_double, _DP, _DP, _DP]),
+        "steps_server_config_init": (None, [C.POINTER(ServerConfig)]),
+        "steps_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(vp)]),
+        "steps_server_destroy": (C.c_int, [vp]),
+        "steps_server_add_sim": (C.c_int, [vp, C.POINTER(ForceDesc), i64, _DP, _DP, _DP, _DP, C.POINTER(i64)]),
+        "steps_server_remove_sim": (C.c_int, [vp, i64]),
+        "steps_server_sim_bytes": (C.c_int64, [vp, i64]),
+        "steps_server_submit": (C.c_int, [vp, i64, i32, i32, C.POINTER(StepParams), C.POINTER(i64)]),
+        "steps_server_wait": (C.c_int, [vp, i64, C.POINTER(RequestResult)]),
+        "steps_server_drain": (C.c_int, [vp]),
+        "steps_server_get_state": (C.c_int, [vp, i64, _DP, _DP, _DP]),
+        "steps_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
+        "steps_server_clock": (C.c_double, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -223,6 +257,9 @@ class Sim:
         except Exception:
             pass
 
+    def set_precision(self, precision: int):
+        _check(self._lib.steps_sim_set_precision(self.h, precision))
+
     def set_state(self, x, M, soft, v=None):
         x, M, soft = f64(x).reshape(-1), f64(M), f64(soft)
         vv = None if v is None else f64(v).reshape(-1)
@@ -300,3 +337,72 @@ def softening_lengths(M, particle_radius: float):
     _check(load_library().steps_softening_lengths(_ptr(M), M.size, particle_radius, _ptr(h), C.byref(mmin),
                                                   C.byref(rp)))
     return h, mmin.value, rp.value
+
+
+class Server:
+    """Continuous-batching request loop (steps_server)."""
+
+    def __init__(self, device=0, streams=4, use_graphs=True, hbm_budget_bytes=200 << 30):
+        self._lib = load_library()
+        cfg = ServerConfig()
+        self._lib.steps_server_config_init(C.byref(cfg))
+        cfg.device, cfg.streams, cfg.use_graphs, cfg.hbm_budget_bytes = device, streams, int(use_graphs), \
+            int(hbm_budget_bytes)
+        h = C.c_void_p()
+        _check(self._lib.steps_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def close(self):
+        if self.h:
+            self._lib.steps_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_sim(self, desc: ForceDesc, x, M, soft, v=None) -> int:
+        x, M, soft = f64(x).reshape(-1), f64(M), f64(soft)
+        vv = None if v is None else f64(v).reshape(-1)
+        sid = C.c_int64()
+        _check(self._lib.steps_server_add_sim(self.h, C.byref(desc), M.size, _ptr(x), _ptr(vv), _ptr(M),
+                                              _ptr(soft), C.byref(sid)))
+        self.sizes[sid.value] = M.size
+        return sid.value
+
+    def sim_bytes(self, sid: int) -> int:
+        return int(self._lib.steps_server_sim_bytes(self.h, sid))
+
+    def remove_sim(self, sid: int):
+        _check(self._lib.steps_server_remove_sim(self.h, sid))
+
+    def submit(self, sid: int, kind: int, precision: int = PREC_FP64, step: StepParams | None = None) -> int:
+        t = C.c_int64()
+        _check(self._lib.steps_server_submit(self.h, sid, kind, precision,
+                                             C.byref(step) if step is not None else None, C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> RequestResult:
+        r = RequestResult()
+        _check(self._lib.steps_server_wait(self.h, ticket, C.byref(r)))
+        return r
+
+    def drain(self):
+        _check(self._lib.steps_server_drain(self.h))
+
+    def get_state(self, sid: int):
+        n = self.sizes[sid]
+        x, v, F = np.empty(3 * n), np.empty(3 * n), np.empty(3 * n)
+        _check(self._lib.steps_server_get_state(self.h, sid, _ptr(x), _ptr(v), _ptr(F)))
+        return x.reshape(-1, 3), v.reshape(-1, 3), F.reshape(-1, 3)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(self._lib.steps_server_get_stats(self.h, C.byref(s)))
+        return {n: getattr(s, n) for n, _ in ServerStats._fields_}
+
+    def clock(self) -> float:
+        return float(self._lib.steps_server_clock(self.h))

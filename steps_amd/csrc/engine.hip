@@ -4,7 +4,7 @@
 // wrappers (forces_cuda.cu:878-1117, which cudaMalloc/copy/free six buffers and
 // the whole Ewald table on every call), device buffers are grown once and
 // reused, and tables are uploaded once per content.
-#include "internal.h"
+#include "sim_internal.h"
 
 #include <algorithm>
 #include <cmath>
@@ -13,7 +13,6 @@
 #include <mutex>
 #include <new>
 #include <string>
-#include <vector>
 
 namespace steps {
 
@@ -35,14 +34,6 @@ int hip_fail(hipError_t e, const char *what) {
                                                                         : STEPS_ERR_DEVICE;
 }
 
-namespace {
-
-#define STEPS_HIP(call, what)                                  \
-    do {                                                       \
-        hipError_t e_ = (call);                                \
-        if (e_ != hipSuccess) return hip_fail(e_, (what));     \
-    } while (0)
-
 int cu_count(int device) {
     int n = 0;
     if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0)
@@ -50,34 +41,28 @@ int cu_count(int device) {
     return n;
 }
 
-// Grow-only device buffer.
-struct DevBuf {
-    void *ptr = nullptr;
-    size_t cap = 0;
-    int reserve(size_t bytes, const char *what) {
-        if (bytes <= cap && ptr) return STEPS_OK;
-        if (ptr) (void)hipFree(ptr);
+int DevBuf::reserve(size_t bytes, const char *what) {
+    if (bytes <= cap && ptr) return STEPS_OK;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
         ptr = nullptr;
-        cap = 0;
-        size_t want = std::max<size_t>(bytes, 256);
-        hipError_t e = hipMalloc(&ptr, want);
-        if (e != hipSuccess) {
-            ptr = nullptr;
-            return hip_fail(e, what);
-        }
-        cap = want;
-        return STEPS_OK;
+        return hip_fail(e, what);
     }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-    }
-    template <class T> T *as() const { return static_cast<T *>(ptr); }
-};
+    cap = want;
+    return STEPS_OK;
+}
 
-// Cheap content fingerprint of a host table (size + strided sample), so a
-// table that changes between calls is re-uploaded while an unchanged one is not.
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+namespace {
 uint64_t fingerprint(const double *h, size_t n) {
     uint64_t f = 1469598103934665603ull ^ (uint64_t)n;
     const size_t step = std::max<size_t>(1, n / 4096);
@@ -93,29 +78,22 @@ uint64_t fingerprint(const double *h, size_t n) {
     }
     return f;
 }
+}  // namespace
 
-struct TableCopy {
-    const double *host = nullptr;
-    size_t n = 0;
-    uint64_t fp = 0;
-    DevBuf dev;
-    int sync(const double *h, size_t count, hipStream_t st, const char *what) {
-        if (!h || count == 0) return STEPS_OK;
-        const uint64_t f = fingerprint(h, count);
-        if (h == host && count == n && f == fp && dev.ptr) return STEPS_OK;
-        int rc = dev.reserve(count * sizeof(double), what);
-        if (rc) return rc;
-        STEPS_HIP(hipMemcpyAsync(dev.ptr, h, count * sizeof(double), hipMemcpyHostToDevice, st), what);
-        STEPS_HIP(hipStreamSynchronize(st), what);
-        host = h;
-        n = count;
-        fp = f;
-        return STEPS_OK;
-    }
-    const double *ptr() const { return dev.as<const double>(); }
-};
+int TableCopy::sync(const double *h, size_t count, hipStream_t st, const char *what) {
+    if (!h || count == 0) return STEPS_OK;
+    const uint64_t f = fingerprint(h, count);
+    if (h == host && count == n && f == fp && dev.ptr) return STEPS_OK;
+    int rc = dev.reserve(count * sizeof(double), what);
+    if (rc) return rc;
+    STEPS_HIP(hipMemcpyAsync(dev.ptr, h, count * sizeof(double), hipMemcpyHostToDevice, st), what);
+    STEPS_HIP(hipStreamSynchronize(st), what);
+    host = h;
+    n = count;
+    fp = f;
+    return STEPS_OK;
+}
 
-// Build the launch plan of one evaluation and validate the descriptor.
 int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_max, EvalPlan &p) {
     if (N < 0) return fail(STEPS_ERR_ARG, "N < 0");
     if (N > (int64_t)INT32_MAX * 4) return fail(STEPS_ERR_ARG, "N too large");
@@ -216,16 +194,6 @@ size_t scratch_bytes(const EvalPlan &p) {
     return ((partial_bytes(p) + 255) & ~(size_t)255) + tile_extra_bytes(p.N, p.precision);
 }
 
-// State of the one-shot entry point (one device at a time, under a lock).
-struct OneShot {
-    std::mutex mu;
-    int device = -1;
-    hipStream_t stream = nullptr;
-    DevBuf x, M, soft, jrec, jsoft, part, F;
-    TableCopy t3, s1r2, radial;
-};
-OneShot g_oneshot;
-
 int set_device(int device) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
@@ -241,24 +209,191 @@ int check_finite(const double *x, int64_t n) {
     return STEPS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// resident simulation
+// ---------------------------------------------------------------------------
+
+int sim_create(const steps_force_desc &d, int64_t N, bool both_precisions, steps_sim **out) {
+    *out = nullptr;
+    if (N <= 0) return fail(STEPS_ERR_ARG, "N must be positive");
+    EvalPlan p;
+    steps_force_desc dd = d;
+    if (both_precisions && d.topology == STEPS_TOPO_R3) dd.precision = STEPS_PREC_MIXED;
+    int rc = make_plan(dd, N, 0, N - 1, p);
+    if (rc) return rc;
+    if ((rc = set_device(d.device))) return rc;
+    steps_sim *s = new (std::nothrow) steps_sim();
+    if (!s) return fail(STEPS_ERR_NOMEM, "host allocation");
+    s->device = d.device;
+    s->desc = d;
+    s->N = N;
+    s->both_precisions = both_precisions;
+    const size_t nb = (size_t)N * sizeof(double);
+    auto cleanup = [&](int code) {
+        sim_destroy(s);
+        return code;
+    };
+    if ((rc = s->jrec.reserve((size_t)N * sizeof(double4), "alloc jrec")) ||
+        (rc = s->jsoft.reserve(nb, "alloc soft")) || (rc = s->v.reserve(3 * nb, "alloc v")) ||
+        (rc = s->F.reserve(3 * nb, "alloc F")) || (rc = s->part.reserve(scratch_bytes(p), "alloc partials")) ||
+        (rc = s->stage.reserve(std::max(3 * nb, (size_t)1024), "alloc stage")) ||
+        (rc = s->blockmax.reserve((size_t)(kdk_reduce_blocks(N) + 1) * sizeof(double), "alloc blockmax")) ||
+        (rc = s->dparams.reserve(sizeof(KdkParams), "alloc step params")))
+        return cleanup(rc);
+    hipError_t e;
+    if ((e = hipHostMalloc((void **)&s->hparams, sizeof(KdkParams) + sizeof(double), hipHostMallocDefault)) !=
+        hipSuccess) {
+        s->hparams = nullptr;
+        return cleanup(hip_fail(e, "pinned step block"));
+    }
+    s->herrmax = (double *)(s->hparams + 1);
+    std::memset(s->hparams, 0, sizeof(KdkParams) + sizeof(double));
+    if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess ||
+        (e = hipEventCreate(&s->ev2)) != hipSuccess)
+        return cleanup(hip_fail(e, "stream/event create"));
+    if (p.variant == PV_T3_EWALD &&
+        (rc = s->t3.sync(d.t3_table, (size_t)d.t3_ngrid * d.t3_ngrid * d.t3_ngrid * 3, s->stream, "T3 table")))
+        return cleanup(rc);
+    if (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC &&
+        (rc = s->s1r2.sync(d.s1r2_table, (size_t)d.s1r2_nrho * d.s1r2_nz * 2, s->stream, "S1R2 table")))
+        return cleanup(rc);
+    if (p.ext_mode == 3 &&
+        (rc = s->radial.sync(d.radial_table, (size_t)d.radial_table_size, s->stream, "radial table")))
+        return cleanup(rc);
+    // the sim owns device copies: host table pointers are not kept
+    s->desc.t3_table = nullptr;
+    s->desc.s1r2_table = nullptr;
+    s->desc.radial_table = nullptr;
+    s->dp = DeviceParticles{s->jrec.as<double4>(), s->jsoft.as<double>(), N};
+    if ((e = hipMemsetAsync(s->v.ptr, 0, 3 * nb, s->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(s->F.ptr, 0, 3 * nb, s->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(s->stream)) != hipSuccess)
+        return cleanup(hip_fail(e, "initialise sim"));
+    *out = s;
+    return STEPS_OK;
+}
+
+void sim_destroy(steps_sim *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto &row : s->graph)
+        for (auto &g : row)
+            if (g) (void)hipGraphExecDestroy(g);
+    s->jrec.release(); s->jsoft.release(); s->v.release(); s->F.release(); s->part.release();
+    s->stage.release(); s->blockmax.release(); s->dparams.release();
+    s->t3.dev.release(); s->s1r2.dev.release(); s->radial.dev.release();
+    if (s->hparams) (void)hipHostFree(s->hparams);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->ev2) (void)hipEventDestroy(s->ev2);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int sim_plan(steps_sim *s, int64_t id_min, int64_t id_max, int precision, EvalPlan &p) {
+    steps_force_desc d = s->desc;  // tables live on the device; the checks only need non-null
+    d.t3_table = s->t3.ptr();
+    d.s1r2_table = s->s1r2.ptr();
+    d.radial_table = s->radial.ptr();
+    d.precision = precision;
+    int rc = make_plan(d, s->N, id_min, id_max, p);
+    if (rc) return rc;
+    if (scratch_bytes(p) > s->part.cap) return fail(STEPS_ERR_STATE, "scratch too small for this precision");
+    p.t3 = s->t3.ptr();
+    p.s1r2 = s->s1r2.ptr();
+    p.radial = s->radial.ptr();
+    return STEPS_OK;
+}
+
+hipError_t sim_enqueue_forces(steps_sim *s, const EvalPlan &p, hipStream_t st, bool events) {
+    return launch_forces(p, s->dp, s->part.as<double>(), s->F.as<double>(), st, events ? s->ev0 : nullptr,
+                         events ? s->ev1 : nullptr, events ? s->ev2 : nullptr);
+}
+
+hipError_t sim_enqueue_step(steps_sim *s, const EvalPlan &p, hipStream_t st) {
+    // step.cc:100-312: params in, kick+drift+wrap, forces, kick+errmax, errmax out
+    const KdkParams *dpar = s->dparams.as<const KdkParams>();
+    hipError_t e;
+    if ((e = hipMemcpyAsync(s->dparams.ptr, s->hparams, sizeof(KdkParams), hipMemcpyHostToDevice, st)) != hipSuccess)
+        return e;
+    if ((e = launch_kick_drift(s->dp, s->v.as<double>(), s->F.as<double>(), dpar, s->desc.topology, s->desc.L,
+                               st)) != hipSuccess)
+        return e;
+    if ((e = launch_forces(p, s->dp, s->part.as<double>(), s->F.as<double>(), st, nullptr, nullptr, nullptr)) !=
+        hipSuccess)
+        return e;
+    const int nb = kdk_reduce_blocks(s->N);
+    double *bm = s->blockmax.as<double>();
+    if ((e = launch_kick_errmax(s->dp, s->v.as<double>(), s->F.as<double>(), dpar, bm, nb, bm + nb, st, true)) !=
+        hipSuccess)
+        return e;
+    return hipMemcpyAsync(s->herrmax, bm + nb, sizeof(double), hipMemcpyDeviceToHost, st);
+}
+
+int sim_fill_params(steps_sim *s, const steps_step_params &sp) {
+    if (!(sp.rho_part > 0)) return fail(STEPS_ERR_ARG, "rho_part must be positive");
+    if (!(sp.a_kick1 > 0) || !(sp.a_kick2 > 0)) return fail(STEPS_ERR_ARG, "scale factors must be positive");
+    KdkParams &k = *s->hparams;
+    k.G = sp.G;
+    k.h = sp.h;
+    k.ainv3_1 = std::pow(sp.a_kick1, -3.0);  // step.cc:135: pow(a, -3.0)
+    k.hubble1 = sp.hubble1;
+    k.ainv3_2 = std::pow(sp.a_kick2, -3.0);
+    k.hubble2 = sp.hubble2;
+    k.const_beta = 3.0 / sp.rho_part / (4.0 * kPi);  // step.cc:250
+    return STEPS_OK;
+}
+
+int sim_upload(steps_sim *s, const double *x, const double *v, const double *M, const double *soft,
+               const double *F) {
+    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    const size_t nb = (size_t)s->N * sizeof(double);
+    double *xs = s->stage.as<double>();
+    // M and soft are staged in the v buffer (v is written right after)
+    double *ms = s->v.as<double>();
+    double *hs = ms + s->N;
+    STEPS_HIP(hipMemcpyAsync(xs, x, 3 * nb, hipMemcpyHostToDevice, s->stream), "copy x");
+    STEPS_HIP(hipMemcpyAsync(ms, M, nb, hipMemcpyHostToDevice, s->stream), "copy M");
+    STEPS_HIP(hipMemcpyAsync(hs, soft, nb, hipMemcpyHostToDevice, s->stream), "copy soft");
+    STEPS_HIP(launch_pack(xs, ms, hs, s->N, s->dp, s->stream), "pack");
+    if (v) STEPS_HIP(hipMemcpyAsync(s->v.ptr, v, 3 * nb, hipMemcpyHostToDevice, s->stream), "copy v");
+    else STEPS_HIP(hipMemsetAsync(s->v.ptr, 0, 3 * nb, s->stream), "zero v");
+    if (F) STEPS_HIP(hipMemcpyAsync(s->F.ptr, F, 3 * nb, hipMemcpyHostToDevice, s->stream), "copy F");
+    else STEPS_HIP(hipMemsetAsync(s->F.ptr, 0, 3 * nb, s->stream), "zero F");
+    STEPS_HIP(hipStreamSynchronize(s->stream), "upload state");
+    s->evaluated = false;
+    return STEPS_OK;
+}
+
+int sim_download(steps_sim *s, double *x, double *v, double *F) {
+    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    const size_t nb = (size_t)s->N * sizeof(double);
+    if (x) {
+        STEPS_HIP(launch_unpack_positions(s->dp, s->stage.as<double>(), s->stream), "unpack");
+        STEPS_HIP(hipMemcpyAsync(x, s->stage.ptr, 3 * nb, hipMemcpyDeviceToHost, s->stream), "copy x");
+    }
+    if (v) STEPS_HIP(hipMemcpyAsync(v, s->v.ptr, 3 * nb, hipMemcpyDeviceToHost, s->stream), "copy v");
+    if (F) STEPS_HIP(hipMemcpyAsync(F, s->F.ptr, 3 * nb, hipMemcpyDeviceToHost, s->stream), "copy F");
+    STEPS_HIP(hipStreamSynchronize(s->stream), "download state");
+    return STEPS_OK;
+}
+
+namespace {
+
+// State of the one-shot entry point (one device at a time, under a lock).
+struct OneShot {
+    std::mutex mu;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    DevBuf x, M, soft, jrec, jsoft, part, F;
+    TableCopy t3, s1r2, radial;
+};
+OneShot g_oneshot;
+
 }  // namespace
 }  // namespace steps
-
-// ---------------------------------------------------------------------------
-// resident simulation handle
-// ---------------------------------------------------------------------------
-struct steps_sim {
-    int device = 0;
-    steps_force_desc desc{};
-    int64_t N = 0;
-    steps::DeviceParticles dp;
-    steps::DevBuf jrec, jsoft, v, F, part, stage, blockmax;
-    steps::TableCopy t3, s1r2, radial;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    int64_t last_nrows = 0;
-    bool evaluated = false;
-};
 
 using namespace steps;
 
@@ -281,7 +416,7 @@ int steps_device_count(void) {
 
 const char *steps_last_error(void) { return g_error.c_str(); }
 
-const char *steps_version(void) { return "steps_amd 0.1 (gfx950)"; }
+const char *steps_version(void) { return "steps_amd 0.2 (gfx950)"; }
 
 int steps_force_eval(const steps_force_desc *d, int64_t N, const double *x, const double *M,
                      const double *soft, double *F, int64_t id_min, int64_t id_max) {
@@ -340,67 +475,20 @@ int steps_force_eval(const steps_force_desc *d, int64_t N, const double *x, cons
 
 int steps_sim_create(const steps_force_desc *d, int64_t N, steps_sim **out) {
     if (!d || !out) return fail(STEPS_ERR_ARG, "null argument");
-    *out = nullptr;
-    if (N <= 0) return fail(STEPS_ERR_ARG, "N must be positive");
-    EvalPlan p;
-    int rc = make_plan(*d, N, 0, N - 1, p);
-    if (rc) return rc;
-    if ((rc = set_device(d->device))) return rc;
-    steps_sim *s = new (std::nothrow) steps_sim();
-    if (!s) return fail(STEPS_ERR_NOMEM, "host allocation");
-    s->device = d->device;
-    s->desc = *d;
-    s->N = N;
-    const size_t nb = (size_t)N * sizeof(double);
-    auto cleanup = [&](int code) {
-        steps_sim_destroy(s);
-        return code;
-    };
-    if ((rc = s->jrec.reserve((size_t)N * sizeof(double4), "alloc jrec")) ||
-        (rc = s->jsoft.reserve(nb, "alloc soft")) || (rc = s->v.reserve(3 * nb, "alloc v")) ||
-        (rc = s->F.reserve(3 * nb, "alloc F")) || (rc = s->part.reserve(scratch_bytes(p), "alloc partials")) ||
-        (rc = s->stage.reserve(std::max(3 * nb, (size_t)1024), "alloc stage")) ||
-        (rc = s->blockmax.reserve((size_t)(kdk_reduce_blocks(N) + 1) * sizeof(double), "alloc blockmax")))
-        return cleanup(rc);
-    hipError_t e;
-    if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess ||
-        (e = hipEventCreate(&s->ev2)) != hipSuccess)
-        return cleanup(hip_fail(e, "stream/event create"));
-    if (p.variant == PV_T3_EWALD &&
-        (rc = s->t3.sync(d->t3_table, (size_t)d->t3_ngrid * d->t3_ngrid * d->t3_ngrid * 3, s->stream, "T3 table")))
-        return cleanup(rc);
-    if (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC &&
-        (rc = s->s1r2.sync(d->s1r2_table, (size_t)d->s1r2_nrho * d->s1r2_nz * 2, s->stream, "S1R2 table")))
-        return cleanup(rc);
-    if (p.ext_mode == 3 &&
-        (rc = s->radial.sync(d->radial_table, (size_t)d->radial_table_size, s->stream, "radial table")))
-        return cleanup(rc);
-    // the sim owns device copies: host table pointers are not kept
-    s->desc.t3_table = nullptr;
-    s->desc.s1r2_table = nullptr;
-    s->desc.radial_table = nullptr;
-    s->dp = DeviceParticles{s->jrec.as<double4>(), s->jsoft.as<double>(), N};
-    if ((e = hipMemsetAsync(s->v.ptr, 0, 3 * nb, s->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(s->F.ptr, 0, 3 * nb, s->stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(s->stream)) != hipSuccess)
-        return cleanup(hip_fail(e, "initialise sim"));
-    *out = s;
-    return STEPS_OK;
+    return sim_create(*d, N, true, out);
 }
 
 int steps_sim_destroy(steps_sim *s) {
-    if (!s) return STEPS_OK;
-    (void)hipSetDevice(s->device);
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
-    s->jrec.release(); s->jsoft.release(); s->v.release(); s->F.release(); s->part.release();
-    s->stage.release(); s->blockmax.release();
-    s->t3.dev.release(); s->s1r2.dev.release(); s->radial.dev.release();
-    if (s->ev0) (void)hipEventDestroy(s->ev0);
-    if (s->ev1) (void)hipEventDestroy(s->ev1);
-    if (s->ev2) (void)hipEventDestroy(s->ev2);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
-    delete s;
+    sim_destroy(s);
+    return STEPS_OK;
+}
+
+int steps_sim_set_precision(steps_sim *s, int32_t precision) {
+    if (!s) return fail(STEPS_ERR_ARG, "null sim");
+    if (precision != STEPS_PREC_FP64 && precision != STEPS_PREC_MIXED) return fail(STEPS_ERR_ARG, "unknown precision");
+    if (precision == STEPS_PREC_MIXED && s->desc.topology != STEPS_TOPO_R3)
+        return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3 only");
+    s->desc.precision = precision;
     return STEPS_OK;
 }
 
@@ -409,22 +497,7 @@ int steps_sim_set_state(steps_sim *s, const double *x, const double *v, const do
     if (!s || !x || !M || !soft) return fail(STEPS_ERR_ARG, "null argument");
     int rc = check_finite(x, 3 * s->N);
     if (rc) return rc;
-    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
-    const size_t nb = (size_t)s->N * sizeof(double);
-    // stage: x (3N) then M, soft in the F buffer's space (F is recomputed anyway)
-    double *xs = s->stage.as<double>();
-    double *ms = s->F.as<double>();
-    double *hs = ms + s->N;
-    STEPS_HIP(hipMemcpyAsync(xs, x, 3 * nb, hipMemcpyHostToDevice, s->stream), "copy x");
-    STEPS_HIP(hipMemcpyAsync(ms, M, nb, hipMemcpyHostToDevice, s->stream), "copy M");
-    STEPS_HIP(hipMemcpyAsync(hs, soft, nb, hipMemcpyHostToDevice, s->stream), "copy soft");
-    STEPS_HIP(launch_pack(xs, ms, hs, s->N, s->dp, s->stream), "pack");
-    if (v) STEPS_HIP(hipMemcpyAsync(s->v.ptr, v, 3 * nb, hipMemcpyHostToDevice, s->stream), "copy v");
-    else STEPS_HIP(hipMemsetAsync(s->v.ptr, 0, 3 * nb, s->stream), "zero v");
-    STEPS_HIP(hipMemsetAsync(s->F.ptr, 0, 3 * nb, s->stream), "zero F");
-    STEPS_HIP(hipStreamSynchronize(s->stream), "set state");
-    s->evaluated = false;
-    return STEPS_OK;
+    return sim_upload(s, x, v, M, soft, nullptr);
 }
 
 int steps_sim_set_positions(steps_sim *s, const double *x) {
@@ -442,58 +515,16 @@ int steps_sim_set_positions(steps_sim *s, const double *x) {
 
 int steps_sim_get_state(steps_sim *s, double *x, double *v) {
     if (!s) return fail(STEPS_ERR_ARG, "null sim");
-    STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
-    const size_t nb = (size_t)s->N * sizeof(double);
-    if (x) {
-        STEPS_HIP(launch_unpack_positions(s->dp, s->stage.as<double>(), s->stream), "unpack");
-        STEPS_HIP(hipMemcpyAsync(x, s->stage.ptr, 3 * nb, hipMemcpyDeviceToHost, s->stream), "copy x");
-    }
-    if (v) STEPS_HIP(hipMemcpyAsync(v, s->v.ptr, 3 * nb, hipMemcpyDeviceToHost, s->stream), "copy v");
-    STEPS_HIP(hipStreamSynchronize(s->stream), "get state");
-    return STEPS_OK;
-}
-
-static int sim_plan(steps_sim *s, int64_t id_min, int64_t id_max, EvalPlan &p) {
-    steps_force_desc d = s->desc;  // tables live on the device; the checks only need non-null
-    d.t3_table = s->t3.ptr();
-    d.s1r2_table = s->s1r2.ptr();
-    d.radial_table = s->radial.ptr();
-    int rc = make_plan(d, s->N, id_min, id_max, p);
-    if (rc) return rc;
-    p.t3 = s->t3.ptr();
-    p.s1r2 = s->s1r2.ptr();
-    p.radial = s->radial.ptr();
-    return STEPS_OK;
-}
-
-// Second kick (optional) and errmax = max_i |A_i| / cbrt(M_i*3/(4 pi rho_part))
-// on the device (step.cc:256-269, :70-80); one double comes back to the host.
-static int sim_kick_errmax(steps_sim *s, double a, double hubble, double G, double h,
-                           double rho_part, double *errmax, bool kick) {
-    if (!(rho_part > 0)) return fail(STEPS_ERR_ARG, "rho_part must be positive");
-    const double const_beta = 3.0 / rho_part / (4.0 * kPi);
-    const int nb = kdk_reduce_blocks(s->N);
-    double *bm = s->blockmax.as<double>();
-    STEPS_HIP(launch_kick_errmax(s->dp, s->v.as<double>(), s->F.as<double>(), G, std::pow(a, -3.0), hubble,
-                                 h, const_beta, bm, nb, bm + nb, s->stream, kick),
-              "kick/errmax");
-    double e = 0;
-    STEPS_HIP(hipMemcpyAsync(&e, bm + nb, sizeof(double), hipMemcpyDeviceToHost, s->stream), "copy errmax");
-    STEPS_HIP(hipStreamSynchronize(s->stream), "step");
-    if (errmax) *errmax = e;
-    return STEPS_OK;
+    return sim_download(s, x, v, nullptr);
 }
 
 int steps_sim_forces(steps_sim *s, int64_t id_min, int64_t id_max) {
     if (!s) return fail(STEPS_ERR_ARG, "null sim");
     EvalPlan p;
-    int rc = sim_plan(s, id_min, id_max, p);
+    int rc = sim_plan(s, id_min, id_max, s->desc.precision, p);
     if (rc) return rc;
-    if ((size_t)scratch_bytes(p) > s->part.cap) return fail(STEPS_ERR_STATE, "scratch too small");
     STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
-    STEPS_HIP(launch_forces(p, s->dp, s->part.as<double>(), s->F.as<double>(), s->stream, s->ev0, s->ev1,
-                            s->ev2),
-              "force kernels");
+    STEPS_HIP(sim_enqueue_forces(s, p, s->stream, true), "force kernels");
     s->last_nrows = p.nrows;
     s->evaluated = true;
     return STEPS_OK;
@@ -533,23 +564,35 @@ int steps_sim_step(steps_sim *s, const steps_step_params *sp, double *errmax) {
     if (!s || !sp) return fail(STEPS_ERR_ARG, "null argument");
     STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
     EvalPlan p;
-    int rc = sim_plan(s, 0, s->N - 1, p);
+    int rc = sim_plan(s, 0, s->N - 1, s->desc.precision, p);
     if (rc) return rc;
-    const double ainv3a = std::pow(sp->a_kick1, -3.0);
-    STEPS_HIP(launch_kick_drift(s->dp, s->v.as<double>(), s->F.as<double>(), sp->G, ainv3a, sp->hubble1, sp->h,
-                                s->desc.topology, s->desc.L, s->stream),
-              "kick-drift");
-    STEPS_HIP(launch_forces(p, s->dp, s->part.as<double>(), s->F.as<double>(), s->stream, s->ev0, s->ev1, s->ev2),
-              "force kernels");
+    if ((rc = sim_fill_params(s, *sp))) return rc;
+    STEPS_HIP(sim_enqueue_step(s, p, s->stream), "KDK step");
+    STEPS_HIP(hipStreamSynchronize(s->stream), "KDK step");
     s->last_nrows = p.nrows;
-    s->evaluated = true;
-    return sim_kick_errmax(s, sp->a_kick2, sp->hubble2, sp->G, sp->h, sp->rho_part, errmax, true);
+    s->evaluated = false;  // the step's force kernels carry no timing events
+    if (errmax) *errmax = *s->herrmax;
+    return STEPS_OK;
 }
 
 int steps_sim_errmax(steps_sim *s, double a, double hubble, double G, double rho_part, double *errmax) {
+    // step.cc:35-98: errmax of the stored forces, no kick
     if (!s) return fail(STEPS_ERR_ARG, "null sim");
     STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
-    return sim_kick_errmax(s, a, hubble, G, 0.0, rho_part, errmax, false);
+    steps_step_params sp{0.0, a, hubble, a, hubble, G, rho_part};
+    int rc = sim_fill_params(s, sp);
+    if (rc) return rc;
+    const int nb = kdk_reduce_blocks(s->N);
+    double *bm = s->blockmax.as<double>();
+    STEPS_HIP(hipMemcpyAsync(s->dparams.ptr, s->hparams, sizeof(KdkParams), hipMemcpyHostToDevice, s->stream),
+              "params");
+    STEPS_HIP(launch_kick_errmax(s->dp, s->v.as<double>(), s->F.as<double>(), s->dparams.as<const KdkParams>(), bm,
+                                 nb, bm + nb, s->stream, false),
+              "errmax");
+    STEPS_HIP(hipMemcpyAsync(s->herrmax, bm + nb, sizeof(double), hipMemcpyDeviceToHost, s->stream), "errmax");
+    STEPS_HIP(hipStreamSynchronize(s->stream), "errmax");
+    if (errmax) *errmax = *s->herrmax;
+    return STEPS_OK;
 }
 
 int steps_softening_lengths(const double *M, int64_t N, double particle_radius, double *soft,

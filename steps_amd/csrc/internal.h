@@ -82,12 +82,19 @@ hipError_t launch_pack_positions(const double *x, int64_t n, DeviceParticles &dp
 // DeviceParticles positions -> AoS x [3n] (device).
 hipError_t launch_unpack_positions(const DeviceParticles &dp, double *x, hipStream_t stream);
 
-// KDK pieces (kdk_kernels.hip)
-hipError_t launch_kick_drift(DeviceParticles &dp, double *v, const double *F, double G,
-                             double ainv3, double hubble, double h, int topology, double L,
-                             hipStream_t stream);
-hipError_t launch_kick_errmax(const DeviceParticles &dp, double *v, const double *F, double G,
-                              double ainv3, double hubble, double h, double const_beta,
+// KDK pieces (kdk_kernels.hip).  Per-step scalars live in device memory
+// (written by the host before the launch) so that captured graphs replay.
+struct KdkParams {
+    double G;
+    double h;
+    double ainv3_1, hubble1;  // before the step (first kick)
+    double ainv3_2, hubble2;  // after the Friedmann update (second kick, errmax)
+    double const_beta;        // 3/(4 pi rho_part)
+    double pad;
+};
+hipError_t launch_kick_drift(DeviceParticles &dp, double *v, const double *F, const KdkParams *prm,
+                             int topology, double L, hipStream_t stream);
+hipError_t launch_kick_errmax(const DeviceParticles &dp, double *v, const double *F, const KdkParams *prm,
                               double *block_max, int nblocks_max, double *errmax_dev,
                               hipStream_t stream, bool kick);
 int kdk_reduce_blocks(int64_t n);

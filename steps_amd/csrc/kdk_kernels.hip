@@ -3,7 +3,9 @@
 // The reference runs the kicks, drift, periodic wrap and the errmax reduction
 // on MPI rank 0's host (step.cc:128-181, :250-269); here they run where the
 // particle state lives, so a step moves no particle data over PCIe.  errmax is
-// a max-reduction (exact, so the block order does not matter).
+// a max-reduction (exact, so the block order does not matter).  The per-step
+// scalars come from a device parameter block written before the launch, so a
+// whole step can be captured once in a hipGraph and replayed.
 #include "internal.h"
 
 #include <algorithm>
@@ -17,10 +19,11 @@ constexpr int kRedBlock = 256;
 __global__ __launch_bounds__(256) void kick_drift_kernel(double4 *__restrict__ jrec,
                                                          double *__restrict__ v,
                                                          const double *__restrict__ F, int64_t n,
-                                                         double G, double ainv3, double hubble,
-                                                         double h, int topology, double L) {
+                                                         const KdkParams *__restrict__ prm,
+                                                         int topology, double L) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double G = prm->G, ainv3 = prm->ainv3_1, hubble = prm->hubble1, h = prm->h;
     double4 p = jrec[i];
     double x[3] = {p.x, p.y, p.z};
 #pragma unroll
@@ -48,9 +51,10 @@ __global__ __launch_bounds__(256) void kick_drift_kernel(double4 *__restrict__ j
 
 __global__ __launch_bounds__(kRedBlock) void kick_errmax_kernel(
     const double4 *__restrict__ jrec, double *__restrict__ v, const double *__restrict__ F, int64_t n,
-    double G, double ainv3, double hubble, double h, double const_beta, int kick,
-    double *__restrict__ block_max) {
+    const KdkParams *__restrict__ prm, int kick, double *__restrict__ block_max) {
     __shared__ double red[kRedBlock];
+    const double G = prm->G, ainv3 = prm->ainv3_2, hubble = prm->hubble2, h = prm->h;
+    const double const_beta = prm->const_beta;
     double m = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -93,21 +97,20 @@ int kdk_reduce_blocks(int64_t n) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + kRedBlock - 1) / kRedBlock));
 }
 
-hipError_t launch_kick_drift(DeviceParticles &dp, double *v, const double *F, double G, double ainv3,
-                             double hubble, double h, int topology, double L, hipStream_t st) {
+hipError_t launch_kick_drift(DeviceParticles &dp, double *v, const double *F, const KdkParams *prm,
+                             int topology, double L, hipStream_t st) {
     if (dp.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(kick_drift_kernel, dim3((unsigned)((dp.n + 255) / 256)), dim3(256), 0, st, dp.jrec, v,
-                       F, dp.n, G, ainv3, hubble, h, topology, L);
+                       F, dp.n, prm, topology, L);
     return hipGetLastError();
 }
 
-hipError_t launch_kick_errmax(const DeviceParticles &dp, double *v, const double *F, double G,
-                              double ainv3, double hubble, double h, double const_beta,
+hipError_t launch_kick_errmax(const DeviceParticles &dp, double *v, const double *F, const KdkParams *prm,
                               double *block_max, int nblocks, double *errmax_dev, hipStream_t st,
                               bool kick) {
     if (dp.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(kick_errmax_kernel, dim3((unsigned)nblocks), dim3(kRedBlock), 0, st, dp.jrec, v, F, dp.n,
-                       G, ainv3, hubble, h, const_beta, kick ? 1 : 0, block_max);
+                       prm, kick ? 1 : 0, block_max);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(max_finish_kernel, dim3(1), dim3(kRedBlock), 0, st, block_max, nblocks, errmax_dev);

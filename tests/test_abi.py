@@ -85,3 +85,20 @@ def test_softening_lengths_host_entry(oracle):
     ho, mmino, rpo = oracle.softening_lengths(M, 0.05)
     assert mmin == mmino and rp == pytest.approx(rpo, rel=1e-15)
     assert np.allclose(h, ho, rtol=1e-15, atol=0)
+
+
+def test_server_without_device_or_with_bad_config():
+    import ctypes as C
+    lib = steps_amd.load_library()
+    cfg = _abi.ServerConfig()
+    lib.steps_server_config_init(C.byref(cfg))
+    assert cfg.streams == 4 and cfg.use_graphs == 1 and cfg.hbm_budget_bytes == 200 << 30
+    h = C.c_void_p()
+    cfg.streams = 0
+    assert lib.steps_server_create(C.byref(cfg), C.byref(h)) == _abi.STEPS_ERR_ARG
+    cfg.streams = 4
+    cfg.hbm_budget_bytes = 0
+    assert lib.steps_server_create(C.byref(cfg), C.byref(h)) == _abi.STEPS_ERR_ARG
+    if steps_amd.device_count() == 0:
+        cfg.hbm_budget_bytes = 1 << 30
+        assert lib.steps_server_create(C.byref(cfg), C.byref(h)) == _abi.STEPS_ERR_DEVICE

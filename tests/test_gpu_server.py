@@ -1,0 +1,159 @@
+"""The serving loop: every request's result is the same alone and in the mix
+(bitwise -- same kernels, plan and reduction order), with and without graph
+replay, under memory pressure (admission + LRU eviction), and a failing
+request does not disturb the others."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import steps_amd
+from steps_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def step_params(w, k):
+    a1 = w.a_start * (1 + 1e-3 * k)
+    a2 = w.a_start * (1 + 1e-3 * (k + 1))
+    H = lambda a: wl.H0 * math.sqrt(wl.OMEGA_M * a ** -3 + wl.OMEGA_L)
+    return steps_amd.StepParams(1e-4, a1, H(a1), a2, H(a2), 1.0, w.rho_part)
+
+
+def make_sims(oracle):
+    sims = []
+    for N, seed in [(3000, 1), (20000, 2), (777, 3), (9000, 4)]:
+        w = wl.r3_compactified(N=N, seed=seed)
+        d, keep = steps_amd.make_desc(steps_amd.TOPO_R3, cosmology=1, comoving=1,
+                                      mass_in_unit_sphere=w.mass_in_unit_sphere)
+        sims.append((w, d, keep))
+    w = wl.t3_lattice(n_side=10, seed=5, is_periodic=1)
+    d, keep = steps_amd.make_desc(steps_amd.TOPO_T3, is_periodic=1, L=w.L, cosmology=1, comoving=1)
+    sims.append((w, d, keep))
+    c = wl.s1r2_cylinder(n_omega=16, n_phi=8, n_z=8, seed=6)
+    T, shape = oracle.s1r2_table(2, c.L, c.Rsim)
+    d, keep = steps_amd.make_desc(steps_amd.TOPO_S1R2, is_periodic=2, L=c.L, Rsim=c.Rsim, cosmology=1, comoving=1,
+                                  mass_in_unit_sphere=c.mass_in_unit_sphere, s1r2_lookup=1, interp_order=2,
+                                  s1r2_table=T, s1r2_shape=shape)
+    sims.append((c, d, keep + [T]))
+    return sims
+
+
+NSTEPS = 3
+
+
+def alone(w, d, precision=0):
+    d.precision = precision
+    s = steps_amd.Sim(d, w.N)
+    s.set_state(w.x, w.M, w.soft, w.v)
+    s.forces()
+    F0 = s.get_forces()
+    errs = []
+    for k in range(NSTEPS):
+        p = step_params(w, k)
+        errs.append(s.step(p.h, p.a_kick1, p.hubble1, p.a_kick2, p.hubble2, p.rho_part))
+    x, v = s.get_state()
+    s.forces()
+    F = s.get_forces()
+    s.close()
+    d.precision = 0
+    return F0, errs, x, v, F
+
+
+@pytest.fixture(scope="module")
+def ensemble(oracle):
+    sims = make_sims(oracle)
+    ref = [alone(w, d) for (w, d, _) in sims]
+    return sims, ref
+
+
+def run_mix(sims, server, precisions=None):
+    ids = [server.add_sim(d, w.x, w.M, w.soft, w.v) for (w, d, _) in sims]
+    tickets = {i: [] for i in range(len(sims))}
+    # round-robin interleaving: force, then NSTEPS steps, per simulation
+    for k in range(NSTEPS + 1):
+        for i, (w, d, _) in enumerate(sims):
+            prec = precisions[i] if precisions else 0
+            if k == 0:
+                tickets[i].append(server.submit(ids[i], steps_amd.REQ_FORCE, prec))
+            else:
+                tickets[i].append(server.submit(ids[i], steps_amd.REQ_STEP, prec, step_params(w, k - 1)))
+    server.drain()
+    results = {i: [server.wait(t) for t in tickets[i]] for i in tickets}
+    states = {i: server.get_state(ids[i]) for i in tickets}
+    return ids, results, states
+
+
+@pytest.mark.parametrize("use_graphs", [True, False])
+def test_mix_equals_alone(ensemble, use_graphs):
+    print(f"case server mix use_graphs={use_graphs}", flush=True)
+    sims, ref = ensemble
+    srv = steps_amd.Server(streams=4, use_graphs=use_graphs)
+    ids, results, states = run_mix(sims, srv)
+    for i, (F0, errs, x, v, F) in enumerate(ref):
+        assert all(r.status == 0 for r in results[i]), [r.status for r in results[i]]
+        assert [r.errmax for r in results[i][1:]] == errs, i
+        xs, vs, Fs = states[i]
+        assert np.array_equal(xs, x) and np.array_equal(vs, v) and np.array_equal(Fs, F), i
+        # per-simulation FIFO: each request starts after the previous one ended
+        for a, b in zip(results[i], results[i][1:]):
+            assert b.t_start >= a.t_end
+    st = srv.stats()
+    assert st["completed"] == len(sims) * (NSTEPS + 1) and st["failed"] == 0
+    if use_graphs:
+        assert st["graph_captures"] == 2 * len(sims)   # one force and one step graph per simulation
+    srv.close()
+
+
+def test_admission_and_eviction_keep_results(ensemble):
+    print("case server small budget", flush=True)
+    sims, ref = ensemble
+    probe = steps_amd.Server(streams=1)
+    sizes = [probe.sim_bytes(probe.add_sim(d, w.x, w.M, w.soft)) for (w, d, _) in sims]
+    probe.close()
+    # room for the largest simulation plus a little: the others have to take turns
+    budget = max(sizes) + min(sizes)
+    srv = steps_amd.Server(streams=2, hbm_budget_bytes=budget)
+    ids, results, states = run_mix(sims, srv)
+    st = srv.stats()
+    print(f"[server small budget] sizes={sizes} budget={budget} {st}", flush=True)
+    assert st["evictions"] > 0 and st["admissions"] > len(sims)
+    assert st["bytes_resident"] <= budget
+    for i, (F0, errs, x, v, F) in enumerate(ref):
+        assert all(r.status == 0 for r in results[i])
+        assert [r.errmax for r in results[i][1:]] == errs, i
+        xs, vs, Fs = states[i]
+        assert np.array_equal(xs, x) and np.array_equal(vs, v) and np.array_equal(Fs, F), i
+    srv.close()
+
+
+def test_mixed_precision_requests_match_alone(ensemble):
+    sims, ref = ensemble
+    r3 = [s for s in sims if s[0].name == "r3_compactified"]
+    ref_mixed = [alone(w, d, precision=1) for (w, d, _) in r3]
+    srv = steps_amd.Server(streams=3)
+    ids, results, states = run_mix(r3, srv, precisions=[1] * len(r3))
+    for i, (F0, errs, x, v, F) in enumerate(ref_mixed):
+        assert [r.errmax for r in results[i][1:]] == errs
+        assert np.array_equal(states[i][2], F)
+    srv.close()
+
+
+def test_failing_request_is_isolated(ensemble):
+    sims, ref = ensemble
+    srv = steps_amd.Server(streams=4)
+    c, d, _ = sims[-1]            # S1R2: the mixed path is not available for it
+    w0, d0, _ = sims[0]
+    bad = srv.add_sim(d, c.x, c.M, c.soft)
+    good = srv.add_sim(d0, w0.x, w0.M, w0.soft, w0.v)
+    t_bad = srv.submit(bad, steps_amd.REQ_FORCE, steps_amd.PREC_MIXED)
+    t_good = srv.submit(good, steps_amd.REQ_FORCE, steps_amd.PREC_FP64)
+    assert srv.wait(t_bad).status != 0
+    assert srv.wait(t_good).status == 0
+    assert np.array_equal(srv.get_state(good)[2], ref[0][0])
+    with pytest.raises(steps_amd.StepsError):
+        srv.submit(99, steps_amd.REQ_FORCE)
+    with pytest.raises(steps_amd.StepsError):
+        srv.submit(good, steps_amd.REQ_STEP)       # a step needs parameters
+    srv.close()
