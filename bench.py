@@ -38,6 +38,16 @@ FLOP_PER_PAIR_R3 = 20
 PEAK_FP64_VECTOR_TFLOPS = 78.6    # MI355X published FP64 vector peak
 PEAK_FP32_VECTOR_TFLOPS = 157.3   # MI355X published FP32 vector peak (packed)
 
+# The same count for the periodic variants (reference lines forces_cuda.cu
+# :616-635 with :93-167 for T3, :826-843 with :188-455 for S1xR2):
+#   S1xR2 CIC  61 = pair 13 + rho 4 + CIC map and weights 28 + Cartesian 4 + combine 12
+#   S1xR2 TSC 104 = pair 13 + rho 4 + TSC map and 9 weights 25 + 18 taps x 2 comps 54
+#                   - CIC's 28 -> 104 with Cartesian and combine as for CIC
+#   S1xR2 NGP  39 = pair 13 + rho 4 + NGP map 6 + Cartesian 4 + combine 12
+#   T3 tricubic 555 = pair 13 + grid map 12 + weights 54 + tensor 464 + combine 12
+FLOP_PER_PAIR = {"t3_ewald": 555, "s1r2_ngp": 39, "s1r2_cic": 61, "s1r2_tsc": 104}
+T3_TAP_BYTES_PER_PAIR = 64 * 3 * 8   # 4^3 table nodes x 3 components x FP64
+
 WORKLOADS = {
     # name: (generator kwargs, description)
     "r3_2m": dict(N=2 ** 21, seed=1003),
@@ -136,13 +146,13 @@ def load_traffic(name: str):
 def serving_leg(nreq: int) -> dict:
     """64 resident simulations (R^3 and S^1 x R^2 alternating, N log-uniform in
     [8192, 524288]) receive a seeded Poisson stream (200 req/s) of KDK step
-    requests, half FP64 and half mixed where the topology has a mixed path;
+    requests, half FP64 and half mixed precision;
     served by the continuous-batching server and, for comparison, one
     request at a time on resident simulations.  The final (x, v) of every
     simulation must be bitwise equal between the two."""
     from steps_amd import serving as sv
     sims = sv.make_ensemble()
-    reqs = sv.make_requests(sims, nreq=nreq, mixed_ok=sv.r3_only_mixed)
+    reqs = sv.make_requests(sims, nreq=nreq)
     print(f"[bench] serving: {len(sims)} sims, {nreq} requests", file=sys.stderr, flush=True)
     srv, fin_s = sv.run_server(sims, reqs)
     print(f"[bench] serving: server done {srv}", file=sys.stderr, flush=True)
@@ -151,12 +161,58 @@ def serving_leg(nreq: int) -> dict:
     same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(fin_s, fin_q))
     pairs = float(sum(sims[r.sim][0].N ** 2 for r in reqs))
     return {"workload": "64 sims R3/S1R2, N log-uniform [8192, 524288], Poisson 200 req/s seed 1004, "
-                        "KDK steps, FP64 / mixed 50/50 on R3",
+                        "KDK steps, FP64 / mixed 50/50",
             "requests": nreq, "pair_interactions": pairs,
             "steps_per_s": srv["requests_per_s"], "p50_ms": srv["p50_ms"], "p99_ms": srv["p99_ms"],
             "server": srv, "sequential": seq,
             "speedup_vs_sequential": srv["requests_per_s"] / seq["requests_per_s"],
             "results_identical": bool(same)}
+
+
+def variants_leg(reps: int) -> dict:
+    """The periodic force variants on their measurement configurations: T3
+    with the tricubic Ewald table (64^3 perturbed lattice, seed 1001) and
+    S1xR2 with CIC and TSC table interpolation (N = 524288, seed 1002); FP64,
+    one warm-up, then `reps` evaluations timed with HIP events."""
+    out = {}
+    t3 = wl.t3_lattice()
+    T = steps_amd.t3_table(2, t3.L)
+    d, keep = steps_amd.make_desc(steps_amd.TOPO_T3, is_periodic=2, L=t3.L, t3_table=T)
+    cases = [("t3_ewald", t3, d, [T, keep])]
+    c = wl.s1r2_cylinder()
+    Ts, shape = steps_amd.s1r2_table(2, c.L, c.Rsim)
+    for name, order in (("s1r2_cic", 2), ("s1r2_tsc", 4)):
+        ds, ks = steps_amd.make_desc(steps_amd.TOPO_S1R2, is_periodic=2, L=c.L, Rsim=c.Rsim, cosmology=1,
+                                     comoving=1, mass_in_unit_sphere=c.mass_in_unit_sphere, s1r2_lookup=1,
+                                     interp_order=order, s1r2_table=Ts, s1r2_shape=shape)
+        cases.append((name, c, ds, [Ts, ks]))
+    for name, w, desc, _keep in cases:
+        sim = steps_amd.Sim(desc, w.N)
+        sim.set_state(w.x, w.M, w.soft, w.v)
+        wall, pair_ms, total_ms = time_path(sim, reps, 1)
+        F64 = sim.get_forces()
+        pairs = float(w.N) ** 2
+        flop = FLOP_PER_PAIR[name]
+        r = {"N": w.N, "kernel_ms": pair_ms, "eval_ms": total_ms,
+             "g_pairs_per_s": pairs / (pair_ms * 1e-3) / 1e9,
+             "flop_per_pair": flop, "achieved_tflops": flop * pairs / (pair_ms * 1e-3) / 1e12,
+             "frac_fp64_peak": flop * pairs / (pair_ms * 1e-3) / 1e12 / PEAK_FP64_VECTOR_TFLOPS}
+        if name == "t3_ewald":
+            r["tap_bytes_per_s"] = T3_TAP_BYTES_PER_PAIR * pairs / (pair_ms * 1e-3)
+        else:
+            # FP32-compensated path on the same state: speed and error against FP64
+            sim.set_precision(steps_amd.PREC_MIXED)
+            _, pair_m, _ = time_path(sim, reps, 1)
+            err = np.linalg.norm(sim.get_forces() - F64, axis=1) / np.linalg.norm(F64, axis=1)
+            r["mixed"] = {"kernel_ms": pair_m, "g_pairs_per_s": pairs / (pair_m * 1e-3) / 1e9,
+                          "speedup_vs_fp64": pair_ms / pair_m,
+                          "rel_err_vs_fp64_p50": float(np.median(err)),
+                          "rel_err_vs_fp64_p99": float(np.percentile(err, 99)),
+                          "rel_err_vs_fp64_max": float(np.max(err))}
+        sim.close()
+        out[name] = r
+        print(f"[bench] variant {name}: {r}", file=sys.stderr, flush=True)
+    return out
 
 
 def main() -> int:
@@ -168,6 +224,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-variants", action="store_true", help="skip the T3 / S1xR2 variant leg")
     ap.add_argument("--serving-requests", type=int, default=160,
                     help="length of the seeded Poisson request stream of the serving leg")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -251,6 +308,9 @@ def main() -> int:
             "roofline": {"bound": "valu", "achieved": ach_m, "peak": PEAK_FP32_VECTOR_TFLOPS,
                          "unit": "TFLOP/s", "frac": ach_m / PEAK_FP32_VECTOR_TFLOPS},
         }
+
+    if not args.no_variants:
+        out["variants"] = variants_leg(max(1, min(args.steps, 3)))
 
     if not args.no_serving:
         out["serving"] = serving_leg(args.serving_requests)

@@ -27,7 +27,8 @@ LIBNAME = "libsteps_amd.so"
 SHIM_VARIANTS = {"r3": [], "t3": ["-DPERIODIC"], "s1r2": ["-DPERIODIC_Z", "-DEWALD_INTERPOLATION_ORDER=4"],
                  "s1r2_nolookup": ["-DPERIODIC_Z", "-DPERIODIC_Z_NOLOOKUP"]}
 
-HIP_SOURCES = ["pair_kernels.hip", "engine.hip", "kdk_kernels.hip", "table_kernels.hip", "server.hip"]
+HIP_SOURCES = ["pair_kernels.hip", "engine.hip", "kdk_kernels.hip", "table_kernels.hip", "order_kernels.hip",
+               "server.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

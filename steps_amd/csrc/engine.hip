@@ -170,13 +170,11 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
     default:
         return fail(STEPS_ERR_ARG, "unknown topology");
     }
-    if (p.precision == STEPS_PREC_MIXED && p.variant != PV_R3)
-        return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3 only");
+    if (p.precision == STEPS_PREC_MIXED && !mixed_supported(p.variant))
+        return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3 and S1R2 (nearest z-image, CIC or TSC correction)");
 
     // split the source range so that (row blocks) x (splits) fills the chip
-    const int ni = (p.variant == PV_T3_EWALD || (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC))
-                       ? 1
-                       : (p.precision == STEPS_PREC_MIXED ? kRowsPerLaneMixed : kRowsPerLane);
+    const int ni = rows_per_lane(p.variant, p.precision);
     const int64_t rows_per_block = (int64_t)kBlock * ni;
     const int64_t nib = std::max<int64_t>(1, (p.nrows + rows_per_block - 1) / rows_per_block);
     const int64_t target = (int64_t)cu_count(d.device) * 8;
@@ -191,7 +189,7 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
 }
 
 size_t scratch_bytes(const EvalPlan &p) {
-    return ((partial_bytes(p) + 255) & ~(size_t)255) + tile_extra_bytes(p.N, p.precision);
+    return ((partial_bytes(p) + 255) & ~(size_t)255) + tile_extra_bytes(p);
 }
 
 int set_device(int device) {
@@ -218,7 +216,7 @@ int sim_create(const steps_force_desc &d, int64_t N, bool both_precisions, steps
     if (N <= 0) return fail(STEPS_ERR_ARG, "N must be positive");
     EvalPlan p;
     steps_force_desc dd = d;
-    if (both_precisions && d.topology == STEPS_TOPO_R3) dd.precision = STEPS_PREC_MIXED;
+    if (both_precisions && desc_mixed_capable(d)) dd.precision = STEPS_PREC_MIXED;
     int rc = make_plan(dd, N, 0, N - 1, p);
     if (rc) return rc;
     if ((rc = set_device(d.device))) return rc;
@@ -486,8 +484,8 @@ int steps_sim_destroy(steps_sim *s) {
 int steps_sim_set_precision(steps_sim *s, int32_t precision) {
     if (!s) return fail(STEPS_ERR_ARG, "null sim");
     if (precision != STEPS_PREC_FP64 && precision != STEPS_PREC_MIXED) return fail(STEPS_ERR_ARG, "unknown precision");
-    if (precision == STEPS_PREC_MIXED && s->desc.topology != STEPS_TOPO_R3)
-        return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3 only");
+    if (precision == STEPS_PREC_MIXED && !desc_mixed_capable(s->desc))
+        return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3 and S1R2 (nearest z-image, CIC or TSC correction)");
     s->desc.precision = precision;
     return STEPS_OK;
 }

@@ -29,6 +29,22 @@ enum PairVariant : int {
     PV_S1R2_IMAGES = 7,   // forces_cuda.cu:711-759   (PERIODIC_Z_NOLOOKUP, IS_PERIODIC >= 2)
 };
 
+constexpr bool is_s1r2_lookup(int v) { return v == PV_S1R2_NGP || v == PV_S1R2_CIC || v == PV_S1R2_TSC; }
+// Variants with a mixed-precision kernel: R3 and S1R2 with the nearest z-image
+// (plain, or with the CIC / TSC interpolated Ewald correction).  NGP is left
+// out: its correction is piecewise constant, so FP32 rounding of (rho, dz)
+// moves pairs near cell boundaries to the neighbouring node.
+constexpr bool mixed_supported(int v) {
+    return v == PV_R3 || v == PV_S1R2_NEAREST || v == PV_S1R2_CIC || v == PV_S1R2_TSC;
+}
+// i-particles per lane of each kernel instantiation: table variants gather
+// 8..192 table words per pair and keep one row per lane.
+constexpr int rows_per_lane(int v, int precision) {
+    return (v == PV_T3_EWALD || is_s1r2_lookup(v))
+               ? 1
+               : (precision == STEPS_PREC_MIXED ? kRowsPerLaneMixed : kRowsPerLane);
+}
+
 // Device-resident particle set in the layout the pair kernels read:
 // jrec[i] = (x, y, z, m) for wave-uniform scalar loads of the j-particle,
 // soft[i] = h.
@@ -61,9 +77,9 @@ struct EvalPlan {
 
 // Scratch of one evaluation: partial sums [splits][3][nrows] (FP64), followed
 // (256-byte aligned) by the source-tile table and, for STEPS_PREC_MIXED, the
-// FP32 tile-recentred source records.
+// FP32 tile-recentred source records and an FP32 copy of the S1R2 table.
 size_t partial_bytes(const EvalPlan &p);
-size_t tile_extra_bytes(int64_t N, int precision);
+size_t tile_extra_bytes(const EvalPlan &p);
 constexpr int kTile = 64;  // sources per tile (one wave64 prepares one tile)
 
 // Launch the tile pass, the pair kernel and the fixed-order reduction; F is
@@ -72,6 +88,12 @@ constexpr int kTile = 64;  // sources per tile (one wave64 prepares one tile)
 hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *partials,
                          double *F, hipStream_t stream, hipEvent_t ev_begin,
                          hipEvent_t ev_pair_end, hipEvent_t ev_end);
+
+// Spatial (Morton) ordering of n sources into sorted copies jrec_s / soft_s;
+// `scratch` holds order_scratch_bytes(n) bytes (order_kernels.hip).
+size_t order_scratch_bytes(int64_t n);
+hipError_t launch_spatial_order(const double4 *jrec, const double *soft, int64_t n, void *scratch,
+                                double4 *jrec_s, double *soft_s, hipStream_t stream);
 
 // AoS x [3n] + M [n] + soft [n] (device) -> DeviceParticles layout.
 hipError_t launch_pack(const double *x, const double *M, const double *soft, int64_t n,

@@ -44,6 +44,10 @@ struct KScalars {
     // Newton-correction constants, passed in so that they live in registers
     // for the whole loop instead of being rematerialised per pair
     double c15, c1875;
+    // reciprocals of the table grid spacings: T3 Ng/L, S1R2 1/drho and Nz/Lz
+    double inv_gs, inv_drho, inv_dzg;
+    // FP32 copies for the mixed-precision S1R2 pair
+    float Lf, halfLf, rho_maxf, inv_drhof, inv_dzgf;
 };
 
 // ---------------------------------------------------------------------------
@@ -94,12 +98,35 @@ __device__ __forceinline__ int imodp(int i, int n) {
     return (r < 0) ? (r + n) : r;
 }
 
+// imodp(i, n) without an integer division for i in [-n, 2n), which covers
+// every index a nearest-image displacement produces; other values (positions
+// outside the box) take the general remainder in a branch waves rarely enter.
+__device__ __forceinline__ int wrap_index(int i, int n) {
+    int r = (i < 0) ? i + n : ((i >= n) ? i - n : i);
+    if ((unsigned)r >= (unsigned)n) r = imodp(i, n);
+    return r;
+}
+
+// One period step for an index known to lie in [-n, 2n).
+__device__ __forceinline__ int wrap_once(int i, int n) {
+    return (i < 0) ? i + n : ((i >= n) ? i - n : i);
+}
+
+// x^(-1/2) for x > 0: v_rsq_f64 seed and a second-order Newton correction,
+// y * (1 - e)^(-1/2) ~= y * (1 + e/2 + 3e^2/8) with e = 1 - x*y^2.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x, y * y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 // forces_cuda.cu:93-167: tricubic Catmull-Rom interpolation of the T3 table
 // (cell-centred grid, periodic index wrap), table layout [(ix*N+iy)*N+iz]*3+c.
+// The reference divides by the grid spacing; multiplying by its reciprocal
+// moves u by at most an ulp, and the interpolant is continuous in u.
 __device__ __forceinline__ void t3_interp(const double *__restrict__ T, int Ng, double L,
-                                          double dx, double dy, double dz, double &Dx,
-                                          double &Dy, double &Dz) {
-    const double gs = L / (double)Ng;
+                                          double inv_gs, double dx, double dy, double dz,
+                                          double &Dx, double &Dy, double &Dz) {
     int ix[4], iy[4], iz[4];
     double wx[4], wy[4], wz[4];
     const double r3v[3] = {dx, dy, dz};
@@ -107,9 +134,9 @@ __device__ __forceinline__ void t3_interp(const double *__restrict__ T, int Ng, 
     double *wt[3] = {wx, wy, wz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const double u = (r3v[a] + L * 0.5) / gs - 0.5;
+        const double u = (r3v[a] + L * 0.5) * inv_gs - 0.5;
         const double uf = floor(u);
-        const int i0 = imodp((int)uf, Ng);
+        const int i0 = wrap_index((int)uf, Ng);
         const double t = u - uf;
         const double t2 = t * t, t3 = t * t2;
         wt[a][0] = -0.5 * t3 + t2 - 0.5 * t;
@@ -117,7 +144,7 @@ __device__ __forceinline__ void t3_interp(const double *__restrict__ T, int Ng, 
         wt[a][2] = -1.5 * t3 + 2.0 * t2 + 0.5 * t;
         wt[a][3] = 0.5 * t3 - 0.5 * t2;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) idx[a][q] = imodp(i0 - 1 + q, Ng);
+        for (int q = 0; q < 4; ++q) idx[a][q] = wrap_once(i0 - 1 + q, Ng);  // Ng >= 3
     }
     double s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
@@ -148,42 +175,40 @@ __device__ __forceinline__ double s1r2_at(const double *__restrict__ T, int Nz, 
 
 // forces_cuda.cu:188-455: NGP / CIC / TSC interpolation of (D_rho, D_z) at
 // (rho, z), rho clamped to [0, rho_max], z periodic; back to Cartesian.
+// Divisions by the grid spacings become multiplications by their reciprocals
+// and rho, 1/rho come from one reciprocal square root (each within an ulp or
+// two of the reference's sqrt and divides; the interpolants are continuous).
 template <int ORDER>
 __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, int Nrho, int Nz,
-                                            double rho_max, double Lz, double dx, double dy,
-                                            double dz, double &Dx, double &Dy, double &Dz) {
-    const double rho = sqrt(dx * dx + dy * dy);
+                                            double rho_max, double Lz, double inv_drho,
+                                            double inv_dzg, double dx, double dy, double dz,
+                                            double &Dx, double &Dy, double &Dz) {
+    const double rxy2 = fma(dx, dx, dy * dy);
+    const double inv_rho = (rxy2 > 0.0) ? rsqrt_f64(rxy2) : 0.0;
+    const double rho = rxy2 * inv_rho;
     const double half = 0.5 * Lz;
-    const double drho = rho_max / (double)max(1, Nrho - 1);
-    const double dzg = Lz / (double)Nz;
     double Dr, Dzv;
     if (ORDER == 0) {
         double rc = rho;
-        if (rho < 0.0) rc = 0.0;
         if (rho > rho_max) rc = rho_max;
-        const double ur = (drho > 0.0) ? (rc / drho) : 0.0;
-        const double uz = (dz + half) / dzg - 0.5;
+        const double ur = rc * inv_drho;  // inv_drho = 0 when drho = 0
+        const double uz = (dz + half) * inv_dzg - 0.5;
         int ir = (int)floor(ur + 0.5);
-        int iz = (int)floor(uz + 0.5);
+        const int iz = wrap_index((int)floor(uz + 0.5), Nz);
         ir = min(max(ir, 0), Nrho - 1);
-        iz = imodp(iz, Nz);
         Dr = s1r2_at(T, Nz, ir, iz, 0);
         Dzv = s1r2_at(T, Nz, ir, iz, 1);
     } else if (ORDER == 2) {
-        double ur;
-        if (rho < 0.0) ur = 0.0;
-        else if (rho > rho_max) ur = (double)(Nrho - 1);
-        else ur = rho / drho;
+        const double ur = (rho > rho_max) ? (double)(Nrho - 1) : rho * inv_drho;
         int ir0 = (int)floor(ur);
         double fr = ur - (double)ir0;
-        if (ir0 < 0) { ir0 = 0; fr = 0.0; }
         if (ir0 > Nrho - 2) { ir0 = max(0, Nrho - 2); fr = 1.0; }
         const int ir1 = ir0 + 1;
-        const double uz = (dz + half) / dzg - 0.5;
-        int iz0 = (int)floor(uz);
-        const double fz = uz - (double)iz0;
-        iz0 = imodp(iz0, Nz);
-        const int iz1 = imodp(iz0 + 1, Nz);
+        const double uz = (dz + half) * inv_dzg - 0.5;
+        const double uzf = floor(uz);
+        const double fz = uz - uzf;
+        const int iz0 = wrap_index((int)uzf, Nz);
+        const int iz1 = wrap_once(iz0 + 1, Nz);
         double w00 = 1.0 - fr; w00 *= (1.0 - fz);
         double w10 = fr; w10 *= (1.0 - fz);
         double w01 = 1.0 - fr; w01 *= fz;
@@ -193,19 +218,17 @@ __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, int Nr
         Dzv = w00 * s1r2_at(T, Nz, ir0, iz0, 1) + w10 * s1r2_at(T, Nz, ir1, iz0, 1) +
               w01 * s1r2_at(T, Nz, ir0, iz1, 1) + w11 * s1r2_at(T, Nz, ir1, iz1, 1);
     } else {
-        double ur;
-        if (rho < 0.0) ur = 0.0;
-        else if (rho > rho_max) ur = (double)(Nrho - 1);
-        else ur = rho / drho;
-        const double uz = (dz + half) / dzg - 0.5;
-        const int jr = (int)floor(ur + 0.5), jz = (int)floor(uz + 0.5);
-        const double sr = ur - (double)jr, sz = uz - (double)jz;
+        const double ur = (rho > rho_max) ? (double)(Nrho - 1) : rho * inv_drho;
+        const double uz = (dz + half) * inv_dzg - 0.5;
+        const double urf = floor(ur + 0.5), uzf = floor(uz + 0.5);
+        const int jr = (int)urf, jz = (int)uzf;
+        const double sr = ur - urf, sz = uz - uzf;
         const double wr[3] = {0.5 * (0.5 - sr) * (0.5 - sr), 0.75 - sr * sr,
                               0.5 * (0.5 + sr) * (0.5 + sr)};
         const double wz[3] = {0.5 * (0.5 - sz) * (0.5 - sz), 0.75 - sz * sz,
                               0.5 * (0.5 + sz) * (0.5 + sz)};
         const int irs[3] = {max(jr - 1, 0), min(max(jr, 0), Nrho - 1), min(jr + 1, Nrho - 1)};
-        const int izs[3] = {imodp(jz - 1, Nz), imodp(jz, Nz), imodp(jz + 1, Nz)};
+        const int izs[3] = {wrap_index(jz - 1, Nz), wrap_index(jz, Nz), wrap_index(jz + 1, Nz)};
         Dr = 0.0;
         Dzv = 0.0;
 #pragma unroll
@@ -217,10 +240,8 @@ __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, int Nr
                    w2 * s1r2_at(T, Nz, irs[2], izs[row], 1);
         }
     }
-    const double ex = (rho > 0.0) ? dx / rho : 0.0;
-    const double ey = (rho > 0.0) ? dy / rho : 0.0;
-    Dx = Dr * ex;
-    Dy = Dr * ey;
+    Dx = Dr * (dx * inv_rho);
+    Dy = Dr * (dy * inv_rho);
     Dz = Dzv;
 }
 
@@ -266,7 +287,7 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
             fz = fma(s, dz, fz);
         } else {
             double Dx, Dy, Dz;
-            t3_interp(table, k.ng, k.L, dx, dy, dz, Dx, Dy, Dz);
+            t3_interp(table, k.ng, k.L, k.inv_gs, dx, dy, dz, Dx, Dy, Dz);
             fx = fma(pj.w, fma(w, dx, -Dx), fx);
             fy = fma(pj.w, fma(w, dy, -Dy), fy);
             fz = fma(pj.w, fma(w, dz, -Dz), fz);
@@ -301,11 +322,11 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
         } else {
             double Dx, Dy, Dz;
             if (V == PV_S1R2_NGP)
-                s1r2_interp<0>(table, k.nrho, k.nz, k.rho_max, k.L, dx, dy, dz, Dx, Dy, Dz);
+                s1r2_interp<0>(table, k.nrho, k.nz, k.rho_max, k.L, k.inv_drho, k.inv_dzg, dx, dy, dz, Dx, Dy, Dz);
             else if (V == PV_S1R2_CIC)
-                s1r2_interp<2>(table, k.nrho, k.nz, k.rho_max, k.L, dx, dy, dz, Dx, Dy, Dz);
+                s1r2_interp<2>(table, k.nrho, k.nz, k.rho_max, k.L, k.inv_drho, k.inv_dzg, dx, dy, dz, Dx, Dy, Dz);
             else
-                s1r2_interp<4>(table, k.nrho, k.nz, k.rho_max, k.L, dx, dy, dz, Dx, Dy, Dz);
+                s1r2_interp<4>(table, k.nrho, k.nz, k.rho_max, k.L, k.inv_drho, k.inv_dzg, dx, dy, dz, Dx, Dy, Dz);
             fx = fma(pj.w, fma(w, dx, -Dx), fx);
             fy = fma(pj.w, fma(w, dy, -Dy), fy);
             fz = fma(pj.w, fma(w, dz, -Dz), fz);
@@ -409,8 +430,11 @@ __device__ __forceinline__ void mixed_pair(const float4 pj, const float xr, cons
     }
 }
 
+// Rows come from (irec, isoft) in the caller's order; sources from the
+// spatially ordered copies (jrec, soft) that jrel and the tiles describe.
 template <int NI, int UNR>
 __global__ __launch_bounds__(kBlock) void pair_kernel_mixed_r3(
+    const double4 *__restrict__ irec, const double *__restrict__ isoft,
     const double4 *__restrict__ jrec, const double *__restrict__ soft,
     const float4 *__restrict__ jrel, const double4 *__restrict__ tiles,
     double *__restrict__ part, const KScalars k0) {
@@ -426,11 +450,11 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_mixed_r3(
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
         const int64_t r = min(row_base + q * kBlock + tid, k.nrows - 1);
-        const double4 p = jrec[k.id_min + r];
+        const double4 p = irec[k.id_min + r];
         xi[q] = p.x;
         yi[q] = p.y;
         zi[q] = p.z;
-        hi[q] = soft[k.id_min + r];
+        hi[q] = isoft[k.id_min + r];
         fx[q] = fy[q] = fz[q] = 0.0;
     }
 
@@ -506,6 +530,199 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_mixed_r3(
             out[2 * k.nrows + r] = fz[q];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// mixed-precision S1xR2 kernel (nearest z-image, optionally with the tabulated
+// Ewald correction): the R3 scheme -- FP32 pair arithmetic on tile-recentred
+// positions, FP32 sums inside a tile, FP64 across tiles, FP64 redo of the
+// pairs inside the softening bound -- with the z-image wrap in FP32 and the
+// correction (D_rho, D_z) interpolated in FP32 from an FP32 copy of the table
+// (forces_cuda.cu:819-863 with :188-455).
+// ---------------------------------------------------------------------------
+constexpr int s1r2_order(int V) { return V == PV_S1R2_CIC ? 2 : 4; }
+
+template <int ORDER>
+__device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, const KScalars &k, float dx,
+                                                float dy, float dz, float &Dx, float &Dy, float &Dz) {
+    const int Nrho = k.nrho, Nz = k.nz;
+    const float rxy2 = fmaf(dx, dx, dy * dy);
+    const float inv_rho = (rxy2 > 0.0f) ? __builtin_amdgcn_rsqf(rxy2) : 0.0f;
+    const float rho = rxy2 * inv_rho;
+    const float uz = (dz + k.halfLf) * k.inv_dzgf - 0.5f;
+    float Dr, Dzv;
+    static_assert(ORDER == 2 || ORDER == 4, "FP32 interpolation is CIC or TSC");
+    if (ORDER == 2) {
+        const float ur = (rho > k.rho_maxf) ? (float)(Nrho - 1) : rho * k.inv_drhof;
+        int ir0 = (int)floorf(ur);
+        float fr = ur - (float)ir0;
+        if (ir0 > Nrho - 2) { ir0 = max(0, Nrho - 2); fr = 1.0f; }
+        const float uzf = floorf(uz);
+        const float fz = uz - uzf;
+        const int iz0 = wrap_index((int)uzf, Nz);
+        const int iz1 = wrap_once(iz0 + 1, Nz);
+        const float2 t00 = T[ir0 * Nz + iz0], t10 = T[(ir0 + 1) * Nz + iz0];
+        const float2 t01 = T[ir0 * Nz + iz1], t11 = T[(ir0 + 1) * Nz + iz1];
+        const float w00 = (1.0f - fr) * (1.0f - fz), w10 = fr * (1.0f - fz);
+        const float w01 = (1.0f - fr) * fz, w11 = fr * fz;
+        Dr = fmaf(w00, t00.x, fmaf(w10, t10.x, fmaf(w01, t01.x, w11 * t11.x)));
+        Dzv = fmaf(w00, t00.y, fmaf(w10, t10.y, fmaf(w01, t01.y, w11 * t11.y)));
+    } else {
+        const float ur = (rho > k.rho_maxf) ? (float)(Nrho - 1) : rho * k.inv_drhof;
+        const float urf = floorf(ur + 0.5f), uzf = floorf(uz + 0.5f);
+        const int jr = (int)urf, jz = (int)uzf;
+        const float sr = ur - urf, sz = uz - uzf;
+        const float wr[3] = {0.5f * (0.5f - sr) * (0.5f - sr), 0.75f - sr * sr, 0.5f * (0.5f + sr) * (0.5f + sr)};
+        const float wz[3] = {0.5f * (0.5f - sz) * (0.5f - sz), 0.75f - sz * sz, 0.5f * (0.5f + sz) * (0.5f + sz)};
+        const int irs[3] = {max(jr - 1, 0), min(max(jr, 0), Nrho - 1), min(jr + 1, Nrho - 1)};
+        const int izs[3] = {wrap_index(jz - 1, Nz), wrap_index(jz, Nz), wrap_index(jz + 1, Nz)};
+        Dr = 0.0f;
+        Dzv = 0.0f;
+#pragma unroll
+        for (int row = 0; row < 3; ++row)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float2 t = T[irs[c] * Nz + izs[row]];
+                const float w = wr[c] * wz[row];
+                Dr = fmaf(w, t.x, Dr);
+                Dzv = fmaf(w, t.y, Dzv);
+            }
+    }
+    Dx = Dr * (dx * inv_rho);
+    Dy = Dr * (dy * inv_rho);
+    Dz = Dzv;
+}
+
+// FP32 displacement with the nearest z-image (forces_cuda.cu:833-836) and r^2.
+__device__ __forceinline__ float mixed_r2_z(const KScalars &k, const float4 pj, float xr, float yr, float zr,
+                                            float &dx, float &dy, float &dz) {
+    dx = pj.x - xr;
+    dy = pj.y - yr;
+    dz = pj.z - zr;
+    if (dz > k.halfLf) dz -= k.Lf;
+    else if (dz < -k.halfLf) dz += k.Lf;
+    return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+}
+
+template <int V>
+__device__ __forceinline__ void mixed_pair_z(const KScalars &k, const float2 *__restrict__ T, const float4 pj,
+                                             float xr, float yr, float zr, float B, bool &near, float &ax,
+                                             float &ay, float &az) {
+    float dx, dy, dz;
+    const float r2 = mixed_r2_z(k, pj, xr, yr, zr, dx, dy, dz);
+    const bool nr = r2 < B;  // left to the FP64 redo (r2 may be 0 here)
+    near = near || nr;
+    const float y = __builtin_amdgcn_rsqf(r2);
+    const float y3 = nr ? 0.0f : (y * y) * y;
+    const float m = nr ? 0.0f : pj.w;
+    if (V == PV_S1R2_NEAREST) {
+        const float s = y3 * m;
+        ax = fmaf(s, dx, ax);
+        ay = fmaf(s, dy, ay);
+        az = fmaf(s, dz, az);
+    } else {
+        float Dx, Dy, Dz;
+        s1r2_interp_f32<s1r2_order(V)>(T, k, dx, dy, dz, Dx, Dy, Dz);
+        ax = fmaf(m, fmaf(y3, dx, -Dx), ax);
+        ay = fmaf(m, fmaf(y3, dy, -Dy), ay);
+        az = fmaf(m, fmaf(y3, dz, -Dz), az);
+    }
+}
+
+template <int V, int NI, int UNR>
+__global__ __launch_bounds__(kBlock) void pair_kernel_mixed_z(
+    const double4 *__restrict__ irec, const double *__restrict__ isoft,
+    const double4 *__restrict__ jrec, const double *__restrict__ soft, const float4 *__restrict__ jrel,
+    const double4 *__restrict__ tiles, const double *__restrict__ table, const float2 *__restrict__ tab32,
+    double *__restrict__ part, const KScalars k0) {
+    KScalars k = k0;
+    k.c15 = in_vgpr(k0.c15);
+    const int tid = threadIdx.x;
+    const int64_t row_base = (int64_t)blockIdx.x * (kBlock * NI);
+    const int split = blockIdx.y;
+    const int64_t j0 = (int64_t)split * k.jchunk;  // multiple of kTile
+    const int64_t j1 = min(k.N, j0 + k.jchunk);
+
+    double xi[NI], yi[NI], zi[NI], hi[NI], fx[NI], fy[NI], fz[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int64_t r = min(row_base + q * kBlock + tid, k.nrows - 1);
+        const double4 p = irec[k.id_min + r];
+        xi[q] = p.x;
+        yi[q] = p.y;
+        zi[q] = p.z;
+        hi[q] = isoft[k.id_min + r];
+        fx[q] = fy[q] = fz[q] = 0.0;
+    }
+
+    for (int64_t t0 = j0; t0 < j1; t0 += kTile) {
+        const double4 c = tiles[t0 / kTile];
+        float xr[NI], yr[NI], zr[NI], B[NI], ax[NI], ay[NI], az[NI];
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            xr[q] = (float)(xi[q] - c.x);
+            yr[q] = (float)(yi[q] - c.y);
+            zr[q] = (float)(zi[q] - c.z);
+            const float b = (float)((hi[q] + c.w) * (1.0 + 1e-5));
+            B[q] = b * b;
+            ax[q] = ay[q] = az[q] = 0.0f;
+        }
+        const int64_t tend = min(j1, t0 + kTile);
+        bool near = false;
+        int64_t j = t0;
+        for (; j + UNR <= tend; j += UNR) {
+            float4 pj[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) pj[u] = jrel[j + u];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int q = 0; q < NI; ++q)
+                    mixed_pair_z<V>(k, tab32, pj[u], xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q]);
+        }
+        for (; j < tend; ++j) {
+            const float4 pj = jrel[j];
+#pragma unroll
+            for (int q = 0; q < NI; ++q)
+                mixed_pair_z<V>(k, tab32, pj, xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q]);
+        }
+        if (__builtin_expect(near, 0)) {
+            for (int64_t jj = t0; jj < tend; ++jj) {
+                const float4 pr = jrel[jj];
+#pragma unroll
+                for (int q = 0; q < NI; ++q) {
+                    float dx, dy, dz;
+                    if (mixed_r2_z(k, pr, xr[q], yr[q], zr[q], dx, dy, dz) < B[q]) {
+                        const double b = hi[q] + c.w;
+                        pair_f64<V>(k, table, soft, jj, jrec[jj], xi[q], yi[q], zi[q], hi[q], b * b, fx[q],
+                                    fy[q], fz[q]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            fx[q] += (double)ax[q];
+            fy[q] += (double)ay[q];
+            fz[q] += (double)az[q];
+        }
+    }
+
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int64_t r = row_base + q * kBlock + tid;
+        if (r < k.nrows) {
+            double *out = part + (size_t)split * 3u * (size_t)k.nrows;
+            out[r] = fx[q];
+            out[k.nrows + r] = fy[q];
+            out[2 * k.nrows + r] = fz[q];
+        }
+    }
+}
+
+__global__ void table_f32_kernel(const double *__restrict__ T, float2 *__restrict__ T32, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) T32[i] = make_float2((float)T[2 * i], (float)T[2 * i + 1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -626,24 +843,29 @@ hipError_t launch_variant(const EvalPlan &p, const DeviceParticles &dp, double *
                           const double4 *tiles, hipStream_t st, const KScalars &k,
                           const double *table);
 
-// Table variants gather 8..192 table words per pair: one row per lane keeps
-// them out of spill territory; the arithmetic variants hold kRowsPerLane rows.
-constexpr int rows_per_lane(int V) {
-    return (V == PV_T3_EWALD || V == PV_S1R2_NGP || V == PV_S1R2_CIC || V == PV_S1R2_TSC)
-               ? 1
-               : kRowsPerLane;
-}
-
 template <int V>
 hipError_t launch_variant(const EvalPlan &p, const DeviceParticles &dp, double *part,
                           const double4 *tiles, hipStream_t st, const KScalars &k,
                           const double *table) {
-    constexpr int NI = rows_per_lane(V);
+    constexpr int NI = rows_per_lane(V, STEPS_PREC_FP64);
     constexpr int UNR = (V == PV_R3 || V == PV_T3_NEAREST || V == PV_S1R2_NEAREST) ? 4 : 1;
     const int64_t rows_per_block = (int64_t)kBlock * NI;
     dim3 grid((unsigned)((p.nrows + rows_per_block - 1) / rows_per_block), (unsigned)p.splits);
     hipLaunchKernelGGL((pair_kernel_f64<V, NI, UNR>), grid, dim3(kBlock), 0, st, dp.jrec, dp.soft,
                        tiles, table, part, k);
+    return hipGetLastError();
+}
+
+template <int V>
+hipError_t launch_mixed_z(const EvalPlan &p, const DeviceParticles &dp, const double4 *jrec_s,
+                          const double *soft_s, double *part, const double4 *tiles, const float4 *jrel,
+                          const float2 *tab32, hipStream_t st, const KScalars &k) {
+    constexpr int NI = rows_per_lane(V, STEPS_PREC_MIXED);
+    constexpr int UNR = (V == PV_S1R2_NEAREST) ? 8 : 2;
+    const int64_t rows_per_block = (int64_t)kBlock * NI;
+    dim3 grid((unsigned)((p.nrows + rows_per_block - 1) / rows_per_block), (unsigned)p.splits);
+    hipLaunchKernelGGL((pair_kernel_mixed_z<V, NI, UNR>), grid, dim3(kBlock), 0, st, dp.jrec, dp.soft, jrec_s,
+                       soft_s, jrel, tiles, p.s1r2, tab32, part, k);
     return hipGetLastError();
 }
 
@@ -656,11 +878,25 @@ size_t partial_bytes(const EvalPlan &p) {
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Scratch after the partial sums (same buffer, 256-byte aligned pieces):
-// tiles [ceil(N/64)] double4, then for the mixed path jrel [N] float4.
-size_t tile_extra_bytes(int64_t N, int precision) {
-    const int64_t nt = (N + kTile - 1) / kTile;
+// tiles [ceil(N/64)] double4, then for the mixed path jrel [N] float4 and,
+// for the S1R2 table variants, the FP32 table [nrho*nz] float2.
+static size_t s1r2_f32_entries(const EvalPlan &p) {
+    return (p.precision == STEPS_PREC_MIXED && is_s1r2_lookup(p.variant)) ? (size_t)p.s1r2_nrho * p.s1r2_nz
+                                                                          : 0;
+}
+
+// The mixed path also holds the spatially ordered source copies (jrec_s
+// double4 [N], soft_s double [N]) and the ordering's own scratch.
+size_t tile_extra_bytes(const EvalPlan &p) {
+    const int64_t nt = (p.N + kTile - 1) / kTile;
+    const size_t n = (size_t)std::max<int64_t>(p.N, 1);
     size_t b = align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4));
-    if (precision == STEPS_PREC_MIXED) b += align256((size_t)std::max<int64_t>(N, 1) * sizeof(float4));
+    if (p.precision == STEPS_PREC_MIXED) {
+        b += align256(n * sizeof(float4));
+        b += align256(s1r2_f32_entries(p) * sizeof(float2));
+        b += align256(n * sizeof(double4)) + align256(n * sizeof(double));
+        b += align256(order_scratch_bytes(p.N));
+    }
     return b + 256;
 }
 
@@ -685,24 +921,58 @@ hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *p
         k.image_cut = (((double)p.images) - 0.4) * p.L;
         k.c15 = 1.5;
         k.c1875 = 1.875;
+        k.inv_gs = (p.t3_ng > 0 && p.L > 0) ? 1.0 / (p.L / (double)p.t3_ng) : 0.0;
+        const double drho = k.rho_max / (double)std::max(1, p.s1r2_nrho - 1);
+        k.inv_drho = (drho > 0.0) ? 1.0 / drho : 0.0;
+        k.inv_dzg = (p.s1r2_nz > 0 && p.L > 0) ? 1.0 / (p.L / (double)p.s1r2_nz) : 0.0;
+        k.Lf = (float)p.L;
+        k.halfLf = (float)(0.5 * p.L);
+        k.rho_maxf = (float)k.rho_max;
+        k.inv_drhof = (float)k.inv_drho;
+        k.inv_dzgf = (float)k.inv_dzg;
         const int64_t nt = (p.N + kTile - 1) / kTile;
-        const bool mixed = (p.precision == STEPS_PREC_MIXED && p.variant == PV_R3);
+        const bool mixed = (p.precision == STEPS_PREC_MIXED && mixed_supported(p.variant));
         double4 *tiles = (double4 *)((char *)partials + align256(partial_bytes(p)));
         float4 *jrel = (float4 *)((char *)tiles + align256((size_t)nt * sizeof(double4)));
-        if (mixed)
-            hipLaunchKernelGGL((tiles_kernel<true>), dim3((unsigned)nt), dim3(kTile), 0, st, dp.jrec, dp.soft,
+        float2 *tab32 = (float2 *)((char *)jrel + align256((size_t)p.N * sizeof(float4)));
+        const int64_t n32 = (int64_t)s1r2_f32_entries(p);
+        if (n32 > 0) {
+            hipLaunchKernelGGL(table_f32_kernel, dim3((unsigned)((n32 + 255) / 256)), dim3(256), 0, st, p.s1r2,
+                               tab32, n32);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        double4 *jrec_s = (double4 *)((char *)tab32 + align256(s1r2_f32_entries(p) * sizeof(float2)));
+        double *soft_s = (double *)((char *)jrec_s + align256((size_t)p.N * sizeof(double4)));
+        void *order_scratch = (char *)soft_s + align256((size_t)p.N * sizeof(double));
+        if (mixed) {
+            if ((e = launch_spatial_order(dp.jrec, dp.soft, p.N, order_scratch, jrec_s, soft_s, st)) != hipSuccess)
+                return e;
+            hipLaunchKernelGGL((tiles_kernel<true>), dim3((unsigned)nt), dim3(kTile), 0, st, jrec_s, soft_s,
                                tiles, jrel, p.N);
-        else
+        } else
             hipLaunchKernelGGL((tiles_kernel<false>), dim3((unsigned)nt), dim3(kTile), 0, st, dp.jrec, dp.soft,
                                tiles, jrel, p.N);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (ev_begin && (e = hipEventRecord(ev_begin, st)) != hipSuccess) return e;
-        if (mixed) {
+        if (mixed && p.variant != PV_R3) {
+            switch (p.variant) {
+            case PV_S1R2_NEAREST:
+                e = launch_mixed_z<PV_S1R2_NEAREST>(p, dp, jrec_s, soft_s, partials, tiles, jrel, tab32, st, k);
+                break;
+            case PV_S1R2_CIC:
+                e = launch_mixed_z<PV_S1R2_CIC>(p, dp, jrec_s, soft_s, partials, tiles, jrel, tab32, st, k);
+                break;
+            case PV_S1R2_TSC:
+                e = launch_mixed_z<PV_S1R2_TSC>(p, dp, jrec_s, soft_s, partials, tiles, jrel, tab32, st, k);
+                break;
+            default: return hipErrorInvalidValue;
+            }
+        } else if (mixed) {
             constexpr int NI = kRowsPerLaneMixed;
             const int64_t rows_per_block = (int64_t)kBlock * NI;
             dim3 grid((unsigned)((p.nrows + rows_per_block - 1) / rows_per_block), (unsigned)p.splits);
-            hipLaunchKernelGGL((pair_kernel_mixed_r3<NI, 8>), grid, dim3(kBlock), 0, st, dp.jrec, dp.soft, jrel,
-                               tiles, partials, k);
+            hipLaunchKernelGGL((pair_kernel_mixed_r3<NI, 8>), grid, dim3(kBlock), 0, st, dp.jrec, dp.soft, jrec_s,
+                               soft_s, jrel, tiles, partials, k);
             e = hipGetLastError();
         } else {
             switch (p.variant) {

@@ -65,7 +65,7 @@ struct Slot {
 size_t footprint(const steps_force_desc &d, int64_t N) {
     EvalPlan p;
     steps_force_desc dd = d;
-    if (d.topology == STEPS_TOPO_R3) dd.precision = STEPS_PREC_MIXED;
+    if (desc_mixed_capable(d)) dd.precision = STEPS_PREC_MIXED;
     // tables only have to look present for the size computation
     static const double dummy = 0;
     if (!dd.t3_table) dd.t3_table = &dummy;

@@ -89,10 +89,6 @@ def make_requests(sims, nreq: int = 200, rate: float = 200.0, seed: int = 1004, 
     return reqs
 
 
-def r3_only_mixed(sim) -> bool:
-    return sim[0].topology == abi.TOPO_R3
-
-
 def _latency_stats(lat, t_first, t_last, n):
     lat = np.asarray(lat)
     return {"requests": n, "wall_s": t_last - t_first, "requests_per_s": n / (t_last - t_first),

@@ -150,3 +150,47 @@ def test_s1r2_full_size_sampled_rows(steps, oracle):
         e = rel_err(F[rows], Fo)
         report(f"S1R2 N={w.N} order={order} sampled", e)
         assert np.median(e) <= 1e-12
+
+
+def test_s1r2_ngp_has_no_mixed_path(steps, oracle, cyl):
+    w, T, shape = cyl
+    d, keep = steps.make_desc(steps.TOPO_S1R2, is_periodic=2, L=w.L, Rsim=w.Rsim, s1r2_lookup=1, interp_order=0,
+                              s1r2_table=T, s1r2_shape=shape, precision=1)
+    with pytest.raises(steps.StepsError):
+        steps.force_eval(d, w.x, w.M, w.soft)
+
+
+@pytest.mark.parametrize("order", [2, 4, "nearest"])
+def test_s1r2_mixed_precision_error(steps, oracle, cyl, order):
+    """FP32 pair arithmetic (nearest z-image, FP32-interpolated Ewald
+    correction) with FP64 accumulation across tiles, against the FP64 path and
+    the oracle: p99 relative error <= 1e-5 for the pair sum alone (cosmology
+    off).  With the comoving term the total force of a particle near the axis
+    is a small difference of the pair sum and the background term
+    2 pi rho_c Omega_m (x, y, 0), so the same absolute error is a larger
+    relative one: p99 <= 3e-5 there."""
+    print(f"case s1r2 mixed order={order}", flush=True)
+    w, T, shape = cyl
+    for cosmology in (0, 1):
+        if order == "nearest":
+            radial = oracle.radial_table(w.Rsim, 0.5 * w.L, 1000, 1000)
+            kw = dict(is_periodic=1, L=w.L, Rsim=w.Rsim, cosmology=cosmology, comoving=cosmology,
+                      mass_in_unit_sphere=w.mass_in_unit_sphere, radial_table=radial)
+        else:
+            kw = dict(is_periodic=2, L=w.L, Rsim=w.Rsim, cosmology=cosmology, comoving=cosmology,
+                      mass_in_unit_sphere=w.mass_in_unit_sphere, s1r2_lookup=1, interp_order=order,
+                      s1r2_table=T, s1r2_shape=shape)
+        d64, k1 = steps.make_desc(steps.TOPO_S1R2, precision=0, **kw)
+        dmx, k2 = steps.make_desc(steps.TOPO_S1R2, precision=1, **kw)
+        F64 = steps.force_eval(d64, w.x, w.M, w.soft)
+        Fmx = steps.force_eval(dmx, w.x, w.M, w.soft)
+        Fo = oracle.forces(oracle.params(2, **kw), w.x, w.M, w.soft)
+        e = rel_err(Fmx, F64)
+        eo = rel_err(Fmx, Fo)
+        report(f"S1R2 mixed order={order} cosmology={cosmology} vs fp64", e)
+        report(f"S1R2 mixed order={order} cosmology={cosmology} vs oracle", eo)
+        bound = 1e-5 if cosmology == 0 else 3e-5
+        assert np.percentile(e, 99) <= bound
+        assert np.percentile(eo, 99) <= bound
+        assert np.median(e) <= 2e-6
+        assert not np.array_equal(Fmx, F64)     # the FP32 path really ran

@@ -130,10 +130,10 @@ def test_admission_and_eviction_keep_results(ensemble):
 
 def test_mixed_precision_requests_match_alone(ensemble):
     sims, ref = ensemble
-    r3 = [s for s in sims if s[0].name == "r3_compactified"]
-    ref_mixed = [alone(w, d, precision=1) for (w, d, _) in r3]
+    capable = [s for k, s in enumerate(sims) if k != 4]     # all but the T3 simulation
+    ref_mixed = [alone(w, d, precision=1) for (w, d, _) in capable]
     srv = steps_amd.Server(streams=3)
-    ids, results, states = run_mix(r3, srv, precisions=[1] * len(r3))
+    ids, results, states = run_mix(capable, srv, precisions=[1] * len(capable))
     for i, (F0, errs, x, v, F) in enumerate(ref_mixed):
         assert [r.errmax for r in results[i][1:]] == errs
         assert np.array_equal(states[i][2], F)
@@ -143,7 +143,7 @@ def test_mixed_precision_requests_match_alone(ensemble):
 def test_failing_request_is_isolated(ensemble):
     sims, ref = ensemble
     srv = steps_amd.Server(streams=4)
-    c, d, _ = sims[-1]            # S1R2: the mixed path is not available for it
+    c, d, _ = sims[4]             # T3: it has no mixed-precision path
     w0, d0, _ = sims[0]
     bad = srv.add_sim(d, c.x, c.M, c.soft)
     good = srv.add_sim(d0, w0.x, w0.M, w0.soft, w0.v)
