@@ -47,8 +47,10 @@ enum steps_topology {
 
 enum steps_precision {
     STEPS_PREC_FP64 = 0, /* all pair arithmetic in FP64                      */
-    STEPS_PREC_MIXED = 1 /* FP32 pair arithmetic on tile-recentred positions,
-                            FP64 accumulation across tiles and corrections   */
+    STEPS_PREC_MIXED = 1 /* FP32 pair arithmetic on tile-recentred positions
+                            (spatially ordered sources), FP64 accumulation
+                            across tiles, FP64 for pairs inside the softening
+                            bound; R3 and S1R2 (nearest z-image, CIC, TSC)    */
 };
 
 /* Everything the reference force routines read from globals. */
@@ -102,8 +104,9 @@ int steps_sim_create(const steps_force_desc *d, int64_t N, steps_sim **out);
 int steps_sim_destroy(steps_sim *s);
 /* Switch the precision of later evaluations (STEPS_PREC_FP64 / _MIXED).  No
  * counterpart in the reference, which fixes the precision at compile time
- * (USE_SINGLE_PRECISION, global_variables.h:26-32); scratch of an R3 sim is
- * sized for both paths, so switching allocates nothing. */
+ * (USE_SINGLE_PRECISION, global_variables.h:26-32); the scratch of a sim
+ * whose variant has a mixed path is sized for both, so switching allocates
+ * nothing. */
 int steps_sim_set_precision(steps_sim *s, int32_t precision);
 /* x, v AoS [3N] (v may be NULL: zero), M and soft [N]; host memory. */
 int steps_sim_set_state(steps_sim *s, const double *x, const double *v, const double *M,
@@ -156,8 +159,10 @@ int steps_radial_table(double R, double Lz, int32_t size, int32_t accuracy, doub
  * batching: simulations are admitted into HBM by memory budget (idle ones are
  * evicted LRU to host memory and re-admitted with their state), up to
  * `streams` requests run concurrently (one per simulation at a time, in
- * submission order per simulation), and each simulation's force/step sequence
- * is replayed from a captured hipGraph.  Results do not depend on the mix. */
+ * submission order per simulation), each simulation's force/step sequence is
+ * replayed from a captured hipGraph, and the pair kernels of ready requests
+ * of small simulations (N <= pack_max_n, same variant and precision) are
+ * packed into one launch.  Results do not depend on the mix. */
 typedef struct steps_server steps_server;
 
 typedef struct steps_server_config {
@@ -165,6 +170,7 @@ typedef struct steps_server_config {
     int32_t streams;           /* concurrent requests on the device, 1..16 (default 4) */
     int32_t use_graphs;        /* 1: replay captured per-simulation graphs (default)    */
     int64_t hbm_budget_bytes;  /* resident simulations' state + scratch (default 200 GiB) */
+    int64_t pack_max_n;        /* pack simulations with N <= this (default 65536; 0: never) */
 } steps_server_config;
 
 enum steps_request_kind { STEPS_REQ_FORCE = 0, STEPS_REQ_STEP = 1 };
@@ -182,6 +188,7 @@ typedef struct steps_server_stats {
     int64_t submitted, completed, failed;
     int64_t admissions, evictions, graph_captures;
     int64_t resident_sims, bytes_resident, max_backlog;
+    int64_t packed_launches, packed_requests;  /* launches holding >= 2 requests, and those requests */
 } steps_server_stats;
 
 void steps_server_config_init(steps_server_config *c);

@@ -86,7 +86,7 @@ class StepParams(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("streams", C.c_int32), ("use_graphs", C.c_int32),
-                ("hbm_budget_bytes", C.c_int64)]
+                ("hbm_budget_bytes", C.c_int64), ("pack_max_n", C.c_int64)]
 
 
 class RequestResult(C.Structure):
@@ -97,7 +97,8 @@ class RequestResult(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("submitted", "completed", "failed", "admissions", "evictions",
-                                        "graph_captures", "resident_sims", "bytes_resident", "max_backlog")]
+                                        "graph_captures", "resident_sims", "bytes_resident", "max_backlog",
+                                        "packed_launches", "packed_requests")]
 
 
 _lib = None
@@ -342,12 +343,14 @@ def softening_lengths(M, particle_radius: float):
 class Server:
     """Continuous-batching request loop (steps_server)."""
 
-    def __init__(self, device=0, streams=4, use_graphs=True, hbm_budget_bytes=200 << 30):
+    def __init__(self, device=0, streams=4, use_graphs=True, hbm_budget_bytes=200 << 30, pack_max_n=None):
         self._lib = load_library()
         cfg = ServerConfig()
         self._lib.steps_server_config_init(C.byref(cfg))
         cfg.device, cfg.streams, cfg.use_graphs, cfg.hbm_budget_bytes = device, streams, int(use_graphs), \
             int(hbm_budget_bytes)
+        if pack_max_n is not None:
+            cfg.pack_max_n = int(pack_max_n)
         h = C.c_void_p()
         _check(self._lib.steps_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -310,24 +310,33 @@ hipError_t sim_enqueue_forces(steps_sim *s, const EvalPlan &p, hipStream_t st, b
                          events ? s->ev1 : nullptr, events ? s->ev2 : nullptr);
 }
 
-hipError_t sim_enqueue_step(steps_sim *s, const EvalPlan &p, hipStream_t st) {
-    // step.cc:100-312: params in, kick+drift+wrap, forces, kick+errmax, errmax out
-    const KdkParams *dpar = s->dparams.as<const KdkParams>();
+hipError_t sim_enqueue_step_before(steps_sim *s, hipStream_t st) {
+    // step.cc:128-181: params in, kick + drift + periodic wrap
     hipError_t e;
     if ((e = hipMemcpyAsync(s->dparams.ptr, s->hparams, sizeof(KdkParams), hipMemcpyHostToDevice, st)) != hipSuccess)
         return e;
-    if ((e = launch_kick_drift(s->dp, s->v.as<double>(), s->F.as<double>(), dpar, s->desc.topology, s->desc.L,
-                               st)) != hipSuccess)
-        return e;
+    return launch_kick_drift(s->dp, s->v.as<double>(), s->F.as<double>(), s->dparams.as<const KdkParams>(),
+                             s->desc.topology, s->desc.L, st);
+}
+
+hipError_t sim_enqueue_step_after(steps_sim *s, hipStream_t st) {
+    // step.cc:250-269: second kick + errmax, errmax out
+    const int nb = kdk_reduce_blocks(s->N);
+    double *bm = s->blockmax.as<double>();
+    hipError_t e = launch_kick_errmax(s->dp, s->v.as<double>(), s->F.as<double>(), s->dparams.as<const KdkParams>(),
+                                      bm, nb, bm + nb, st, true);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(s->herrmax, bm + nb, sizeof(double), hipMemcpyDeviceToHost, st);
+}
+
+hipError_t sim_enqueue_step(steps_sim *s, const EvalPlan &p, hipStream_t st) {
+    // step.cc:100-312
+    hipError_t e;
+    if ((e = sim_enqueue_step_before(s, st)) != hipSuccess) return e;
     if ((e = launch_forces(p, s->dp, s->part.as<double>(), s->F.as<double>(), st, nullptr, nullptr, nullptr)) !=
         hipSuccess)
         return e;
-    const int nb = kdk_reduce_blocks(s->N);
-    double *bm = s->blockmax.as<double>();
-    if ((e = launch_kick_errmax(s->dp, s->v.as<double>(), s->F.as<double>(), dpar, bm, nb, bm + nb, st, true)) !=
-        hipSuccess)
-        return e;
-    return hipMemcpyAsync(s->herrmax, bm + nb, sizeof(double), hipMemcpyDeviceToHost, st);
+    return sim_enqueue_step_after(s, st);
 }
 
 int sim_fill_params(steps_sim *s, const steps_step_params &sp) {

@@ -88,6 +88,28 @@ constexpr int kTile = 64;  // sources per tile (one wave64 prepares one tile)
 hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *partials,
                          double *F, hipStream_t stream, hipEvent_t ev_begin,
                          hipEvent_t ev_pair_end, hipEvent_t ev_end);
+// The same in three phases: source preparation (tiles; for the mixed path the
+// FP32 table copy, spatial ordering and recentred records), the pair kernel,
+// and the fixed-order reduction with the external term.
+hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, double *partials,
+                                 hipStream_t stream);
+hipError_t launch_forces_pairs(const EvalPlan &p, const DeviceParticles &dp, double *partials,
+                               hipStream_t stream);
+hipError_t launch_forces_finish(const EvalPlan &p, const DeviceParticles &dp, double *partials, double *F,
+                                hipStream_t stream);
+
+// Packed launches: the pair kernels of up to kMaxPack evaluations of one
+// (variant, precision) in one grid.  pack_describe writes the descriptor of
+// evaluation idx (host memory, pack_desc_bytes() each) whose workgroups start
+// at blk_begin and returns its workgroup count; launch_pair_packed runs the
+// descriptors after they have been copied to device memory.  Each evaluation
+// runs exactly the kernel body, plan and reduction order it runs alone.
+constexpr int kMaxPack = 16;
+size_t pack_desc_bytes();
+int64_t pack_describe(const EvalPlan &p, const DeviceParticles &dp, double *partials, void *host_descs, int idx,
+                      int64_t blk_begin);
+hipError_t launch_pair_packed(int variant, int precision, const void *dev_descs, int nseg, int64_t total_blocks,
+                              hipStream_t stream);
 
 // Spatial (Morton) ordering of n sources into sorted copies jrec_s / soft_s;
 // `scratch` holds order_scratch_bytes(n) bytes (order_kernels.hip).

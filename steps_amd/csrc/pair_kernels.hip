@@ -50,6 +50,30 @@ struct KScalars {
     float Lf, halfLf, rho_maxf, inv_drhof, inv_dzgf;
 };
 
+// Everything one evaluation's pair kernel reads: rows (caller order),
+// sources (the same arrays for FP64; spatially ordered copies for the mixed
+// path, which jrel and the tiles describe), tables, partial sums, scalars.
+struct PairArgs {
+    const double4 *irec;
+    const double *isoft;
+    const double4 *jrec;
+    const double *soft;
+    const float4 *jrel;
+    const double4 *tiles;
+    const double *table;
+    const float2 *tab32;
+    double *part;
+    KScalars k;
+};
+
+// One evaluation inside a packed launch: its arguments, its row-block count
+// and the first flat workgroup index it owns.
+struct PackSeg {
+    PairArgs a;
+    int nbx;
+    int blk_begin;
+};
+
 // ---------------------------------------------------------------------------
 // FP64 building blocks
 // ---------------------------------------------------------------------------
@@ -168,9 +192,11 @@ __device__ __forceinline__ void t3_interp(const double *__restrict__ T, int Ng, 
     Dz = s2;
 }
 
-__device__ __forceinline__ double s1r2_at(const double *__restrict__ T, int Nz, int ir, int iz,
-                                          int c) {
-    return T[((size_t)ir * (size_t)Nz + (size_t)iz) * 2u + (size_t)c];
+// (D_rho, D_z) of node (ir, iz): one 16-byte load at a 32-bit byte offset
+// from the table base (the table has fewer than 2^28 nodes).
+__device__ __forceinline__ double2 s1r2_node(const double *__restrict__ T, int Nz, int ir, int iz) {
+    const unsigned off = ((unsigned)ir * (unsigned)Nz + (unsigned)iz) * 16u;
+    return *(const double2 *)((const char *)T + off);
 }
 
 // forces_cuda.cu:188-455: NGP / CIC / TSC interpolation of (D_rho, D_z) at
@@ -179,47 +205,45 @@ __device__ __forceinline__ double s1r2_at(const double *__restrict__ T, int Nz, 
 // and rho, 1/rho come from one reciprocal square root (each within an ulp or
 // two of the reference's sqrt and divides; the interpolants are continuous).
 template <int ORDER>
-__device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, int Nrho, int Nz,
-                                            double rho_max, double Lz, double inv_drho,
-                                            double inv_dzg, double dx, double dy, double dz,
-                                            double &Dx, double &Dy, double &Dz) {
+__device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, const KScalars &k, double dx, double dy,
+                                            double dz, double &Dx, double &Dy, double &Dz) {
+    const int Nrho = k.nrho, Nz = k.nz;
+    const double rho_max = k.rho_max;
     const double rxy2 = fma(dx, dx, dy * dy);
     const double inv_rho = (rxy2 > 0.0) ? rsqrt_f64(rxy2) : 0.0;
     const double rho = rxy2 * inv_rho;
-    const double half = 0.5 * Lz;
+    const double uz = fma(dz + k.halfL, k.inv_dzg, -0.5);
     double Dr, Dzv;
     if (ORDER == 0) {
         double rc = rho;
         if (rho > rho_max) rc = rho_max;
-        const double ur = rc * inv_drho;  // inv_drho = 0 when drho = 0
-        const double uz = (dz + half) * inv_dzg - 0.5;
+        const double ur = rc * k.inv_drho;  // inv_drho = 0 when drho = 0
         int ir = (int)floor(ur + 0.5);
         const int iz = wrap_index((int)floor(uz + 0.5), Nz);
         ir = min(max(ir, 0), Nrho - 1);
-        Dr = s1r2_at(T, Nz, ir, iz, 0);
-        Dzv = s1r2_at(T, Nz, ir, iz, 1);
+        const double2 t = s1r2_node(T, Nz, ir, iz);
+        Dr = t.x;
+        Dzv = t.y;
     } else if (ORDER == 2) {
-        const double ur = (rho > rho_max) ? (double)(Nrho - 1) : rho * inv_drho;
+        const double ur = (rho > rho_max) ? (double)(Nrho - 1) : rho * k.inv_drho;
         int ir0 = (int)floor(ur);
         double fr = ur - (double)ir0;
         if (ir0 > Nrho - 2) { ir0 = max(0, Nrho - 2); fr = 1.0; }
         const int ir1 = ir0 + 1;
-        const double uz = (dz + half) * inv_dzg - 0.5;
         const double uzf = floor(uz);
         const double fz = uz - uzf;
         const int iz0 = wrap_index((int)uzf, Nz);
         const int iz1 = wrap_once(iz0 + 1, Nz);
+        const double2 t00 = s1r2_node(T, Nz, ir0, iz0), t10 = s1r2_node(T, Nz, ir1, iz0);
+        const double2 t01 = s1r2_node(T, Nz, ir0, iz1), t11 = s1r2_node(T, Nz, ir1, iz1);
         double w00 = 1.0 - fr; w00 *= (1.0 - fz);
         double w10 = fr; w10 *= (1.0 - fz);
         double w01 = 1.0 - fr; w01 *= fz;
         const double w11 = fr * fz;
-        Dr = w00 * s1r2_at(T, Nz, ir0, iz0, 0) + w10 * s1r2_at(T, Nz, ir1, iz0, 0) +
-             w01 * s1r2_at(T, Nz, ir0, iz1, 0) + w11 * s1r2_at(T, Nz, ir1, iz1, 0);
-        Dzv = w00 * s1r2_at(T, Nz, ir0, iz0, 1) + w10 * s1r2_at(T, Nz, ir1, iz0, 1) +
-              w01 * s1r2_at(T, Nz, ir0, iz1, 1) + w11 * s1r2_at(T, Nz, ir1, iz1, 1);
+        Dr = w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x;
+        Dzv = w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y;
     } else {
-        const double ur = (rho > rho_max) ? (double)(Nrho - 1) : rho * inv_drho;
-        const double uz = (dz + half) * inv_dzg - 0.5;
+        const double ur = (rho > rho_max) ? (double)(Nrho - 1) : rho * k.inv_drho;
         const double urf = floor(ur + 0.5), uzf = floor(uz + 0.5);
         const int jr = (int)urf, jz = (int)uzf;
         const double sr = ur - urf, sz = uz - uzf;
@@ -228,16 +252,17 @@ __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, int Nr
         const double wz[3] = {0.5 * (0.5 - sz) * (0.5 - sz), 0.75 - sz * sz,
                               0.5 * (0.5 + sz) * (0.5 + sz)};
         const int irs[3] = {max(jr - 1, 0), min(max(jr, 0), Nrho - 1), min(jr + 1, Nrho - 1)};
-        const int izs[3] = {wrap_index(jz - 1, Nz), wrap_index(jz, Nz), wrap_index(jz + 1, Nz)};
+        const int izm = wrap_index(jz, Nz);
+        const int izs[3] = {wrap_once(izm - 1, Nz), izm, wrap_once(izm + 1, Nz)};
         Dr = 0.0;
         Dzv = 0.0;
 #pragma unroll
         for (int row = 0; row < 3; ++row) {
             const double w0 = wr[0] * wz[row], w1 = wr[1] * wz[row], w2 = wr[2] * wz[row];
-            Dr += w0 * s1r2_at(T, Nz, irs[0], izs[row], 0) + w1 * s1r2_at(T, Nz, irs[1], izs[row], 0) +
-                  w2 * s1r2_at(T, Nz, irs[2], izs[row], 0);
-            Dzv += w0 * s1r2_at(T, Nz, irs[0], izs[row], 1) + w1 * s1r2_at(T, Nz, irs[1], izs[row], 1) +
-                   w2 * s1r2_at(T, Nz, irs[2], izs[row], 1);
+            const double2 a = s1r2_node(T, Nz, irs[0], izs[row]), b = s1r2_node(T, Nz, irs[1], izs[row]),
+                          c = s1r2_node(T, Nz, irs[2], izs[row]);
+            Dr += w0 * a.x + w1 * b.x + w2 * c.x;
+            Dzv += w0 * a.y + w1 * b.y + w2 * c.y;
         }
     }
     Dx = Dr * (dx * inv_rho);
@@ -322,11 +347,11 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
         } else {
             double Dx, Dy, Dz;
             if (V == PV_S1R2_NGP)
-                s1r2_interp<0>(table, k.nrho, k.nz, k.rho_max, k.L, k.inv_drho, k.inv_dzg, dx, dy, dz, Dx, Dy, Dz);
+                s1r2_interp<0>(table, k, dx, dy, dz, Dx, Dy, Dz);
             else if (V == PV_S1R2_CIC)
-                s1r2_interp<2>(table, k.nrho, k.nz, k.rho_max, k.L, k.inv_drho, k.inv_dzg, dx, dy, dz, Dx, Dy, Dz);
+                s1r2_interp<2>(table, k, dx, dy, dz, Dx, Dy, Dz);
             else
-                s1r2_interp<4>(table, k.nrho, k.nz, k.rho_max, k.L, k.inv_drho, k.inv_dzg, dx, dy, dz, Dx, Dy, Dz);
+                s1r2_interp<4>(table, k, dx, dy, dz, Dx, Dy, Dz);
             fx = fma(pj.w, fma(w, dx, -Dx), fx);
             fy = fma(pj.w, fma(w, dy, -Dy), fy);
             fz = fma(pj.w, fma(w, dz, -Dz), fz);
@@ -340,17 +365,16 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
 // scalar loads and are applied to the NI rows of the lane.
 // ---------------------------------------------------------------------------
 template <int V, int NI, int UNR>
-__global__ __launch_bounds__(kBlock) void pair_kernel_f64(const double4 *__restrict__ jrec,
-                                                          const double *__restrict__ soft,
-                                                          const double4 *__restrict__ tiles,
-                                                          const double *__restrict__ table,
-                                                          double *__restrict__ part,
-                                                          const KScalars k0) {
-    KScalars k = k0;
-    k.c15 = in_vgpr(k0.c15);
+__device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const int split) {
+    const double4 *__restrict__ jrec = a.jrec;
+    const double *__restrict__ soft = a.soft;
+    const double4 *__restrict__ tiles = a.tiles;
+    const double *__restrict__ table = a.table;
+    double *__restrict__ part = a.part;
+    KScalars k = a.k;
+    k.c15 = in_vgpr(a.k.c15);
     const int tid = threadIdx.x;
-    const int64_t row_base = (int64_t)blockIdx.x * (kBlock * NI);
-    const int split = blockIdx.y;
+    const int64_t row_base = (int64_t)bx * (kBlock * NI);
     const int64_t j0 = (int64_t)split * k.jchunk;  // multiple of kTile
     const int64_t j1 = min(k.N, j0 + k.jchunk);
 
@@ -433,16 +457,18 @@ __device__ __forceinline__ void mixed_pair(const float4 pj, const float xr, cons
 // Rows come from (irec, isoft) in the caller's order; sources from the
 // spatially ordered copies (jrec, soft) that jrel and the tiles describe.
 template <int NI, int UNR>
-__global__ __launch_bounds__(kBlock) void pair_kernel_mixed_r3(
-    const double4 *__restrict__ irec, const double *__restrict__ isoft,
-    const double4 *__restrict__ jrec, const double *__restrict__ soft,
-    const float4 *__restrict__ jrel, const double4 *__restrict__ tiles,
-    double *__restrict__ part, const KScalars k0) {
-    KScalars k = k0;
-    k.c15 = in_vgpr(k0.c15);
+__device__ __forceinline__ void body_mixed_r3(const PairArgs &a, const int bx, const int split) {
+    const double4 *__restrict__ irec = a.irec;
+    const double *__restrict__ isoft = a.isoft;
+    const double4 *__restrict__ jrec = a.jrec;
+    const double *__restrict__ soft = a.soft;
+    const float4 *__restrict__ jrel = a.jrel;
+    const double4 *__restrict__ tiles = a.tiles;
+    double *__restrict__ part = a.part;
+    KScalars k = a.k;
+    k.c15 = in_vgpr(a.k.c15);
     const int tid = threadIdx.x;
-    const int64_t row_base = (int64_t)blockIdx.x * (kBlock * NI);
-    const int split = blockIdx.y;
+    const int64_t row_base = (int64_t)bx * (kBlock * NI);
     const int64_t j0 = (int64_t)split * k.jchunk;  // multiple of kTile
     const int64_t j1 = min(k.N, j0 + k.jchunk);
 
@@ -542,6 +568,11 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_mixed_r3(
 // ---------------------------------------------------------------------------
 constexpr int s1r2_order(int V) { return V == PV_S1R2_CIC ? 2 : 4; }
 
+__device__ __forceinline__ float2 node32(const float2 *__restrict__ T, int Nz, int ir, int iz) {
+    const unsigned off = ((unsigned)ir * (unsigned)Nz + (unsigned)iz) * 8u;
+    return *(const float2 *)((const char *)T + off);
+}
+
 template <int ORDER>
 __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, const KScalars &k, float dx,
                                                 float dy, float dz, float &Dx, float &Dy, float &Dz) {
@@ -549,7 +580,7 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
     const float rxy2 = fmaf(dx, dx, dy * dy);
     const float inv_rho = (rxy2 > 0.0f) ? __builtin_amdgcn_rsqf(rxy2) : 0.0f;
     const float rho = rxy2 * inv_rho;
-    const float uz = (dz + k.halfLf) * k.inv_dzgf - 0.5f;
+    const float uz = fmaf(dz + k.halfLf, k.inv_dzgf, -0.5f);
     float Dr, Dzv;
     static_assert(ORDER == 2 || ORDER == 4, "FP32 interpolation is CIC or TSC");
     if (ORDER == 2) {
@@ -561,8 +592,8 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
         const float fz = uz - uzf;
         const int iz0 = wrap_index((int)uzf, Nz);
         const int iz1 = wrap_once(iz0 + 1, Nz);
-        const float2 t00 = T[ir0 * Nz + iz0], t10 = T[(ir0 + 1) * Nz + iz0];
-        const float2 t01 = T[ir0 * Nz + iz1], t11 = T[(ir0 + 1) * Nz + iz1];
+        const float2 t00 = node32(T, Nz, ir0, iz0), t10 = node32(T, Nz, ir0 + 1, iz0);
+        const float2 t01 = node32(T, Nz, ir0, iz1), t11 = node32(T, Nz, ir0 + 1, iz1);
         const float w00 = (1.0f - fr) * (1.0f - fz), w10 = fr * (1.0f - fz);
         const float w01 = (1.0f - fr) * fz, w11 = fr * fz;
         Dr = fmaf(w00, t00.x, fmaf(w10, t10.x, fmaf(w01, t01.x, w11 * t11.x)));
@@ -575,14 +606,15 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
         const float wr[3] = {0.5f * (0.5f - sr) * (0.5f - sr), 0.75f - sr * sr, 0.5f * (0.5f + sr) * (0.5f + sr)};
         const float wz[3] = {0.5f * (0.5f - sz) * (0.5f - sz), 0.75f - sz * sz, 0.5f * (0.5f + sz) * (0.5f + sz)};
         const int irs[3] = {max(jr - 1, 0), min(max(jr, 0), Nrho - 1), min(jr + 1, Nrho - 1)};
-        const int izs[3] = {wrap_index(jz - 1, Nz), wrap_index(jz, Nz), wrap_index(jz + 1, Nz)};
+        const int izm = wrap_index(jz, Nz);
+        const int izs[3] = {wrap_once(izm - 1, Nz), izm, wrap_once(izm + 1, Nz)};
         Dr = 0.0f;
         Dzv = 0.0f;
 #pragma unroll
         for (int row = 0; row < 3; ++row)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float2 t = T[irs[c] * Nz + izs[row]];
+                const float2 t = node32(T, Nz, irs[c], izs[row]);
                 const float w = wr[c] * wz[row];
                 Dr = fmaf(w, t.x, Dr);
                 Dzv = fmaf(w, t.y, Dzv);
@@ -630,16 +662,20 @@ __device__ __forceinline__ void mixed_pair_z(const KScalars &k, const float2 *__
 }
 
 template <int V, int NI, int UNR>
-__global__ __launch_bounds__(kBlock) void pair_kernel_mixed_z(
-    const double4 *__restrict__ irec, const double *__restrict__ isoft,
-    const double4 *__restrict__ jrec, const double *__restrict__ soft, const float4 *__restrict__ jrel,
-    const double4 *__restrict__ tiles, const double *__restrict__ table, const float2 *__restrict__ tab32,
-    double *__restrict__ part, const KScalars k0) {
-    KScalars k = k0;
-    k.c15 = in_vgpr(k0.c15);
+__device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, const int split) {
+    const double4 *__restrict__ irec = a.irec;
+    const double *__restrict__ isoft = a.isoft;
+    const double4 *__restrict__ jrec = a.jrec;
+    const double *__restrict__ soft = a.soft;
+    const float4 *__restrict__ jrel = a.jrel;
+    const double4 *__restrict__ tiles = a.tiles;
+    const double *__restrict__ table = a.table;
+    const float2 *__restrict__ tab32 = a.tab32;
+    double *__restrict__ part = a.part;
+    KScalars k = a.k;
+    k.c15 = in_vgpr(a.k.c15);
     const int tid = threadIdx.x;
-    const int64_t row_base = (int64_t)blockIdx.x * (kBlock * NI);
-    const int split = blockIdx.y;
+    const int64_t row_base = (int64_t)bx * (kBlock * NI);
     const int64_t j0 = (int64_t)split * k.jchunk;  // multiple of kTile
     const int64_t j1 = min(k.N, j0 + k.jchunk);
 
@@ -718,6 +754,36 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_mixed_z(
             out[2 * k.nrows + r] = fz[q];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// kernels: one evaluation per launch (grid row-blocks x splits), or several
+// evaluations of one variant packed into one launch (flat grid, each
+// workgroup finds its evaluation among at most kMaxPack segments)
+// ---------------------------------------------------------------------------
+enum BodyKind : int { BK_F64 = 0, BK_MIXED_R3 = 1, BK_MIXED_Z = 2 };
+
+template <int BK, int V, int NI, int UNR>
+__device__ __forceinline__ void run_body(const PairArgs &a, int bx, int by) {
+    if (BK == BK_F64) body_f64<V, NI, UNR>(a, bx, by);
+    else if (BK == BK_MIXED_R3) body_mixed_r3<NI, UNR>(a, bx, by);
+    else body_mixed_z<V, NI, UNR>(a, bx, by);
+}
+
+template <int BK, int V, int NI, int UNR>
+__global__ __launch_bounds__(kBlock) void pair_kernel(const PairArgs a) {
+    run_body<BK, V, NI, UNR>(a, blockIdx.x, blockIdx.y);
+}
+
+template <int BK, int V, int NI, int UNR>
+__global__ __launch_bounds__(kBlock) void pair_kernel_packed(const PackSeg *__restrict__ segs, int nseg) {
+    const int b = blockIdx.x;
+    int s = 0;
+    for (int q = 1; q < nseg; ++q)
+        if (segs[q].blk_begin <= b) s = q;
+    const PackSeg &g = segs[s];
+    const int local = b - g.blk_begin;
+    run_body<BK, V, NI, UNR>(g.a, local % g.nbx, local / g.nbx);
 }
 
 __global__ void table_f32_kernel(const double *__restrict__ T, float2 *__restrict__ T32, int64_t n) {
@@ -838,36 +904,134 @@ __global__ __launch_bounds__(kTile) void tiles_kernel(const double4 *__restrict_
         jrel[i] = make_float4((float)(p.x - cx), (float)(p.y - cy), (float)(p.z - cz), (float)p.w);
 }
 
-template <int V>
-hipError_t launch_variant(const EvalPlan &p, const DeviceParticles &dp, double *part,
-                          const double4 *tiles, hipStream_t st, const KScalars &k,
-                          const double *table);
+constexpr int unroll_of(int v, int precision) {
+    return precision == STEPS_PREC_MIXED ? ((v == PV_R3 || v == PV_S1R2_NEAREST) ? 8 : 2)
+                                         : ((v == PV_R3 || v == PV_T3_NEAREST || v == PV_S1R2_NEAREST) ? 4 : 1);
+}
 
-template <int V>
-hipError_t launch_variant(const EvalPlan &p, const DeviceParticles &dp, double *part,
-                          const double4 *tiles, hipStream_t st, const KScalars &k,
-                          const double *table) {
-    constexpr int NI = rows_per_lane(V, STEPS_PREC_FP64);
-    constexpr int UNR = (V == PV_R3 || V == PV_T3_NEAREST || V == PV_S1R2_NEAREST) ? 4 : 1;
-    const int64_t rows_per_block = (int64_t)kBlock * NI;
-    dim3 grid((unsigned)((p.nrows + rows_per_block - 1) / rows_per_block), (unsigned)p.splits);
-    hipLaunchKernelGGL((pair_kernel_f64<V, NI, UNR>), grid, dim3(kBlock), 0, st, dp.jrec, dp.soft,
-                       tiles, table, part, k);
+template <int BK, int V>
+hipError_t launch_pair_t(const PairArgs *a, const PackSeg *segs, int nseg, unsigned gx, unsigned gy,
+                         hipStream_t st) {
+    constexpr int prec = (BK == BK_F64) ? STEPS_PREC_FP64 : STEPS_PREC_MIXED;
+    constexpr int NI = rows_per_lane(V, prec);
+    constexpr int UNR = unroll_of(V, prec);
+    if (segs)
+        hipLaunchKernelGGL((pair_kernel_packed<BK, V, NI, UNR>), dim3(gx), dim3(kBlock), 0, st, segs, nseg);
+    else
+        hipLaunchKernelGGL((pair_kernel<BK, V, NI, UNR>), dim3(gx, gy), dim3(kBlock), 0, st, *a);
     return hipGetLastError();
 }
 
-template <int V>
-hipError_t launch_mixed_z(const EvalPlan &p, const DeviceParticles &dp, const double4 *jrec_s,
-                          const double *soft_s, double *part, const double4 *tiles, const float4 *jrel,
-                          const float2 *tab32, hipStream_t st, const KScalars &k) {
-    constexpr int NI = rows_per_lane(V, STEPS_PREC_MIXED);
-    constexpr int UNR = (V == PV_S1R2_NEAREST) ? 8 : 2;
-    const int64_t rows_per_block = (int64_t)kBlock * NI;
-    dim3 grid((unsigned)((p.nrows + rows_per_block - 1) / rows_per_block), (unsigned)p.splits);
-    hipLaunchKernelGGL((pair_kernel_mixed_z<V, NI, UNR>), grid, dim3(kBlock), 0, st, dp.jrec, dp.soft, jrec_s,
-                       soft_s, jrel, tiles, p.s1r2, tab32, part, k);
-    return hipGetLastError();
+// Runtime (variant, precision) -> kernel instantiation; one evaluation (a)
+// on a gx x gy grid, or nseg packed evaluations (segs, device) on gx blocks.
+hipError_t launch_pair(int variant, int precision, const PairArgs *a, const PackSeg *segs, int nseg, unsigned gx,
+                       unsigned gy, hipStream_t st) {
+    if (precision == STEPS_PREC_MIXED) {
+        switch (variant) {
+        case PV_R3: return launch_pair_t<BK_MIXED_R3, PV_R3>(a, segs, nseg, gx, gy, st);
+        case PV_S1R2_NEAREST: return launch_pair_t<BK_MIXED_Z, PV_S1R2_NEAREST>(a, segs, nseg, gx, gy, st);
+        case PV_S1R2_CIC: return launch_pair_t<BK_MIXED_Z, PV_S1R2_CIC>(a, segs, nseg, gx, gy, st);
+        case PV_S1R2_TSC: return launch_pair_t<BK_MIXED_Z, PV_S1R2_TSC>(a, segs, nseg, gx, gy, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    switch (variant) {
+    case PV_R3: return launch_pair_t<BK_F64, PV_R3>(a, segs, nseg, gx, gy, st);
+    case PV_T3_NEAREST: return launch_pair_t<BK_F64, PV_T3_NEAREST>(a, segs, nseg, gx, gy, st);
+    case PV_T3_EWALD: return launch_pair_t<BK_F64, PV_T3_EWALD>(a, segs, nseg, gx, gy, st);
+    case PV_S1R2_NEAREST: return launch_pair_t<BK_F64, PV_S1R2_NEAREST>(a, segs, nseg, gx, gy, st);
+    case PV_S1R2_NGP: return launch_pair_t<BK_F64, PV_S1R2_NGP>(a, segs, nseg, gx, gy, st);
+    case PV_S1R2_CIC: return launch_pair_t<BK_F64, PV_S1R2_CIC>(a, segs, nseg, gx, gy, st);
+    case PV_S1R2_TSC: return launch_pair_t<BK_F64, PV_S1R2_TSC>(a, segs, nseg, gx, gy, st);
+    case PV_S1R2_IMAGES: return launch_pair_t<BK_F64, PV_S1R2_IMAGES>(a, segs, nseg, gx, gy, st);
+    default: return hipErrorInvalidValue;
+    }
 }
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+bool is_mixed(const EvalPlan &p) { return p.precision == STEPS_PREC_MIXED && mixed_supported(p.variant); }
+
+size_t s1r2_f32_entries(const EvalPlan &p) {
+    return (is_mixed(p) && is_s1r2_lookup(p.variant)) ? (size_t)p.s1r2_nrho * p.s1r2_nz : 0;
+}
+
+// Pieces of an evaluation's scratch buffer, in the order tile_extra_bytes
+// counts them.
+struct ScratchView {
+    double *part;
+    double4 *tiles;
+    float4 *jrel;
+    float2 *tab32;
+    double4 *jrec_s;
+    double *soft_s;
+    void *order;
+};
+
+ScratchView scratch_view(const EvalPlan &p, double *partials) {
+    ScratchView v;
+    const int64_t nt = (p.N + kTile - 1) / kTile;
+    const size_t n = (size_t)std::max<int64_t>(p.N, 1);
+    v.part = partials;
+    v.tiles = (double4 *)((char *)partials + align256(partial_bytes(p)));
+    v.jrel = (float4 *)((char *)v.tiles + align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4)));
+    v.tab32 = (float2 *)((char *)v.jrel + align256(n * sizeof(float4)));
+    v.jrec_s = (double4 *)((char *)v.tab32 + align256(s1r2_f32_entries(p) * sizeof(float2)));
+    v.soft_s = (double *)((char *)v.jrec_s + align256(n * sizeof(double4)));
+    v.order = (char *)v.soft_s + align256(n * sizeof(double));
+    return v;
+}
+
+KScalars make_scalars(const EvalPlan &p) {
+    KScalars k{};
+    k.N = p.N;
+    k.id_min = p.id_min;
+    k.nrows = p.nrows;
+    k.jchunk = p.jchunk;
+    k.L = p.L;
+    k.halfL = 0.5 * p.L;
+    k.ng = p.t3_ng;
+    k.nrho = p.s1r2_nrho;
+    k.nz = p.s1r2_nz;
+    k.rho_max = kRadialExtent * p.Rsim;
+    k.images = p.images;
+    k.image_cut = (((double)p.images) - 0.4) * p.L;
+    k.c15 = 1.5;
+    k.c1875 = 1.875;
+    k.inv_gs = (p.t3_ng > 0 && p.L > 0) ? 1.0 / (p.L / (double)p.t3_ng) : 0.0;
+    const double drho = k.rho_max / (double)std::max(1, p.s1r2_nrho - 1);
+    k.inv_drho = (drho > 0.0) ? 1.0 / drho : 0.0;
+    k.inv_dzg = (p.s1r2_nz > 0 && p.L > 0) ? 1.0 / (p.L / (double)p.s1r2_nz) : 0.0;
+    k.Lf = (float)p.L;
+    k.halfLf = (float)(0.5 * p.L);
+    k.rho_maxf = (float)k.rho_max;
+    k.inv_drhof = (float)k.inv_drho;
+    k.inv_dzgf = (float)k.inv_dzg;
+    return k;
+}
+
+PairArgs make_args(const EvalPlan &p, const DeviceParticles &dp, const ScratchView &v) {
+    PairArgs a{};
+    const bool mixed = is_mixed(p);
+    a.irec = dp.jrec;
+    a.isoft = dp.soft;
+    a.jrec = mixed ? v.jrec_s : dp.jrec;
+    a.soft = mixed ? v.soft_s : dp.soft;
+    a.jrel = v.jrel;
+    a.tiles = v.tiles;
+    a.table = (p.variant == PV_T3_EWALD) ? p.t3 : (is_s1r2_lookup(p.variant) ? p.s1r2 : nullptr);
+    a.tab32 = v.tab32;
+    a.part = v.part;
+    a.k = make_scalars(p);
+    return a;
+}
+
+unsigned row_blocks(const EvalPlan &p) {
+    const int64_t rpb = (int64_t)kBlock * rows_per_lane(p.variant, p.precision);
+    return (unsigned)((p.nrows + rpb - 1) / rpb);
+}
+
+bool has_pairs(const EvalPlan &p) { return p.nrows > 0 && p.N > 0; }
 
 }  // namespace
 
@@ -875,18 +1039,10 @@ size_t partial_bytes(const EvalPlan &p) {
     return (size_t)p.splits * 3u * (size_t)std::max<int64_t>(p.nrows, 1) * sizeof(double);
 }
 
-static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
-
 // Scratch after the partial sums (same buffer, 256-byte aligned pieces):
-// tiles [ceil(N/64)] double4, then for the mixed path jrel [N] float4 and,
-// for the S1R2 table variants, the FP32 table [nrho*nz] float2.
-static size_t s1r2_f32_entries(const EvalPlan &p) {
-    return (p.precision == STEPS_PREC_MIXED && is_s1r2_lookup(p.variant)) ? (size_t)p.s1r2_nrho * p.s1r2_nz
-                                                                          : 0;
-}
-
-// The mixed path also holds the spatially ordered source copies (jrec_s
-// double4 [N], soft_s double [N]) and the ordering's own scratch.
+// tiles [ceil(N/64)] double4; for the mixed path also jrel [N] float4, the
+// FP32 S1R2 table [nrho*nz] float2, the spatially ordered source copies
+// (jrec_s double4 [N], soft_s double [N]) and the ordering's own scratch.
 size_t tile_extra_bytes(const EvalPlan &p) {
     const int64_t nt = (p.N + kTile - 1) / kTile;
     const size_t n = (size_t)std::max<int64_t>(p.N, 1);
@@ -900,105 +1056,77 @@ size_t tile_extra_bytes(const EvalPlan &p) {
     return b + 256;
 }
 
-hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *partials,
-                         double *F, hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_pair_end,
-                         hipEvent_t ev_end) {
+hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, double *partials,
+                                 hipStream_t st) {
+    if (!has_pairs(p)) return hipSuccess;
+    const ScratchView v = scratch_view(p, partials);
+    const unsigned nt = (unsigned)((p.N + kTile - 1) / kTile);
     hipError_t e;
-    if (!(p.nrows > 0 && p.N > 0) && ev_begin && (e = hipEventRecord(ev_begin, st)) != hipSuccess) return e;
-    if (p.nrows > 0 && p.N > 0) {
-        KScalars k;
-        k.N = p.N;
-        k.id_min = p.id_min;
-        k.nrows = p.nrows;
-        k.jchunk = p.jchunk;
-        k.L = p.L;
-        k.halfL = 0.5 * p.L;
-        k.ng = p.t3_ng;
-        k.nrho = p.s1r2_nrho;
-        k.nz = p.s1r2_nz;
-        k.rho_max = kRadialExtent * p.Rsim;
-        k.images = p.images;
-        k.image_cut = (((double)p.images) - 0.4) * p.L;
-        k.c15 = 1.5;
-        k.c1875 = 1.875;
-        k.inv_gs = (p.t3_ng > 0 && p.L > 0) ? 1.0 / (p.L / (double)p.t3_ng) : 0.0;
-        const double drho = k.rho_max / (double)std::max(1, p.s1r2_nrho - 1);
-        k.inv_drho = (drho > 0.0) ? 1.0 / drho : 0.0;
-        k.inv_dzg = (p.s1r2_nz > 0 && p.L > 0) ? 1.0 / (p.L / (double)p.s1r2_nz) : 0.0;
-        k.Lf = (float)p.L;
-        k.halfLf = (float)(0.5 * p.L);
-        k.rho_maxf = (float)k.rho_max;
-        k.inv_drhof = (float)k.inv_drho;
-        k.inv_dzgf = (float)k.inv_dzg;
-        const int64_t nt = (p.N + kTile - 1) / kTile;
-        const bool mixed = (p.precision == STEPS_PREC_MIXED && mixed_supported(p.variant));
-        double4 *tiles = (double4 *)((char *)partials + align256(partial_bytes(p)));
-        float4 *jrel = (float4 *)((char *)tiles + align256((size_t)nt * sizeof(double4)));
-        float2 *tab32 = (float2 *)((char *)jrel + align256((size_t)p.N * sizeof(float4)));
-        const int64_t n32 = (int64_t)s1r2_f32_entries(p);
-        if (n32 > 0) {
-            hipLaunchKernelGGL(table_f32_kernel, dim3((unsigned)((n32 + 255) / 256)), dim3(256), 0, st, p.s1r2,
-                               tab32, n32);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
-        }
-        double4 *jrec_s = (double4 *)((char *)tab32 + align256(s1r2_f32_entries(p) * sizeof(float2)));
-        double *soft_s = (double *)((char *)jrec_s + align256((size_t)p.N * sizeof(double4)));
-        void *order_scratch = (char *)soft_s + align256((size_t)p.N * sizeof(double));
-        if (mixed) {
-            if ((e = launch_spatial_order(dp.jrec, dp.soft, p.N, order_scratch, jrec_s, soft_s, st)) != hipSuccess)
-                return e;
-            hipLaunchKernelGGL((tiles_kernel<true>), dim3((unsigned)nt), dim3(kTile), 0, st, jrec_s, soft_s,
-                               tiles, jrel, p.N);
-        } else
-            hipLaunchKernelGGL((tiles_kernel<false>), dim3((unsigned)nt), dim3(kTile), 0, st, dp.jrec, dp.soft,
-                               tiles, jrel, p.N);
+    const int64_t n32 = (int64_t)s1r2_f32_entries(p);
+    if (n32 > 0) {
+        hipLaunchKernelGGL(table_f32_kernel, dim3((unsigned)((n32 + 255) / 256)), dim3(256), 0, st, p.s1r2,
+                           v.tab32, n32);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if (ev_begin && (e = hipEventRecord(ev_begin, st)) != hipSuccess) return e;
-        if (mixed && p.variant != PV_R3) {
-            switch (p.variant) {
-            case PV_S1R2_NEAREST:
-                e = launch_mixed_z<PV_S1R2_NEAREST>(p, dp, jrec_s, soft_s, partials, tiles, jrel, tab32, st, k);
-                break;
-            case PV_S1R2_CIC:
-                e = launch_mixed_z<PV_S1R2_CIC>(p, dp, jrec_s, soft_s, partials, tiles, jrel, tab32, st, k);
-                break;
-            case PV_S1R2_TSC:
-                e = launch_mixed_z<PV_S1R2_TSC>(p, dp, jrec_s, soft_s, partials, tiles, jrel, tab32, st, k);
-                break;
-            default: return hipErrorInvalidValue;
-            }
-        } else if (mixed) {
-            constexpr int NI = kRowsPerLaneMixed;
-            const int64_t rows_per_block = (int64_t)kBlock * NI;
-            dim3 grid((unsigned)((p.nrows + rows_per_block - 1) / rows_per_block), (unsigned)p.splits);
-            hipLaunchKernelGGL((pair_kernel_mixed_r3<NI, 8>), grid, dim3(kBlock), 0, st, dp.jrec, dp.soft, jrec_s,
-                               soft_s, jrel, tiles, partials, k);
-            e = hipGetLastError();
-        } else {
-            switch (p.variant) {
-            case PV_R3: e = launch_variant<PV_R3>(p, dp, partials, tiles, st, k, nullptr); break;
-            case PV_T3_NEAREST: e = launch_variant<PV_T3_NEAREST>(p, dp, partials, tiles, st, k, nullptr); break;
-            case PV_T3_EWALD: e = launch_variant<PV_T3_EWALD>(p, dp, partials, tiles, st, k, p.t3); break;
-            case PV_S1R2_NEAREST: e = launch_variant<PV_S1R2_NEAREST>(p, dp, partials, tiles, st, k, nullptr); break;
-            case PV_S1R2_NGP: e = launch_variant<PV_S1R2_NGP>(p, dp, partials, tiles, st, k, p.s1r2); break;
-            case PV_S1R2_CIC: e = launch_variant<PV_S1R2_CIC>(p, dp, partials, tiles, st, k, p.s1r2); break;
-            case PV_S1R2_TSC: e = launch_variant<PV_S1R2_TSC>(p, dp, partials, tiles, st, k, p.s1r2); break;
-            case PV_S1R2_IMAGES: e = launch_variant<PV_S1R2_IMAGES>(p, dp, partials, tiles, st, k, nullptr); break;
-            default: return hipErrorInvalidValue;
-            }
-        }
-        if (e != hipSuccess) return e;
     }
+    if (is_mixed(p)) {
+        if ((e = launch_spatial_order(dp.jrec, dp.soft, p.N, v.order, v.jrec_s, v.soft_s, st)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL((tiles_kernel<true>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles, v.jrel,
+                           p.N);
+    } else {
+        hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles, v.jrel,
+                           p.N);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_forces_pairs(const EvalPlan &p, const DeviceParticles &dp, double *partials, hipStream_t st) {
+    if (!has_pairs(p)) return hipSuccess;
+    const PairArgs a = make_args(p, dp, scratch_view(p, partials));
+    return launch_pair(p.variant, is_mixed(p) ? STEPS_PREC_MIXED : STEPS_PREC_FP64, &a, nullptr, 0, row_blocks(p),
+                       (unsigned)p.splits, st);
+}
+
+hipError_t launch_forces_finish(const EvalPlan &p, const DeviceParticles &dp, double *partials, double *F,
+                                hipStream_t st) {
+    if (p.nrows <= 0) return hipSuccess;
+    const int splits = (p.N > 0) ? p.splits : 0;
+    hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)((p.nrows + 255) / 256)), dim3(256), 0, st, partials,
+                       dp.jrec, p.radial, F, p.nrows, p.id_min, splits, p.ext_mode, p.ext_coeff, p.Rsim,
+                       p.radial_size);
+    return hipGetLastError();
+}
+
+hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *partials, double *F, hipStream_t st,
+                         hipEvent_t ev_begin, hipEvent_t ev_pair_end, hipEvent_t ev_end) {
+    hipError_t e;
+    if ((e = launch_forces_prepare(p, dp, partials, st)) != hipSuccess) return e;
+    if (ev_begin && (e = hipEventRecord(ev_begin, st)) != hipSuccess) return e;
+    if ((e = launch_forces_pairs(p, dp, partials, st)) != hipSuccess) return e;
     if (ev_pair_end && (e = hipEventRecord(ev_pair_end, st)) != hipSuccess) return e;
-    if (p.nrows > 0) {
-        const int splits = (p.N > 0) ? p.splits : 0;
-        hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)((p.nrows + 255) / 256)), dim3(256), 0, st,
-                           partials, dp.jrec, p.radial, F, p.nrows, p.id_min, splits, p.ext_mode,
-                           p.ext_coeff, p.Rsim, p.radial_size);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
+    if ((e = launch_forces_finish(p, dp, partials, F, st)) != hipSuccess) return e;
     if (ev_end && (e = hipEventRecord(ev_end, st)) != hipSuccess) return e;
     return hipSuccess;
+}
+
+size_t pack_desc_bytes() { return sizeof(PackSeg); }
+
+int64_t pack_describe(const EvalPlan &p, const DeviceParticles &dp, double *partials, void *host_descs, int idx,
+                      int64_t blk_begin) {
+    PackSeg g;
+    g.a = make_args(p, dp, scratch_view(p, partials));
+    g.nbx = std::max(1u, row_blocks(p));
+    g.blk_begin = (int)blk_begin;
+    ((PackSeg *)host_descs)[idx] = g;
+    return has_pairs(p) ? (int64_t)g.nbx * p.splits : 0;  // an empty segment owns no workgroup
+}
+
+hipError_t launch_pair_packed(int variant, int precision, const void *dev_descs, int nseg, int64_t total_blocks,
+                              hipStream_t st) {
+    if (nseg <= 0 || total_blocks <= 0) return hipSuccess;
+    if (nseg > kMaxPack || total_blocks > INT32_MAX) return hipErrorInvalidValue;
+    return launch_pair(variant, precision, nullptr, (const PackSeg *)dev_descs, nseg, (unsigned)total_blocks, 1,
+                       st);
 }
 
 hipError_t launch_pack(const double *x, const double *M, const double *soft, int64_t n,

@@ -14,7 +14,11 @@
 //     slots and memory free up, so their kernels overlap on the device;
 //   * each simulation's force and step sequences are captured once per
 //     precision in a hipGraph and replayed (one launch per request); the
-//     per-step scalars travel through a pinned block the graph copies in.
+//     per-step scalars travel through a pinned block the graph copies in;
+//   * ready requests of small simulations (N <= pack_max_n) with the same
+//     pair variant and precision are packed: their pair kernels run as one
+//     launch (a flat grid over all their row blocks and splits), the cheap
+//     per-simulation kernels around it are launched directly.
 // A request's result does not depend on the mix: its kernels, plan and
 // reduction order are those of the simulation alone.
 #include "sim_internal.h"
@@ -46,6 +50,7 @@ struct Request {
 struct SimEntry {
     steps_force_desc desc{};
     int64_t N = 0;
+    int variant = 0;                        // pair-kernel variant (make_plan)
     std::vector<double> x, v, M, soft, F;   // host copy, authoritative while not resident
     std::vector<double> t3, s1r2, radial;   // host tables, needed to (re)admit
     steps_sim *res = nullptr;
@@ -58,7 +63,10 @@ struct SimEntry {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    int64_t ticket = -1;
+    int64_t ticket = -1;          // the request on the slot, or the first of a packed batch
+    std::vector<int64_t> batch;   // all requests of a packed launch (empty otherwise)
+    DevBuf dseg;                  // packed-launch descriptors: device copy ...
+    void *hseg = nullptr;         // ... and pinned host staging
 };
 
 // Device bytes a resident simulation takes (what sim_create allocates).
@@ -193,6 +201,94 @@ struct steps_server {
         return STEPS_OK;
     }
 
+    // Ready requests that can share one packed pair launch with the head of
+    // ready[qi] (resident, small, same variant and precision, valid plan); the
+    // head itself first.  Members are kept from eviction while the group forms.
+    std::vector<int64_t> form_pack(const std::vector<int64_t> &ready, size_t qi, const std::vector<char> &taken) {
+        const int64_t k = ready[qi];
+        std::vector<int64_t> group{k};
+        SimEntry &e = sims[k];
+        if (cfg.pack_max_n <= 0 || e.N > cfg.pack_max_n) return group;
+        const int prec = reqs[e.pending.front()].precision;
+        EvalPlan p;
+        if (sim_plan(e.res, 0, e.N - 1, prec, p) != STEPS_OK) return group;
+        e.busy = true;
+        for (size_t qj = qi + 1; qj < ready.size() && (int)group.size() < kMaxPack; ++qj) {
+            const int64_t k2 = ready[qj];
+            SimEntry &e2 = sims[k2];
+            if (taken[k2] || e2.removed || e2.busy || e2.N > cfg.pack_max_n || e2.variant != e.variant) continue;
+            if (reqs[e2.pending.front()].precision != prec) continue;
+            if (ensure_resident(e2, k2) != 0) continue;
+            EvalPlan p2;
+            if (sim_plan(e2.res, 0, e2.N - 1, prec, p2) != STEPS_OK) continue;
+            e2.busy = true;
+            group.push_back(k2);
+        }
+        for (const int64_t g : group) sims[g].busy = false;
+        return group;
+    }
+
+    // One packed launch for the head requests of `group`: per simulation the
+    // step's first half and the source preparation, one pair kernel over all,
+    // per simulation the reduction and the step's second half.  No graphs
+    // (the composition changes from launch to launch).
+    void issue_batch(const std::vector<int64_t> &group, Slot &sl) {
+        const size_t n = group.size();
+        std::vector<EvalPlan> plans(n);
+        std::vector<int64_t> tickets(n);
+        const double t0 = now();
+        for (size_t i = 0; i < n; ++i) {
+            SimEntry &e = sims[group[i]];
+            tickets[i] = e.pending.front();
+            e.pending.pop_front();
+            reqs[tickets[i]].t_start = t0;
+        }
+        auto fail_all = [&](int code) {
+            for (const int64_t t : tickets) finish(reqs[t], code);
+        };
+        for (size_t i = 0; i < n; ++i) {
+            SimEntry &e = sims[group[i]];
+            const Request &r = reqs[tickets[i]];
+            int rc = sim_plan(e.res, 0, e.N - 1, r.precision, plans[i]);
+            if (!rc && r.kind == STEPS_REQ_STEP) rc = sim_fill_params(e.res, r.sp);
+            if (rc) return fail_all(rc);
+        }
+        const hipStream_t qs = sl.stream;
+        hipError_t he = hipSuccess;
+        for (size_t i = 0; i < n && he == hipSuccess; ++i)
+            if (reqs[tickets[i]].kind == STEPS_REQ_STEP) he = sim_enqueue_step_before(sims[group[i]].res, qs);
+        for (size_t i = 0; i < n && he == hipSuccess; ++i) {
+            steps_sim *s = sims[group[i]].res;
+            he = launch_forces_prepare(plans[i], s->dp, s->part.as<double>(), qs);
+        }
+        int64_t blocks = 0;
+        for (size_t i = 0; i < n && he == hipSuccess; ++i) {
+            steps_sim *s = sims[group[i]].res;
+            blocks += pack_describe(plans[i], s->dp, s->part.as<double>(), sl.hseg, (int)i, blocks);
+        }
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(sl.dseg.ptr, sl.hseg, n * pack_desc_bytes(), hipMemcpyHostToDevice, qs);
+        if (he == hipSuccess) he = launch_pair_packed(plans[0].variant, plans[0].precision, sl.dseg.ptr, (int)n, blocks, qs);
+        for (size_t i = 0; i < n && he == hipSuccess; ++i) {
+            steps_sim *s = sims[group[i]].res;
+            he = launch_forces_finish(plans[i], s->dp, s->part.as<double>(), s->F.as<double>(), qs);
+        }
+        for (size_t i = 0; i < n && he == hipSuccess; ++i)
+            if (reqs[tickets[i]].kind == STEPS_REQ_STEP) he = sim_enqueue_step_after(sims[group[i]].res, qs);
+        if (he == hipSuccess) he = hipEventRecord(sl.done, qs);
+        if (he != hipSuccess) {
+            // work already queued on this stream must drain before the
+            // simulations can be used again
+            (void)hipStreamSynchronize(qs);
+            return fail_all(hip_fail(he, "packed launch"));
+        }
+        for (const int64_t g : group) sims[g].busy = true;
+        sl.batch = tickets;
+        sl.ticket = tickets[0];
+        st.packed_launches++;
+        st.packed_requests += (int64_t)n;
+    }
+
     void loop() {
         (void)hipSetDevice(cfg.device);
         std::unique_lock<std::mutex> lk(mu);
@@ -207,24 +303,28 @@ struct steps_server {
                     inflight++;
                     continue;
                 }
-                Request &r = reqs[sl.ticket];
-                SimEntry &e = sims[r.sim];
-                if (q == hipSuccess) {
-                    if (r.kind == STEPS_REQ_STEP) r.errmax = *e.res->herrmax;
-                    e.has_F = true;
-                    finish(r, STEPS_OK);
-                } else {
-                    finish(r, hip_fail(q, "request"));
+                if (sl.batch.empty()) sl.batch.push_back(sl.ticket);
+                for (const int64_t t : sl.batch) {
+                    Request &r = reqs[t];
+                    SimEntry &e = sims[r.sim];
+                    if (q == hipSuccess) {
+                        if (r.kind == STEPS_REQ_STEP) r.errmax = *e.res->herrmax;
+                        e.has_F = true;
+                        finish(r, STEPS_OK);
+                    } else {
+                        finish(r, hip_fail(q, "request"));
+                    }
+                    e.busy = false;
+                    e.last_used = now();
+                    if (e.removed && e.pending.empty() && e.res) {
+                        sim_destroy(e.res);
+                        e.res = nullptr;
+                        used -= e.bytes;
+                    }
                 }
-                e.busy = false;
-                e.last_used = now();
+                sl.batch.clear();
                 sl.ticket = -1;
                 progressed = true;
-                if (e.removed && e.pending.empty() && e.res) {
-                    sim_destroy(e.res);
-                    e.res = nullptr;
-                    used -= e.bytes;
-                }
             }
             // issue: oldest head request first, while slots are free
             std::vector<int64_t> ready;
@@ -232,7 +332,10 @@ struct steps_server {
                 if (!sims[k].busy && !sims[k].pending.empty()) ready.push_back((int64_t)k);
             std::sort(ready.begin(), ready.end(),
                       [&](int64_t a, int64_t b) { return sims[a].pending.front() < sims[b].pending.front(); });
-            for (int64_t k : ready) {
+            std::vector<char> taken(sims.size(), 0);
+            for (size_t qi = 0; qi < ready.size(); ++qi) {
+                const int64_t k = ready[qi];
+                if (taken[k]) continue;
                 Slot *free_slot = nullptr;
                 for (Slot &sl : slots)
                     if (sl.ticket < 0) { free_slot = &sl; break; }
@@ -248,12 +351,21 @@ struct steps_server {
                 }
                 const int adm = ensure_resident(e, k);
                 if (adm == 1) continue;  // no room until something finishes
-                e.pending.pop_front();
-                progressed = true;
                 if (adm < 0) {
+                    e.pending.pop_front();
+                    progressed = true;
                     finish(r, adm);
                     continue;
                 }
+                std::vector<int64_t> group = form_pack(ready, qi, taken);
+                for (const int64_t g : group) taken[g] = 1;
+                progressed = true;
+                if (group.size() >= 2) {
+                    issue_batch(group, *free_slot);
+                    if (free_slot->ticket >= 0) inflight++;
+                    continue;
+                }
+                e.pending.pop_front();
                 r.t_start = now();
                 const int rc = issue(e, r, *free_slot);
                 if (rc != STEPS_OK) {
@@ -286,6 +398,7 @@ void steps_server_config_init(steps_server_config *c) {
     c->streams = 4;
     c->use_graphs = 1;
     c->hbm_budget_bytes = (int64_t)200 << 30;
+    c->pack_max_n = 65536;
 }
 
 int steps_server_create(const steps_server_config *cfg, steps_server **out) {
@@ -293,6 +406,7 @@ int steps_server_create(const steps_server_config *cfg, steps_server **out) {
     *out = nullptr;
     if (cfg->streams < 1 || cfg->streams > 16) return fail(STEPS_ERR_ARG, "streams must be in [1, 16]");
     if (cfg->hbm_budget_bytes <= 0) return fail(STEPS_ERR_ARG, "budget must be positive");
+    if (cfg->pack_max_n < 0) return fail(STEPS_ERR_ARG, "pack_max_n must be >= 0");
     int rc = set_device(cfg->device);
     if (rc) return rc;
     steps_server *S = new (std::nothrow) steps_server();
@@ -302,8 +416,13 @@ int steps_server_create(const steps_server_config *cfg, steps_server **out) {
     for (Slot &sl : S->slots) {
         hipError_t e;
         if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess ||
-            (e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess) {
+            (e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess ||
+            (e = hipHostMalloc(&sl.hseg, kMaxPack * pack_desc_bytes(), hipHostMallocDefault)) != hipSuccess) {
             rc = hip_fail(e, "server streams");
+            steps_server_destroy(S);
+            return rc;
+        }
+        if ((rc = sl.dseg.reserve(kMaxPack * pack_desc_bytes(), "pack descriptors"))) {
             steps_server_destroy(S);
             return rc;
         }
@@ -334,6 +453,8 @@ int steps_server_destroy(steps_server *S) {
     for (Slot &sl : S->slots) {
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.hseg) (void)hipHostFree(sl.hseg);
+        sl.dseg.release();
     }
     delete S;
     return STEPS_OK;
@@ -360,6 +481,7 @@ int steps_server_add_sim(steps_server *S, const steps_force_desc *d, int64_t N, 
         e.s1r2.assign(d->s1r2_table, d->s1r2_table + (size_t)d->s1r2_nrho * d->s1r2_nz * 2);
     if (p.ext_mode == 3) e.radial.assign(d->radial_table, d->radial_table + d->radial_table_size);
     e.desc.t3_table = e.desc.s1r2_table = e.desc.radial_table = nullptr;
+    e.variant = p.variant;
     e.bytes = footprint(*d, N);
     if ((int64_t)e.bytes > S->cfg.hbm_budget_bytes)
         return fail(STEPS_ERR_ADMISSION, "simulation larger than the whole memory budget");

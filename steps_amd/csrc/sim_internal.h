@@ -81,6 +81,10 @@ int sim_plan(steps_sim *s, int64_t id_min, int64_t id_max, int precision, EvalPl
 // scalars the caller has written to s->hparams.  Both are graph-capturable.
 hipError_t sim_enqueue_forces(steps_sim *s, const EvalPlan &p, hipStream_t st, bool events);
 hipError_t sim_enqueue_step(steps_sim *s, const EvalPlan &p, hipStream_t st);
+// The step around its force evaluation: parameters in, first kick and drift
+// (before); second kick, errmax and its copy to the pinned host block (after).
+hipError_t sim_enqueue_step_before(steps_sim *s, hipStream_t st);
+hipError_t sim_enqueue_step_after(steps_sim *s, hipStream_t st);
 // Fill the pinned parameter block from the public step parameters.
 int sim_fill_params(steps_sim *s, const steps_step_params &sp);
 // Host <-> device state (synchronous on the sim's stream).

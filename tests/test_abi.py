@@ -93,7 +93,11 @@ def test_server_without_device_or_with_bad_config():
     cfg = _abi.ServerConfig()
     lib.steps_server_config_init(C.byref(cfg))
     assert cfg.streams == 4 and cfg.use_graphs == 1 and cfg.hbm_budget_bytes == 200 << 30
+    assert cfg.pack_max_n == 65536
     h = C.c_void_p()
+    cfg.pack_max_n = -1
+    assert lib.steps_server_create(C.byref(cfg), C.byref(h)) == _abi.STEPS_ERR_ARG
+    cfg.pack_max_n = 65536
     cfg.streams = 0
     assert lib.steps_server_create(C.byref(cfg), C.byref(h)) == _abi.STEPS_ERR_ARG
     cfg.streams = 4

@@ -85,11 +85,12 @@ def run_mix(sims, server, precisions=None):
     return ids, results, states
 
 
-@pytest.mark.parametrize("use_graphs", [True, False])
-def test_mix_equals_alone(ensemble, use_graphs):
-    print(f"case server mix use_graphs={use_graphs}", flush=True)
+@pytest.mark.parametrize("use_graphs,pack_max_n,streams", [(True, 0, 4), (False, 0, 4), (True, 65536, 4),
+                                                            (True, 65536, 1)])
+def test_mix_equals_alone(ensemble, use_graphs, pack_max_n, streams):
+    print(f"case server mix use_graphs={use_graphs} pack_max_n={pack_max_n} streams={streams}", flush=True)
     sims, ref = ensemble
-    srv = steps_amd.Server(streams=4, use_graphs=use_graphs)
+    srv = steps_amd.Server(streams=streams, use_graphs=use_graphs, pack_max_n=pack_max_n)
     ids, results, states = run_mix(sims, srv)
     for i, (F0, errs, x, v, F) in enumerate(ref):
         assert all(r.status == 0 for r in results[i]), [r.status for r in results[i]]
@@ -100,9 +101,16 @@ def test_mix_equals_alone(ensemble, use_graphs):
         for a, b in zip(results[i], results[i][1:]):
             assert b.t_start >= a.t_end
     st = srv.stats()
+    print(f"[server mix] {st}", flush=True)
     assert st["completed"] == len(sims) * (NSTEPS + 1) and st["failed"] == 0
-    if use_graphs:
+    if use_graphs and pack_max_n == 0:
         assert st["graph_captures"] == 2 * len(sims)   # one force and one step graph per simulation
+    if pack_max_n == 0:
+        assert st["packed_launches"] == 0
+    if streams == 1 and pack_max_n > 0:
+        # one slot: while it is busy the other simulations' requests queue up,
+        # so the next issue finds several small ones ready together
+        assert st["packed_launches"] >= 1 and st["packed_requests"] >= 2
     srv.close()
 
 
