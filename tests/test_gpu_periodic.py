@@ -65,6 +65,29 @@ def test_t3_forces(steps, oracle, t3_tab, is_periodic):
     assert np.percentile(e, 99) <= 1e-9
 
 
+def test_t3_row_subset_and_shuffled_input(steps, oracle, t3_tab):
+    """A row subset agrees with the oracle and with the all-rows evaluation,
+    and the result does not depend on the order of the input particles beyond
+    summation order."""
+    print("case t3 row subset / shuffled input", flush=True)
+    w = wl.t3_lattice(n_side=16, seed=1001)
+    d, keep = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=w.L, t3_table=t3_tab)
+    F = steps.force_eval(d, w.x, w.M, w.soft)
+    Fs = steps.force_eval(d, w.x, w.M, w.soft, id_min=100, id_max=1699)
+    p = oracle.params(1, is_periodic=2, L=w.L, t3_table=t3_tab, interp_order=4)
+    Fo = oracle.forces(p, w.x, w.M, w.soft)
+    report("T3 row subset vs oracle", rel_err(Fs, Fo[100:1700]))
+    assert np.median(rel_err(Fs, Fo[100:1700])) <= 1e-12
+    e = rel_err(Fs, F[100:1700])
+    report("T3 row subset vs all rows", e)
+    assert np.median(e) <= 1e-12 and np.percentile(e, 99) <= 1e-10
+    perm = np.random.default_rng(7).permutation(w.N)
+    Fp = steps.force_eval(d, w.x[perm], w.M[perm], w.soft[perm])
+    e = rel_err(Fp, F[perm])
+    report("T3 shuffled input vs original order", e)
+    assert np.median(e) <= 1e-12 and np.percentile(e, 99) <= 1e-10
+
+
 def test_t3_full_size_sampled_rows(steps, oracle, t3_tab):
     """The 64^3 T3 configuration at full size, checked on 96 sampled rows."""
     print("case t3 forces n_side=64 (N=262144) sampled rows", flush=True)
