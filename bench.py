@@ -227,6 +227,8 @@ def main() -> int:
     ap.add_argument("--no-variants", action="store_true", help="skip the T3 / S1xR2 variant leg")
     ap.add_argument("--serving-requests", type=int, default=160,
                     help="length of the seeded Poisson request stream of the serving leg")
+    ap.add_argument("--only", choices=["variants", "serving"],
+                    help="run only that leg and print its JSON (profiling runs); skips the headline line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -236,6 +238,13 @@ def main() -> int:
         print("bench.py measures one GPU: this build runs the full N x N interaction on a single "
               "MI355X; multi-GPU sharding is design only (see DESIGN.md)", file=sys.stderr)
         return 2
+
+    if args.only == "variants":
+        print(json.dumps({"variants": variants_leg(max(1, min(args.steps, 3)))}), flush=True)
+        return 0
+    if args.only == "serving":
+        print(json.dumps({"serving": serving_leg(args.serving_requests)}), flush=True)
+        return 0
 
     gen = WORKLOADS[args.workload]
     w = wl.r3_compactified(**gen)
