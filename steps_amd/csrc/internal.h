@@ -14,7 +14,7 @@ namespace steps {
 constexpr double kPi = 3.14159265358979323846264338327950288419716939937510;  // global_variables.h:16
 constexpr double kRadialExtent = 2.25;                                          // global_variables.h:39
 constexpr int kBlock = 256;      // threads per workgroup of the pair kernels (4 wave64)
-constexpr int kRowsPerLane = 2;  // i-particles held in registers per lane (FP64 path)
+constexpr int kRowsPerLane = 1;  // i-particles held in registers per lane (FP64 path)
 constexpr int kRowsPerLaneMixed = 2;
 
 // Pair-term variants: one kernel instantiation each.
