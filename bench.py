@@ -72,10 +72,15 @@ def cpu_baseline(w, target_s: float = 12.0) -> dict:
         oracle_lib.forces_rows(p, w.x, w.M, w.soft, rows, kind=kind)
         return time.perf_counter() - t0
 
-    calib = max(16, threads * 2)
-    t = run(calib, "faithful")
-    nrows = int(min(N, max(calib, calib * target_s / max(t, 1e-3))))
+    # grow the row sample until it takes about target_s (thread start-up
+    # makes small samples look slower per row than they are)
+    nrows = max(16, threads * 2)
     t = run(nrows, "faithful")
+    for _ in range(4):
+        if t >= 0.6 * target_s or nrows >= N:
+            break
+        nrows = int(min(N, max(nrows + 1, nrows * target_s / max(t, 1e-3))))
+        t = run(nrows, "faithful")
     rate = nrows * N / t
     tf = run(nrows, "fast")
     cpu_model = ""
