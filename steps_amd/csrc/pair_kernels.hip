@@ -60,6 +60,7 @@ struct PairArgs {
     const double *soft;
     const float4 *jrel;
     const double4 *tiles;
+    const double4 *tile_ext;  // half extents of the tiles' bounding boxes (sorted sources)
     const double *table;
     const float2 *tab32;
     double *part;
@@ -365,11 +366,35 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
 // B = (h_i + h_max,tile)^2 once, then UNR sources at a time come in through
 // scalar loads and are applied to the NI rows of the lane.
 // ---------------------------------------------------------------------------
+// R3 pair of a tile that no row of the wave can reach within its softening
+// bound: the plain r^-3 law, no per-pair test.
+__device__ __forceinline__ void pair_r3_far(const KScalars &k, const double4 pj, const double xi, const double yi,
+                                            const double zi, double &fx, double &fy, double &fz) {
+    const double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double s = pj.w * inv_r3(r2, k.c15);
+    fx = fma(s, dx, fx);
+    fy = fma(s, dy, fy);
+    fz = fma(s, dz, fz);
+}
+
+// Squared distance from a point to an axis-aligned box (centre c, half
+// extents e); every source of the tile is at least this far from the point.
+__device__ __forceinline__ double box_dist2(double x, double y, double z, const double4 &c, const double4 &e) {
+    const double gx = fmax(fabs(x - c.x) - e.x, 0.0);
+    const double gy = fmax(fabs(y - c.y) - e.y, 0.0);
+    const double gz = fmax(fabs(z - c.z) - e.z, 0.0);
+    return fma(gx, gx, fma(gy, gy, gz * gz));
+}
+
 template <int V, int NI, int UNR>
 __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const int split) {
+    const double4 *__restrict__ irec = a.irec;
+    const double *__restrict__ isoft = a.isoft;
     const double4 *__restrict__ jrec = a.jrec;
     const double *__restrict__ soft = a.soft;
     const double4 *__restrict__ tiles = a.tiles;
+    const double4 *__restrict__ ext = a.tile_ext;
     const double *__restrict__ table = a.table;
     double *__restrict__ part = a.part;
     KScalars k = a.k;
@@ -383,16 +408,17 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
         const int64_t r = min(row_base + q * kBlock + tid, k.nrows - 1);
-        const double4 p = jrec[k.id_min + r];
+        const double4 p = irec[k.id_min + r];
         xi[q] = p.x;
         yi[q] = p.y;
         zi[q] = p.z;
-        hi[q] = soft[k.id_min + r];
+        hi[q] = isoft[k.id_min + r];
         fx[q] = fy[q] = fz[q] = 0.0;
     }
 
     for (int64_t t0 = j0; t0 < j1; t0 += kTile) {
-        const double hmax = tiles[t0 / kTile].w;
+        const double4 tc = tiles[t0 / kTile];
+        const double hmax = tc.w;
         double B[NI];
 #pragma unroll
         for (int q = 0; q < NI; ++q) {
@@ -401,6 +427,32 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
         }
         const int64_t tend = min(j1, t0 + kTile);
         int64_t j = t0;
+        if (V == PV_R3) {
+            // R3 sources are spatially ordered: when the tile's bounding box
+            // lies outside every lane's softening bound (B widened by 1e-9
+            // for rounding), its pairs skip the per-pair test
+            const double4 te = ext[t0 / kTile];
+            bool far = true;
+#pragma unroll
+            for (int q = 0; q < NI; ++q) far = far && box_dist2(xi[q], yi[q], zi[q], tc, te) > B[q] * (1.0 + 1e-9);
+            if (__all(far)) {
+                for (; j + UNR <= tend; j += UNR) {
+                    double4 pj[UNR];
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) pj[u] = jrec[j + u];
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                        for (int q = 0; q < NI; ++q) pair_r3_far(k, pj[u], xi[q], yi[q], zi[q], fx[q], fy[q], fz[q]);
+                }
+                for (; j < tend; ++j) {
+                    const double4 pj = jrec[j];
+#pragma unroll
+                    for (int q = 0; q < NI; ++q) pair_r3_far(k, pj, xi[q], yi[q], zi[q], fx[q], fy[q], fz[q]);
+                }
+                continue;
+            }
+        }
         for (; j + UNR <= tend; j += UNR) {
             double4 pj[UNR];
 #pragma unroll
@@ -881,7 +933,8 @@ template <bool MIXED>
 __global__ __launch_bounds__(kTile) void tiles_kernel(const double4 *__restrict__ jrec,
                                                      const double *__restrict__ soft,
                                                      double4 *__restrict__ tiles,
-                                                     float4 *__restrict__ jrel, int64_t n) {
+                                                     float4 *__restrict__ jrel, double4 *__restrict__ ext,
+                                                     int64_t n) {
     const int64_t t = blockIdx.x;
     const int lane = threadIdx.x;
     const int64_t i = t * kTile + lane;
@@ -901,6 +954,10 @@ __global__ __launch_bounds__(kTile) void tiles_kernel(const double4 *__restrict_
     }
     const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
     if (lane == 0) tiles[t] = make_double4(cx, cy, cz, hm);
+    // half extents, rounded up so that centre +- extent contains every source
+    if (ext && lane == 0)
+        ext[t] = make_double4((hi[0] - lo[0]) * (0.5 + 1e-15), (hi[1] - lo[1]) * (0.5 + 1e-15),
+                              (hi[2] - lo[2]) * (0.5 + 1e-15), 0.0);
     if (MIXED && ok)
         jrel[i] = make_float4((float)(p.x - cx), (float)(p.y - cy), (float)(p.z - cz), (float)p.w);
 }
@@ -956,6 +1013,9 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 bool is_mixed(const EvalPlan &p) { return p.precision == STEPS_PREC_MIXED && mixed_supported(p.variant); }
 
+// The mixed kernels and the R3 FP64 kernel read spatially ordered sources.
+bool sorted_sources(const EvalPlan &p) { return is_mixed(p) || p.variant == PV_R3; }
+
 size_t s1r2_f32_entries(const EvalPlan &p) {
     return (is_mixed(p) && is_s1r2_lookup(p.variant)) ? (size_t)p.s1r2_nrho * p.s1r2_nz : 0;
 }
@@ -965,6 +1025,7 @@ size_t s1r2_f32_entries(const EvalPlan &p) {
 struct ScratchView {
     double *part;
     double4 *tiles;
+    double4 *tile_ext;
     float4 *jrel;
     float2 *tab32;
     double4 *jrec_s;
@@ -976,9 +1037,11 @@ ScratchView scratch_view(const EvalPlan &p, double *partials) {
     ScratchView v;
     const int64_t nt = (p.N + kTile - 1) / kTile;
     const size_t n = (size_t)std::max<int64_t>(p.N, 1);
+    const size_t tb = align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4));
     v.part = partials;
     v.tiles = (double4 *)((char *)partials + align256(partial_bytes(p)));
-    v.jrel = (float4 *)((char *)v.tiles + align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4)));
+    v.tile_ext = (double4 *)((char *)v.tiles + tb);
+    v.jrel = (float4 *)((char *)v.tile_ext + tb);
     v.tab32 = (float2 *)((char *)v.jrel + align256(n * sizeof(float4)));
     v.jrec_s = (double4 *)((char *)v.tab32 + align256(s1r2_f32_entries(p) * sizeof(float2)));
     v.soft_s = (double *)((char *)v.jrec_s + align256(n * sizeof(double4)));
@@ -1015,13 +1078,14 @@ KScalars make_scalars(const EvalPlan &p) {
 
 PairArgs make_args(const EvalPlan &p, const DeviceParticles &dp, const ScratchView &v) {
     PairArgs a{};
-    const bool mixed = is_mixed(p);
+    const bool sorted = sorted_sources(p);
     a.irec = dp.jrec;
     a.isoft = dp.soft;
-    a.jrec = mixed ? v.jrec_s : dp.jrec;
-    a.soft = mixed ? v.soft_s : dp.soft;
+    a.jrec = sorted ? v.jrec_s : dp.jrec;
+    a.soft = sorted ? v.soft_s : dp.soft;
     a.jrel = v.jrel;
     a.tiles = v.tiles;
+    a.tile_ext = v.tile_ext;
     a.table = (p.variant == PV_T3_EWALD) ? p.t3 : (is_s1r2_lookup(p.variant) ? p.s1r2 : nullptr);
     a.tab32 = v.tab32;
     a.part = v.part;
@@ -1042,15 +1106,16 @@ size_t partial_bytes(const EvalPlan &p) {
     return (size_t)p.splits * 3u * (size_t)std::max<int64_t>(p.nrows, 1) * sizeof(double);
 }
 
-// Scratch after the partial sums (same buffer, 256-byte aligned pieces):
-// tiles [ceil(N/64)] double4; for the mixed path also jrel [N] float4, the
-// FP32 S1R2 table [nrho*nz] float2, the spatially ordered source copies
-// (jrec_s double4 [N], soft_s double [N]) and the ordering's own scratch.
+// Scratch after the partial sums (same buffer, 256-byte aligned pieces, in
+// ScratchView order): tiles and their half extents [ceil(N/64)] double4; for
+// spatially ordered sources (mixed path, R3) also jrel [N] float4, the FP32
+// S1R2 table [nrho*nz] float2, the ordered copies (jrec_s double4 [N],
+// soft_s double [N]) and the ordering's own scratch.
 size_t tile_extra_bytes(const EvalPlan &p) {
     const int64_t nt = (p.N + kTile - 1) / kTile;
     const size_t n = (size_t)std::max<int64_t>(p.N, 1);
-    size_t b = align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4));
-    if (p.precision == STEPS_PREC_MIXED) {
+    size_t b = 2 * align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4));
+    if (p.precision == STEPS_PREC_MIXED || p.variant == PV_R3) {
         b += align256(n * sizeof(float4));
         b += align256(s1r2_f32_entries(p) * sizeof(float2));
         b += align256(n * sizeof(double4)) + align256(n * sizeof(double));
@@ -1071,14 +1136,18 @@ hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, d
                            v.tab32, n32);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if (is_mixed(p)) {
+    if (sorted_sources(p)) {
         if ((e = launch_spatial_order(dp.jrec, dp.soft, p.N, v.order, v.jrec_s, v.soft_s, st)) != hipSuccess)
             return e;
-        hipLaunchKernelGGL((tiles_kernel<true>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles, v.jrel,
-                           p.N);
+        if (is_mixed(p))
+            hipLaunchKernelGGL((tiles_kernel<true>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles,
+                               v.jrel, v.tile_ext, p.N);
+        else
+            hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles,
+                               v.jrel, v.tile_ext, p.N);
     } else {
         hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles, v.jrel,
-                           p.N);
+                           (double4 *)nullptr, p.N);
     }
     return hipGetLastError();
 }
