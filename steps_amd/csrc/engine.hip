@@ -139,8 +139,8 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
         if (d.is_periodic == 1) {
             p.variant = PV_S1R2_NEAREST;
         } else if (d.s1r2_lookup) {
-            if (!d.s1r2_table || d.s1r2_nrho < 1 || d.s1r2_nz < 1)
-                return fail(STEPS_ERR_TABLE, "S1R2 Ewald table missing");
+            if (!d.s1r2_table || d.s1r2_nrho < 2 || d.s1r2_nz < 1)
+                return fail(STEPS_ERR_TABLE, "S1R2 Ewald table missing (or fewer than 2 rho nodes)");
             p.variant = d.interp_order == 0 ? PV_S1R2_NGP
                         : d.interp_order == 2 ? PV_S1R2_CIC
                                               : PV_S1R2_TSC;  // forces_cuda.cu:449-451 fallback

@@ -46,6 +46,7 @@ struct KScalars {
     double c15;
     // reciprocals of the table grid spacings: T3 Ng/L, S1R2 1/drho and Nz/Lz
     double inv_gs, inv_drho, inv_dzg;
+    double last_r;  // S1R2: index of the last rho node, Nrho - 1 (Nrho >= 2)
     // FP32 copies for the mixed-precision S1R2 pair
     float Lf, halfLf, rho_maxf, inv_drhof, inv_dzgf;
 };
@@ -138,12 +139,13 @@ __device__ __forceinline__ int wrap_once(int i, int n) {
     return (i < 0) ? i + n : ((i >= n) ? i - n : i);
 }
 
-// x^(-1/2) for x > 0: v_rsq_f64 seed and a second-order Newton correction,
-// y * (1 - e)^(-1/2) ~= y * (1 + e/2 + 3e^2/8) with e = 1 - x*y^2.
+// x^(-1/2) for x > 0: v_rsq_f64 seed and a first-order Newton correction,
+// y * (1 - e)^(-1/2) ~= y * (1 + e/2) with e = 1 - x*y^2; the dropped
+// 3e^2/8 bounds the relative error by 4e-15.
 __device__ __forceinline__ double rsqrt_f64(double x) {
     const double y = __builtin_amdgcn_rsq(x);
     const double e = fma(-x, y * y, 1.0);
-    return fma(y * e, fma(e, 0.375, 0.5), y);
+    return fma(y * e, 0.5, y);
 }
 
 // forces_cuda.cu:93-167: tricubic Catmull-Rom interpolation of the T3 table
@@ -227,10 +229,13 @@ __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, const 
         Dr = t.x;
         Dzv = t.y;
     } else if (ORDER == 2) {
-        const double ur = (rho > rho_max) ? (double)(Nrho - 1) : rho * k.inv_drho;
-        int ir0 = (int)floor(ur);
-        double fr = ur - (double)ir0;
-        if (ir0 > Nrho - 2) { ir0 = max(0, Nrho - 2); fr = 1.0; }
+        // rho beyond rho_max clamps to the last node (forces_cuda.cu:225-239);
+        // min() and the clamped floor give the reference's (ir0, fr) up to the
+        // rounding of rho * (1/drho), where the interpolant is continuous
+        const double ur = fmin(rho * k.inv_drho, k.last_r);
+        const double urf = fmin(floor(ur), k.last_r - 1.0);
+        const double fr = ur - urf;
+        const int ir0 = (int)urf;
         const int ir1 = ir0 + 1;
         const double uzf = floor(uz);
         const double fz = uz - uzf;
@@ -238,12 +243,12 @@ __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, const 
         const int iz1 = wrap_once(iz0 + 1, Nz);
         const double2 t00 = s1r2_node(T, Nz, ir0, iz0), t10 = s1r2_node(T, Nz, ir1, iz0);
         const double2 t01 = s1r2_node(T, Nz, ir0, iz1), t11 = s1r2_node(T, Nz, ir1, iz1);
-        double w00 = 1.0 - fr; w00 *= (1.0 - fz);
-        double w10 = fr; w10 *= (1.0 - fz);
-        double w01 = 1.0 - fr; w01 *= fz;
-        const double w11 = fr * fz;
-        Dr = w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x;
-        Dzv = w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y;
+        // bilinear as two lerps in rho and one in z (same interpolant as the
+        // reference's four weights, fewer operations)
+        const double a0 = fma(fr, t10.x - t00.x, t00.x), b0 = fma(fr, t11.x - t01.x, t01.x);
+        const double a1 = fma(fr, t10.y - t00.y, t00.y), b1 = fma(fr, t11.y - t01.y, t01.y);
+        Dr = fma(fz, b0 - a0, a0);
+        Dzv = fma(fz, b1 - a1, a1);
     } else {
         const double ur = (rho > rho_max) ? (double)(Nrho - 1) : rho * k.inv_drho;
         const double urf = floor(ur + 0.5), uzf = floor(uz + 0.5);
@@ -637,20 +642,21 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
     float Dr, Dzv;
     static_assert(ORDER == 2 || ORDER == 4, "FP32 interpolation is CIC or TSC");
     if (ORDER == 2) {
-        const float ur = (rho > k.rho_maxf) ? (float)(Nrho - 1) : rho * k.inv_drhof;
-        int ir0 = (int)floorf(ur);
-        float fr = ur - (float)ir0;
-        if (ir0 > Nrho - 2) { ir0 = max(0, Nrho - 2); fr = 1.0f; }
+        const float lastr = (float)k.last_r;
+        const float ur = fminf(rho * k.inv_drhof, lastr);
+        const float urf = fminf(floorf(ur), lastr - 1.0f);
+        const float fr = ur - urf;
+        const int ir0 = (int)urf;
         const float uzf = floorf(uz);
         const float fz = uz - uzf;
         const int iz0 = wrap_index((int)uzf, Nz);
         const int iz1 = wrap_once(iz0 + 1, Nz);
         const float2 t00 = node32(T, Nz, ir0, iz0), t10 = node32(T, Nz, ir0 + 1, iz0);
         const float2 t01 = node32(T, Nz, ir0, iz1), t11 = node32(T, Nz, ir0 + 1, iz1);
-        const float w00 = (1.0f - fr) * (1.0f - fz), w10 = fr * (1.0f - fz);
-        const float w01 = (1.0f - fr) * fz, w11 = fr * fz;
-        Dr = fmaf(w00, t00.x, fmaf(w10, t10.x, fmaf(w01, t01.x, w11 * t11.x)));
-        Dzv = fmaf(w00, t00.y, fmaf(w10, t10.y, fmaf(w01, t01.y, w11 * t11.y)));
+        const float a0 = fmaf(fr, t10.x - t00.x, t00.x), b0 = fmaf(fr, t11.x - t01.x, t01.x);
+        const float a1 = fmaf(fr, t10.y - t00.y, t00.y), b1 = fmaf(fr, t11.y - t01.y, t01.y);
+        Dr = fmaf(fz, b0 - a0, a0);
+        Dzv = fmaf(fz, b1 - a1, a1);
     } else {
         const float ur = (rho > k.rho_maxf) ? (float)(Nrho - 1) : rho * k.inv_drhof;
         const float urf = floorf(ur + 0.5f), uzf = floorf(uz + 0.5f);
@@ -1068,6 +1074,7 @@ KScalars make_scalars(const EvalPlan &p) {
     const double drho = k.rho_max / (double)std::max(1, p.s1r2_nrho - 1);
     k.inv_drho = (drho > 0.0) ? 1.0 / drho : 0.0;
     k.inv_dzg = (p.s1r2_nz > 0 && p.L > 0) ? 1.0 / (p.L / (double)p.s1r2_nz) : 0.0;
+    k.last_r = (double)std::max(1, p.s1r2_nrho - 1);
     k.Lf = (float)p.L;
     k.halfLf = (float)(0.5 * p.L);
     k.rho_maxf = (float)k.rho_max;
