@@ -57,3 +57,43 @@ def test_kdk_steps_match_oracle(steps, oracle, topology):
         rel = np.linalg.norm(F - Fo.reshape(-1, 3), axis=1) / np.linalg.norm(Fo.reshape(-1, 3), axis=1)
         assert np.median(rel) <= 1e-12
     sim.close()
+
+
+def test_c0_force_and_ten_steps(steps, oracle):
+    """The CPU-runnable configuration: compactified R3, N = 16384 (seed 1000),
+    one force evaluation then ten leapfrog steps, FP64, against the oracle
+    advanced the same way (the host supplies a and H(a) as StePS does)."""
+    print("case c0 R3 N=16384 force + 10 KDK steps", flush=True)
+    w = wl.r3_compactified(N=16384, seed=1000)
+    kw = dict(cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere)
+    desc, keep = steps.make_desc(steps.TOPO_R3, **kw)
+    p = oracle.params(0, **kw)
+    cosmo = oracle.OracleCosmo(wl.H0, wl.OMEGA_M, wl.OMEGA_L, 0.0, 1.0 - wl.OMEGA_M - wl.OMEGA_L)
+    lib = oracle.lib()
+    a = w.a_start
+    st = oracle.OracleStepState(a, lib.oracle_hubble(C.byref(cosmo), a), 1e-4, w.rho_part, 1.0)
+    xo = w.x.reshape(-1).copy()
+    vo = w.v.reshape(-1).copy()
+    Fo = oracle.forces(p, w.x, w.M, w.soft).reshape(-1).copy()
+    sim = steps.Sim(desc, w.N)
+    sim.set_state(w.x, w.M, w.soft, w.v)
+    sim.forces()
+    F = sim.get_forces()
+    rel = np.linalg.norm(F - Fo.reshape(-1, 3), axis=1) / np.linalg.norm(Fo.reshape(-1, 3), axis=1)
+    assert np.median(rel) <= 1e-12
+    h = st.h
+    for k in range(10):
+        a1, h1 = st.a, st.hubble
+        eo = lib.oracle_kdk_step(C.byref(p), C.byref(cosmo), C.byref(st), w.N, oracle.dp(xo), oracle.dp(vo),
+                                 oracle.dp(Fo), oracle.dp(w.M), oracle.dp(w.soft))
+        eg = sim.step(h, a1, h1, st.a, st.hubble, w.rho_part)
+        assert eg == pytest.approx(eo, rel=1e-9), k
+    x, v = sim.get_state()
+    assert np.allclose(x.reshape(-1), xo, rtol=0, atol=1e-11 * np.abs(xo).max())
+    assert np.allclose(v.reshape(-1), vo, rtol=1e-8, atol=1e-11 * np.abs(vo).max())
+    sim.forces()
+    F = sim.get_forces()
+    rel = np.linalg.norm(F - Fo.reshape(-1, 3), axis=1) / np.linalg.norm(Fo.reshape(-1, 3), axis=1)
+    print(f"[c0 after 10 steps] median {np.median(rel):.3e} p99 {np.percentile(rel, 99):.3e}", flush=True)
+    assert np.median(rel) <= 1e-12
+    sim.close()
