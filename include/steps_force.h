@@ -143,7 +143,8 @@ int steps_sim_errmax(steps_sim *s, double a, double hubble, double G, double rho
 
 /* ---- tables (built on the device, returned in host memory) ----------------
  * T3 Ewald correction, ewald_space.cc:288-383 with main.cc:422-448 params;
- * call with table==NULL to get *ngrid, then with a [ngrid^3*3] buffer. */
+ * call with table==NULL to get *ngrid, then with a [ngrid^3*3] buffer.
+ * The three builders cache their results by parameters (up to 2 GiB). */
 int steps_t3_table(int32_t is_periodic, double L, double *table, int32_t *ngrid);
 /* S1R2 Tornberg 1P correction, ewald_space.cc:754-798 with main.cc:569-602;
  * table==NULL queries (*nrho, *nz). */
@@ -198,7 +199,12 @@ int steps_server_create(const steps_server_config *c, steps_server **out);
 int steps_server_destroy(steps_server *s);
 /* Registers a simulation (host arrays are copied; v may be NULL).  It becomes
  * resident when it first has work.  Fails with STEPS_ERR_ADMISSION if it
- * could never fit the budget. */
+ * could never fit the budget.  Table pointers of *d may be NULL: the server
+ * then builds the tables the reference would for that descriptor (T3 and
+ * S1R2 Ewald tables, main.cc:422-448 and :569-602; the radial table with
+ * RADIAL_FORCE_TABLE_SIZE = RADIAL_FORCE_ACCURACY = 1000 unless
+ * radial_table_size is set, main.cc:1266-1302); built tables are cached by
+ * their parameters, so an ensemble sharing a box builds each table once. */
 int steps_server_add_sim(steps_server *s, const steps_force_desc *d, int64_t N, const double *x,
                          const double *v, const double *M, const double *soft, int64_t *sim_id);
 int steps_server_remove_sim(steps_server *s, int64_t sim_id);

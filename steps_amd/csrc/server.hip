@@ -89,6 +89,45 @@ size_t footprint(const steps_force_desc &d, int64_t N) {
     return b;
 }
 
+// T3 Ewald table (main.cc:422-448), S1R2 Ewald table (main.cc:569-602) and
+// the cylindrical radial table (main.cc:1266-1302: R = Rsim, Lz = L/2 with
+// IS_PERIODIC == 1, L * (IS_PERIODIC + 1 - 0.4) for the image sum;
+// RADIAL_FORCE_TABLE_SIZE and RADIAL_FORCE_ACCURACY 1000, read_paramfile.cc:
+// 128-129) for every table the descriptor needs and does not carry.
+int build_missing_tables(steps_force_desc &d, std::vector<double> &t3, std::vector<double> &s1r2,
+                         std::vector<double> &radial) {
+    int rc = STEPS_OK;
+    if (d.topology == STEPS_TOPO_T3 && d.is_periodic >= 2 && !d.t3_table) {
+        int ng = 0;
+        if ((rc = build_t3_table(d.is_periodic, d.L, nullptr, &ng))) return rc;
+        t3.resize((size_t)ng * ng * ng * 3);
+        if ((rc = build_t3_table(d.is_periodic, d.L, t3.data(), &ng))) return rc;
+        d.t3_table = t3.data();
+        d.t3_ngrid = ng;
+    }
+    if (d.topology == STEPS_TOPO_S1R2) {
+        if (d.is_periodic >= 2 && d.s1r2_lookup && !d.s1r2_table) {
+            int a = 0, b = 0;
+            if ((rc = build_s1r2_table(d.is_periodic, d.L, d.Rsim, nullptr, &a, &b))) return rc;
+            s1r2.resize((size_t)std::max(a, 0) * b * 2);
+            if ((rc = build_s1r2_table(d.is_periodic, d.L, d.Rsim, s1r2.data(), &a, &b))) return rc;
+            d.s1r2_table = s1r2.data();
+            d.s1r2_nrho = a;
+            d.s1r2_nz = b;
+        }
+        const bool radial_used = d.cosmology == 1 && d.comoving == 1 && (d.is_periodic == 1 || !d.s1r2_lookup);
+        if (radial_used && !d.radial_table && d.is_periodic >= 1) {
+            const int size = d.radial_table_size > 0 ? d.radial_table_size : 1000;
+            const double Lz = (d.is_periodic == 1) ? 0.5 * d.L : d.L * ((double)(d.is_periodic + 1) - 0.4);
+            radial.resize((size_t)size);
+            if ((rc = build_radial_table(d.Rsim, Lz, size, 1000, radial.data()))) return rc;
+            d.radial_table = radial.data();
+            d.radial_table_size = size;
+        }
+    }
+    return rc;
+}
+
 }  // namespace
 }  // namespace steps
 
@@ -464,8 +503,15 @@ int steps_server_add_sim(steps_server *S, const steps_force_desc *d, int64_t N, 
                          const double *M, const double *soft, int64_t *sim_id) {
     if (!S || !d || !x || !M || !soft || !sim_id || N <= 0) return fail(STEPS_ERR_ARG, "bad add_sim arguments");
     if (d->device != S->cfg.device) return fail(STEPS_ERR_ARG, "simulation device differs from the server's");
+    // Tables the descriptor leaves out are built here with the reference's
+    // parameters (build_* keeps them cached by parameters across simulations).
+    steps_force_desc dd = *d;
+    std::vector<double> t3, s1r2, radial;
+    int rc = build_missing_tables(dd, t3, s1r2, radial);
+    if (rc) return rc;
+    d = &dd;
     EvalPlan p;
-    int rc = make_plan(*d, N, 0, N - 1, p);  // validates the descriptor and its tables
+    rc = make_plan(*d, N, 0, N - 1, p);  // validates the descriptor and its tables
     if (rc) return rc;
     if ((rc = check_finite(x, 3 * N))) return rc;
     SimEntry e;

@@ -7,6 +7,10 @@
 // descriptor takes host tables (the reference's globals are host arrays).
 #include "internal.h"
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -305,7 +309,7 @@ struct DevMem {
 
 }  // namespace
 
-int build_t3_table(int is_periodic, double L, double *table, int *ngrid) {
+int build_t3_table_uncached(int is_periodic, double L, double *table, int *ngrid) {
     // main.cc:422-448
     if (is_periodic < 2) return fail(STEPS_ERR_ARG, "T3 Ewald table needs IS_PERIODIC >= 2");
     if (!(L > 0)) return fail(STEPS_ERR_ARG, "T3 table needs L > 0");
@@ -346,7 +350,7 @@ int build_t3_table(int is_periodic, double L, double *table, int *ngrid) {
     return STEPS_OK;
 }
 
-int build_s1r2_table(int is_periodic, double L, double Rsim, double *table, int *nrho, int *nz) {
+int build_s1r2_table_uncached(int is_periodic, double L, double Rsim, double *table, int *nrho, int *nz) {
     // main.cc:569-602 (Tornberg 1P parameters)
     if (is_periodic < 2) return fail(STEPS_ERR_ARG, "S1R2 Ewald table needs IS_PERIODIC >= 2");
     if (!(L > 0) || !(Rsim > 0)) return fail(STEPS_ERR_ARG, "S1R2 table needs L > 0 and Rsim > 0");
@@ -373,7 +377,7 @@ int build_s1r2_table(int is_periodic, double L, double Rsim, double *table, int 
     return STEPS_OK;
 }
 
-int build_radial_table(double R, double Lz, int size, int accuracy, double *table) {
+int build_radial_table_uncached(double R, double Lz, int size, int accuracy, double *table) {
     if (!table || size < 1 || accuracy < 1 || !(R > 0) || !(Lz > 0))
         return fail(STEPS_ERR_ARG, "bad radial table arguments");
     DevMem dT;
@@ -394,6 +398,114 @@ int build_radial_table(double R, double Lz, int size, int accuracy, double *tabl
         const double B = Y1 - A * X1;
         table[0] = A * 0.0 + B;
     }
+    return STEPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Built tables are kept, keyed by their kind and parameters, so that a second
+// request with the same parameters (another simulation of an ensemble with
+// the same box, the size query and the fill of one caller) is a host copy.
+// The cache holds at most kTableCacheBytes; beyond that it starts over.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TableKey {
+    int kind, is_periodic, size, accuracy;
+    uint64_t a, b;  // bit patterns of the two length parameters
+    bool operator<(const TableKey &o) const {
+        return std::tie(kind, is_periodic, size, accuracy, a, b) <
+               std::tie(o.kind, o.is_periodic, o.size, o.accuracy, o.a, o.b);
+    }
+};
+
+struct CachedTable {
+    std::vector<double> data;
+    int d0 = 0, d1 = 0;
+};
+
+constexpr size_t kTableCacheBytes = (size_t)2 << 30;
+std::mutex g_table_mu;
+std::map<TableKey, CachedTable> g_tables;
+size_t g_table_bytes = 0;
+
+uint64_t bits(double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, sizeof u);
+    return u;
+}
+
+const CachedTable *cache_find(const TableKey &k) {
+    auto it = g_tables.find(k);
+    return it == g_tables.end() ? nullptr : &it->second;
+}
+
+void cache_put(const TableKey &k, CachedTable t) {
+    const size_t b = t.data.size() * sizeof(double);
+    if (b > kTableCacheBytes) return;
+    if (g_table_bytes + b > kTableCacheBytes) {
+        g_tables.clear();
+        g_table_bytes = 0;
+    }
+    g_table_bytes += b;
+    g_tables[k] = std::move(t);
+}
+
+}  // namespace
+
+int build_t3_table(int is_periodic, double L, double *table, int *ngrid) {
+    int ng = 0;
+    int rc = build_t3_table_uncached(is_periodic, L, nullptr, &ng);  // validates, sizes
+    if (ngrid) *ngrid = ng;
+    if (rc || !table) return rc;
+    const TableKey key{1, is_periodic, ng, 0, bits(L), 0};
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    if (const CachedTable *c = cache_find(key)) {
+        std::memcpy(table, c->data.data(), c->data.size() * sizeof(double));
+        return STEPS_OK;
+    }
+    CachedTable t;
+    t.data.resize((size_t)ng * ng * ng * 3);
+    if ((rc = build_t3_table_uncached(is_periodic, L, t.data.data(), &ng))) return rc;
+    std::memcpy(table, t.data.data(), t.data.size() * sizeof(double));
+    cache_put(key, std::move(t));
+    return STEPS_OK;
+}
+
+int build_s1r2_table(int is_periodic, double L, double Rsim, double *table, int *nrho, int *nz) {
+    int a = 0, b = 0;
+    int rc = build_s1r2_table_uncached(is_periodic, L, Rsim, nullptr, &a, &b);
+    if (nrho) *nrho = a;
+    if (nz) *nz = b;
+    if (rc || !table) return rc;
+    const TableKey key{2, is_periodic, a, b, bits(L), bits(Rsim)};
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    if (const CachedTable *c = cache_find(key)) {
+        std::memcpy(table, c->data.data(), c->data.size() * sizeof(double));
+        return STEPS_OK;
+    }
+    CachedTable t;
+    t.data.resize((size_t)std::max(a, 0) * b * 2);
+    if ((rc = build_s1r2_table_uncached(is_periodic, L, Rsim, t.data.data(), &a, &b))) return rc;
+    std::memcpy(table, t.data.data(), t.data.size() * sizeof(double));
+    cache_put(key, std::move(t));
+    return STEPS_OK;
+}
+
+int build_radial_table(double R, double Lz, int size, int accuracy, double *table) {
+    if (!table || size < 1 || accuracy < 1 || !(R > 0) || !(Lz > 0))
+        return fail(STEPS_ERR_ARG, "bad radial table arguments");
+    const TableKey key{3, 0, size, accuracy, bits(R), bits(Lz)};
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    if (const CachedTable *c = cache_find(key)) {
+        std::memcpy(table, c->data.data(), c->data.size() * sizeof(double));
+        return STEPS_OK;
+    }
+    CachedTable t;
+    t.data.resize((size_t)size);
+    int rc = build_radial_table_uncached(R, Lz, size, accuracy, t.data.data());
+    if (rc) return rc;
+    std::memcpy(table, t.data.data(), t.data.size() * sizeof(double));
+    cache_put(key, std::move(t));
     return STEPS_OK;
 }
 

@@ -165,3 +165,35 @@ def test_failing_request_is_isolated(ensemble):
     with pytest.raises(steps_amd.StepsError):
         srv.submit(good, steps_amd.REQ_STEP)       # a step needs parameters
     srv.close()
+
+
+def test_server_builds_missing_tables():
+    """Descriptors without tables: the server builds the reference's tables
+    (and caches them by parameters); results equal those with the same tables
+    passed in explicitly."""
+    print("case server builds missing tables", flush=True)
+    c = wl.s1r2_cylinder(n_omega=16, n_phi=8, n_z=8, seed=6)
+    t = wl.t3_lattice(n_side=10, seed=5)
+    T, shape = steps_amd.s1r2_table(2, c.L, c.Rsim)
+    radial = steps_amd.radial_table(c.Rsim, 0.5 * c.L, 1000, 1000)
+    T3 = steps_amd.t3_table(2, t.L)
+    base = dict(L=c.L, Rsim=c.Rsim, cosmology=1, comoving=1, mass_in_unit_sphere=c.mass_in_unit_sphere)
+    cases = [
+        (c, dict(is_periodic=2, s1r2_lookup=1, interp_order=2, **base), dict(s1r2_table=T, s1r2_shape=shape),
+         steps_amd.TOPO_S1R2),
+        (c, dict(is_periodic=1, **base), dict(radial_table=radial), steps_amd.TOPO_S1R2),
+        (t, dict(is_periodic=2, L=t.L), dict(t3_table=T3), steps_amd.TOPO_T3),
+    ]
+    srv = steps_amd.Server(streams=2)
+    for w, kw, tables, topo in cases:
+        d0, k0 = steps_amd.make_desc(topo, **kw)
+        d1, k1 = steps_amd.make_desc(topo, **kw, **tables)
+        a = srv.add_sim(d0, w.x, w.M, w.soft)
+        b = srv.add_sim(d1, w.x, w.M, w.soft)
+        ta = srv.submit(a, steps_amd.REQ_FORCE)
+        tb = srv.submit(b, steps_amd.REQ_FORCE)
+        assert srv.wait(ta).status == 0 and srv.wait(tb).status == 0
+        Fa, Fb = srv.get_state(a)[2], srv.get_state(b)[2]
+        assert np.array_equal(Fa, Fb), kw
+        assert np.all(np.isfinite(Fa)) and np.abs(Fa).max() > 0
+    srv.close()
