@@ -190,6 +190,7 @@ typedef struct steps_server_stats {
     int64_t admissions, evictions, graph_captures;
     int64_t resident_sims, bytes_resident, max_backlog;
     int64_t packed_launches, packed_requests;  /* launches holding >= 2 requests, and those requests */
+    int64_t tickets_held;      /* requests whose result has not been collected yet */
 } steps_server_stats;
 
 void steps_server_config_init(steps_server_config *c);
@@ -214,7 +215,11 @@ int64_t steps_server_sim_bytes(steps_server *s, int64_t sim_id);
  * per request (enum steps_precision). */
 int steps_server_submit(steps_server *s, int64_t sim_id, int32_t kind, int32_t precision,
                         const steps_step_params *p, int64_t *ticket);
+/* Blocks until the request is done and returns its result once: the ticket
+ * is then collected, and the server forgets collected tickets in order, so a
+ * long-running loop keeps only the window of tickets not yet collected. */
 int steps_server_wait(steps_server *s, int64_t ticket, steps_request_result *out);
+/* Blocks until every submitted request is done (results stay collectable). */
 int steps_server_drain(steps_server *s);
 /* After the simulation's requests finished: x, v, F (AoS [3N], any may be NULL). */
 int steps_server_get_state(steps_server *s, int64_t sim_id, double *x, double *v, double *F);

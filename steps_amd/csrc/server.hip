@@ -45,6 +45,7 @@ struct Request {
     int status = STEPS_OK;
     double errmax = 0;
     bool done = false;
+    bool collected = false;  // its result was returned by steps_server_wait
 };
 
 struct SimEntry {
@@ -140,7 +141,19 @@ struct steps_server {
     std::thread thr;
     bool stop = false;
     std::vector<SimEntry> sims;
-    std::deque<Request> reqs;  // index == ticket
+    // Requests by ticket: reqs[t - base].  Requests that are done and whose
+    // result was collected leave the front, so a long-running server keeps
+    // only the window of tickets still in use.
+    std::deque<Request> reqs;
+    int64_t base = 0;
+    Request &req(int64_t t) { return reqs[(size_t)(t - base)]; }
+    bool known(int64_t t) const { return t >= base && t < base + (int64_t)reqs.size(); }
+    void prune() {
+        while (!reqs.empty() && reqs.front().done && reqs.front().collected) {
+            reqs.pop_front();
+            ++base;
+        }
+    }
     std::vector<Slot> slots;
     size_t used = 0;
     steps_server_stats st{};
@@ -248,7 +261,7 @@ struct steps_server {
         std::vector<int64_t> group{k};
         SimEntry &e = sims[k];
         if (cfg.pack_max_n <= 0 || e.N > cfg.pack_max_n) return group;
-        const int prec = reqs[e.pending.front()].precision;
+        const int prec = req(e.pending.front()).precision;
         EvalPlan p;
         if (sim_plan(e.res, 0, e.N - 1, prec, p) != STEPS_OK) return group;
         e.busy = true;
@@ -256,7 +269,7 @@ struct steps_server {
             const int64_t k2 = ready[qj];
             SimEntry &e2 = sims[k2];
             if (taken[k2] || e2.removed || e2.busy || e2.N > cfg.pack_max_n || e2.variant != e.variant) continue;
-            if (reqs[e2.pending.front()].precision != prec) continue;
+            if (req(e2.pending.front()).precision != prec) continue;
             if (ensure_resident(e2, k2) != 0) continue;
             EvalPlan p2;
             if (sim_plan(e2.res, 0, e2.N - 1, prec, p2) != STEPS_OK) continue;
@@ -280,14 +293,14 @@ struct steps_server {
             SimEntry &e = sims[group[i]];
             tickets[i] = e.pending.front();
             e.pending.pop_front();
-            reqs[tickets[i]].t_start = t0;
+            req(tickets[i]).t_start = t0;
         }
         auto fail_all = [&](int code) {
-            for (const int64_t t : tickets) finish(reqs[t], code);
+            for (const int64_t t : tickets) finish(req(t), code);
         };
         for (size_t i = 0; i < n; ++i) {
             SimEntry &e = sims[group[i]];
-            const Request &r = reqs[tickets[i]];
+            const Request &r = req(tickets[i]);
             int rc = sim_plan(e.res, 0, e.N - 1, r.precision, plans[i]);
             if (!rc && r.kind == STEPS_REQ_STEP) rc = sim_fill_params(e.res, r.sp);
             if (rc) return fail_all(rc);
@@ -295,7 +308,7 @@ struct steps_server {
         const hipStream_t qs = sl.stream;
         hipError_t he = hipSuccess;
         for (size_t i = 0; i < n && he == hipSuccess; ++i)
-            if (reqs[tickets[i]].kind == STEPS_REQ_STEP) he = sim_enqueue_step_before(sims[group[i]].res, qs);
+            if (req(tickets[i]).kind == STEPS_REQ_STEP) he = sim_enqueue_step_before(sims[group[i]].res, qs);
         for (size_t i = 0; i < n && he == hipSuccess; ++i) {
             steps_sim *s = sims[group[i]].res;
             he = launch_forces_prepare(plans[i], s->dp, s->part.as<double>(), qs);
@@ -313,7 +326,7 @@ struct steps_server {
             he = launch_forces_finish(plans[i], s->dp, s->part.as<double>(), s->F.as<double>(), qs);
         }
         for (size_t i = 0; i < n && he == hipSuccess; ++i)
-            if (reqs[tickets[i]].kind == STEPS_REQ_STEP) he = sim_enqueue_step_after(sims[group[i]].res, qs);
+            if (req(tickets[i]).kind == STEPS_REQ_STEP) he = sim_enqueue_step_after(sims[group[i]].res, qs);
         if (he == hipSuccess) he = hipEventRecord(sl.done, qs);
         if (he != hipSuccess) {
             // work already queued on this stream must drain before the
@@ -344,7 +357,7 @@ struct steps_server {
                 }
                 if (sl.batch.empty()) sl.batch.push_back(sl.ticket);
                 for (const int64_t t : sl.batch) {
-                    Request &r = reqs[t];
+                    Request &r = req(t);
                     SimEntry &e = sims[r.sim];
                     if (q == hipSuccess) {
                         if (r.kind == STEPS_REQ_STEP) r.errmax = *e.res->herrmax;
@@ -381,7 +394,7 @@ struct steps_server {
                 if (!free_slot) break;
                 SimEntry &e = sims[k];
                 const int64_t t = e.pending.front();
-                Request &r = reqs[t];
+                Request &r = req(t);
                 if (e.removed) {
                     e.pending.pop_front();
                     finish(r, fail(STEPS_ERR_STATE, "simulation removed"));
@@ -478,7 +491,7 @@ int steps_server_destroy(steps_server *S) {
         S->stop = true;
         for (auto &e : S->sims) {  // pending work is abandoned
             while (!e.pending.empty()) {
-                Request &r = S->reqs[e.pending.front()];
+                Request &r = S->req(e.pending.front());
                 e.pending.pop_front();
                 S->finish(r, fail(STEPS_ERR_STATE, "server destroyed"));
             }
@@ -570,7 +583,7 @@ int steps_server_submit(steps_server *S, int64_t sim, int32_t kind, int32_t prec
         return fail(STEPS_ERR_ARG, "unknown simulation");
     if (S->stop) return fail(STEPS_ERR_STATE, "server stopping");
     Request r;
-    r.ticket = (int64_t)S->reqs.size();
+    r.ticket = S->base + (int64_t)S->reqs.size();
     r.sim = sim;
     r.kind = kind;
     r.precision = precision;
@@ -588,9 +601,10 @@ int steps_server_submit(steps_server *S, int64_t sim, int32_t kind, int32_t prec
 int steps_server_wait(steps_server *S, int64_t ticket, steps_request_result *out) {
     if (!S) return fail(STEPS_ERR_ARG, "null server");
     std::unique_lock<std::mutex> lk(S->mu);
-    if (ticket < 0 || ticket >= (int64_t)S->reqs.size()) return fail(STEPS_ERR_ARG, "unknown ticket");
-    S->cv_done.wait(lk, [&] { return S->reqs[ticket].done; });
-    const Request &r = S->reqs[ticket];
+    if (!S->known(ticket)) return fail(STEPS_ERR_ARG, "unknown ticket (or its result was already collected)");
+    S->cv_done.wait(lk, [&] { return !S->known(ticket) || S->req(ticket).done; });
+    if (!S->known(ticket)) return fail(STEPS_ERR_ARG, "the result of this ticket was already collected");
+    Request &r = S->req(ticket);
     if (out) {
         out->status = r.status;
         out->kind = r.kind;
@@ -601,6 +615,8 @@ int steps_server_wait(steps_server *S, int64_t ticket, steps_request_result *out
         out->t_start = r.t_start;
         out->t_end = r.t_end;
     }
+    r.collected = true;
+    S->prune();
     return STEPS_OK;
 }
 
@@ -637,6 +653,7 @@ int steps_server_get_stats(steps_server *S, steps_server_stats *out) {
     std::lock_guard<std::mutex> lk(S->mu);
     *out = S->st;
     out->bytes_resident = (int64_t)S->used;
+    out->tickets_held = (int64_t)S->reqs.size();
     int64_t n = 0;
     for (const auto &e : S->sims) n += e.res ? 1 : 0;
     out->resident_sims = n;

@@ -197,3 +197,26 @@ def test_server_builds_missing_tables():
         assert np.array_equal(Fa, Fb), kw
         assert np.all(np.isfinite(Fa)) and np.abs(Fa).max() > 0
     srv.close()
+
+
+def test_collected_tickets_are_forgotten():
+    """A long-running server keeps only the tickets whose results were not
+    collected; a collected ticket cannot be waited for again."""
+    print("case server ticket window", flush=True)
+    w = wl.r3_compactified(N=1000, seed=9)
+    d, keep = steps_amd.make_desc(steps_amd.TOPO_R3)
+    srv = steps_amd.Server(streams=2)
+    sid = srv.add_sim(d, w.x, w.M, w.soft)
+    tickets = [srv.submit(sid, steps_amd.REQ_FORCE) for _ in range(40)]
+    assert srv.stats()["tickets_held"] <= 40
+    for t in tickets[:20]:
+        assert srv.wait(t).status == 0
+    assert srv.stats()["tickets_held"] <= 20
+    for t in tickets[20:]:
+        assert srv.wait(t).status == 0
+    assert srv.stats()["tickets_held"] == 0
+    with pytest.raises(steps_amd.StepsError):
+        srv.wait(tickets[0])
+    t = srv.submit(sid, steps_amd.REQ_FORCE)
+    assert t == tickets[-1] + 1 and srv.wait(t).status == 0
+    srv.close()
