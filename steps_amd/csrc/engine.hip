@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <vector>
 #include <string>
 
 namespace steps {
@@ -94,6 +95,25 @@ int TableCopy::sync(const double *h, size_t count, hipStream_t st, const char *w
     return STEPS_OK;
 }
 
+int TableCopy::sync_t3(const double *h, int ng, hipStream_t st) {
+    const size_t count = (size_t)ng * ng * ng * 3;
+    if (!h || ng < 1) return STEPS_OK;
+    const uint64_t f = fingerprint(h, count);
+    if (h == host && count == n && f == fp && dev.ptr) return STEPS_OK;
+    std::vector<double> layout;
+    const int l = t3_device_layout(h, ng, layout);
+    int rc = dev.reserve(layout.size() * sizeof(double), "T3 table");
+    if (rc) return rc;
+    STEPS_HIP(hipMemcpyAsync(dev.ptr, layout.data(), layout.size() * sizeof(double), hipMemcpyHostToDevice, st),
+              "T3 table");
+    STEPS_HIP(hipStreamSynchronize(st), "T3 table");
+    host = h;
+    n = count;
+    fp = f;
+    lo = l;
+    return STEPS_OK;
+}
+
 int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_max, EvalPlan &p) {
     if (N < 0) return fail(STEPS_ERR_ARG, "N < 0");
     if (N > (int64_t)INT32_MAX * 4) return fail(STEPS_ERR_ARG, "N too large");
@@ -128,6 +148,8 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
             p.variant = PV_T3_NEAREST;
         } else {
             if (!d.t3_table || d.t3_ngrid < 1) return fail(STEPS_ERR_TABLE, "T3 Ewald table missing");
+            if (d.t3_ngrid < 3 || d.t3_ngrid > kT3MaxGrid)
+                return fail(STEPS_ERR_TABLE, "T3 table grid must have 3 to 1023 nodes per axis");
             p.variant = PV_T3_EWALD;
             p.t3_ng = d.t3_ngrid;
         }
@@ -260,8 +282,7 @@ int sim_create(const steps_force_desc &d, int64_t N, bool both_precisions, steps
         (e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess ||
         (e = hipEventCreate(&s->ev2)) != hipSuccess)
         return cleanup(hip_fail(e, "stream/event create"));
-    if (p.variant == PV_T3_EWALD &&
-        (rc = s->t3.sync(d.t3_table, (size_t)d.t3_ngrid * d.t3_ngrid * d.t3_ngrid * 3, s->stream, "T3 table")))
+    if (p.variant == PV_T3_EWALD && (rc = s->t3.sync_t3(d.t3_table, d.t3_ngrid, s->stream)))
         return cleanup(rc);
     if (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC &&
         (rc = s->s1r2.sync(d.s1r2_table, (size_t)d.s1r2_nrho * d.s1r2_nz * 2, s->stream, "S1R2 table")))
@@ -310,6 +331,7 @@ int sim_plan(steps_sim *s, int64_t id_min, int64_t id_max, int precision, EvalPl
     if (rc) return rc;
     if (scratch_bytes(p) > s->part.cap) return fail(STEPS_ERR_STATE, "scratch too small for this precision");
     p.t3 = s->t3.ptr();
+    p.t3_lo = s->t3.lo;
     p.s1r2 = s->s1r2.ptr();
     p.radial = s->radial.ptr();
     return STEPS_OK;
@@ -464,15 +486,14 @@ int steps_force_eval(const steps_force_desc *d, int64_t N, const double *x, cons
         (rc = o.F.reserve(3 * (size_t)p.nrows * sizeof(double), "alloc F")))
         return rc;
     hipStream_t st = o.stream;
-    if (p.variant == PV_T3_EWALD &&
-        (rc = o.t3.sync(d->t3_table, (size_t)d->t3_ngrid * d->t3_ngrid * d->t3_ngrid * 3, st, "T3 table")))
-        return rc;
+    if (p.variant == PV_T3_EWALD && (rc = o.t3.sync_t3(d->t3_table, d->t3_ngrid, st))) return rc;
     if (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC &&
         (rc = o.s1r2.sync(d->s1r2_table, (size_t)d->s1r2_nrho * d->s1r2_nz * 2, st, "S1R2 table")))
         return rc;
     if (p.ext_mode == 3 && (rc = o.radial.sync(d->radial_table, (size_t)d->radial_table_size, st, "radial table")))
         return rc;
     p.t3 = o.t3.ptr();
+    p.t3_lo = o.t3.lo;
     p.s1r2 = o.s1r2.ptr();
     p.radial = o.radial.ptr();
     STEPS_HIP(hipMemcpyAsync(o.x.ptr, x, 3 * nb, hipMemcpyHostToDevice, st), "copy x");

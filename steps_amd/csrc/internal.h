@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "steps_force.h"
 
@@ -70,6 +71,7 @@ struct EvalPlan {
     int ext_mode = 0;      // 0 none, 1 xyz, 2 xy, 3 xy * radial correction
     // tables (device)
     const double *t3 = nullptr; int t3_ng = 0;
+    int t3_lo = 0;         // first x/y node of the T3 device layout (t3_device_layout)
     const double *s1r2 = nullptr; int s1r2_nrho = 0, s1r2_nz = 0;
     const double *radial = nullptr; int radial_size = 0;
     int images = 0;        // S1R2 image-sum: ewald_max
@@ -142,6 +144,19 @@ hipError_t launch_kick_errmax(const DeviceParticles &dp, double *v, const double
                               double *block_max, int nblocks_max, double *errmax_dev,
                               hipStream_t stream, bool kick);
 int kdk_reduce_blocks(int64_t n);
+
+// T3 table in the layout the pair kernel reads.  The reference table is
+// [(ix*Ng + iy)*Ng + iz]*3 + c.  The device layout keeps x and y nodes
+// lo..Ng-1 and every z node plus one periodic ghost below and two above:
+// element ((mx*(Ng-lo) + my)*(Ng+3) + iz+1)*3 + c holds node (lo+mx, lo+my, iz).
+// lo = Ng/2 when the table is exactly antisymmetric in x and in y (the
+// kernel reads a node below lo at its mirror Ng-1-i with that component
+// negated), else 0.  Returns lo.
+int t3_device_layout(const double *T, int ng, std::vector<double> &out);
+inline size_t t3_device_entries(int ng, int lo) {
+    return (size_t)(ng - lo) * (size_t)(ng - lo) * (size_t)(ng + 3) * 3u;
+}
+constexpr int kT3MaxGrid = 1023;  // element offsets of the layout stay below 2^32
 
 // Table builders (table_kernels.hip); all buffers host memory.
 int build_t3_table(int is_periodic, double L, double *table, int *ngrid);

@@ -33,8 +33,8 @@ struct KScalars {
     int64_t nrows;
     int64_t jchunk;
     double L, halfL;
-    // T3 table
-    int ng;
+    // T3 table: grid nodes per axis, first x/y node of the device layout
+    int ng, t3_lo;
     // S1R2 table
     int nrho, nz;
     double rho_max;
@@ -148,48 +148,80 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
     return fma(y * e, 0.5, y);
 }
 
+// Eight-byte aligned pair of doubles: a 16-byte load at any 8-byte boundary.
+typedef double double2u __attribute__((ext_vector_type(2), aligned(8)));
+
+// Catmull-Rom weights and first node of one axis (forces_cuda.cu:93-111):
+// u in grid units from the first cell centre, nodes i0-1 .. i0+2.
+__device__ __forceinline__ int t3_axis(double d, const KScalars &k, double w[4]) {
+    const double u = (d + k.L * 0.5) * k.inv_gs - 0.5;
+    const double uf = floor(u);
+    const double t = u - uf;
+    const double t2 = t * t, t3 = t * t2;
+    w[0] = -0.5 * t3 + t2 - 0.5 * t;
+    w[1] = 1.5 * t3 - 2.5 * t2 + 1.0;
+    w[2] = -1.5 * t3 + 2.0 * t2 + 0.5 * t;
+    w[3] = 0.5 * t3 - 0.5 * t2;
+    return wrap_index((int)uf, k.ng);
+}
+
 // forces_cuda.cu:93-167: tricubic Catmull-Rom interpolation of the T3 table
-// (cell-centred grid, periodic index wrap), table layout [(ix*N+iy)*N+iz]*3+c.
+// (cell-centred grid, periodic index wrap), read from the device layout of
+// t3_device_layout: x and y nodes lo..Ng-1, a node i < lo read at its mirror
+// Ng-1-i with its own component negated (the sign rides on that axis'
+// weight, so x-negated and y-negated values never need a select), and each
+// stencil's four z nodes as twelve contiguous doubles (six 16-byte loads).
+// The contraction runs z, then y, then x: Sum_i wx_i Sum_j wy_j Sum_k wz_k T.
 // The reference divides by the grid spacing; multiplying by its reciprocal
 // moves u by at most an ulp, and the interpolant is continuous in u.
-__device__ __forceinline__ void t3_interp(const double *__restrict__ T, int Ng, double L,
-                                          double inv_gs, double dx, double dy, double dz,
-                                          double &Dx, double &Dy, double &Dz) {
-    int ix[4], iy[4], iz[4];
-    double wx[4], wy[4], wz[4];
-    const double r3v[3] = {dx, dy, dz};
-    int *idx[3] = {ix, iy, iz};
-    double *wt[3] = {wx, wy, wz};
+__device__ __forceinline__ void t3_interp(const double *__restrict__ T, const KScalars &k, double dx,
+                                          double dy, double dz, double &Dx, double &Dy, double &Dz) {
+    const int Ng = k.ng, lo = k.t3_lo;
+    const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3);
+    double wx[4], wy[4], wz[4], wxs[4], wys[4];
+    unsigned mx[4], my[4];
+    const int ix0 = t3_axis(dx, k, wx);
+    const int iy0 = t3_axis(dy, k, wy);
+    const int iz0 = t3_axis(dz, k, wz);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const double u = (r3v[a] + L * 0.5) * inv_gs - 0.5;
-        const double uf = floor(u);
-        const int i0 = wrap_index((int)uf, Ng);
-        const double t = u - uf;
-        const double t2 = t * t, t3 = t * t2;
-        wt[a][0] = -0.5 * t3 + t2 - 0.5 * t;
-        wt[a][1] = 1.5 * t3 - 2.5 * t2 + 1.0;
-        wt[a][2] = -1.5 * t3 + 2.0 * t2 + 0.5 * t;
-        wt[a][3] = 0.5 * t3 - 0.5 * t2;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) idx[a][q] = wrap_once(i0 - 1 + q, Ng);  // Ng >= 3
+    for (int q = 0; q < 4; ++q) {
+        const int jx = wrap_once(ix0 - 1 + q, Ng);  // Ng >= 3
+        const int jy = wrap_once(iy0 - 1 + q, Ng);
+        mx[q] = (unsigned)((jx >= lo) ? jx - lo : Ng - 1 - lo - jx);
+        my[q] = (unsigned)((jy >= lo) ? jy - lo : Ng - 1 - lo - jy);
+        wxs[q] = (jx >= lo) ? wx[q] : -wx[q];
+        wys[q] = (jy >= lo) ? wy[q] : -wy[q];
     }
-    double s0 = 0, s1 = 0, s2 = 0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+        double u0 = 0.0, u1 = 0.0, u2 = 0.0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const double wxy = wx[i] * wy[j];
-            const double *row = T + (size_t)((ix[i] * Ng + iy[j]) * Ng) * 3u;
+            // z node iz0-1+q sits at layout index iz0+q (one ghost below)
+            const unsigned e = ((mx[i] * NY + my[j]) * NZ + (unsigned)iz0) * 3u;
+            const double2u *row = (const double2u *)(T + e);
+            double v[12];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const double w = wxy * wz[k];
-                const double *p = row + (size_t)iz[k] * 3u;
-                s0 += w * p[0];
-                s1 += w * p[1];
-                s2 += w * p[2];
+            for (int h = 0; h < 6; ++h) {
+                const double2u a = row[h];
+                v[2 * h] = a.x;
+                v[2 * h + 1] = a.y;
             }
+            double t0 = wz[0] * v[0], t1 = wz[0] * v[1], t2 = wz[0] * v[2];
+#pragma unroll
+            for (int q = 1; q < 4; ++q) {
+                t0 = fma(wz[q], v[3 * q], t0);
+                t1 = fma(wz[q], v[3 * q + 1], t1);
+                t2 = fma(wz[q], v[3 * q + 2], t2);
+            }
+            u0 = fma(wy[j], t0, u0);
+            u1 = fma(wys[j], t1, u1);
+            u2 = fma(wy[j], t2, u2);
         }
+        s0 = fma(wxs[i], u0, s0);
+        s1 = fma(wx[i], u1, s1);
+        s2 = fma(wx[i], u2, s2);
     }
     Dx = s0;
     Dy = s1;
@@ -319,7 +351,7 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
             fz = fma(s, dz, fz);
         } else {
             double Dx, Dy, Dz;
-            t3_interp(table, k.ng, k.L, k.inv_gs, dx, dy, dz, Dx, Dy, Dz);
+            t3_interp(table, k, dx, dy, dz, Dx, Dy, Dz);
             fx = fma(pj.w, fma(w, dx, -Dx), fx);
             fy = fma(pj.w, fma(w, dy, -Dy), fy);
             fz = fma(pj.w, fma(w, dz, -Dz), fz);
@@ -1067,6 +1099,7 @@ KScalars make_scalars(const EvalPlan &p) {
     k.L = p.L;
     k.halfL = 0.5 * p.L;
     k.ng = p.t3_ng;
+    k.t3_lo = p.t3_lo;
     k.nrho = p.s1r2_nrho;
     k.nz = p.s1r2_nz;
     k.rho_max = kRadialExtent * p.Rsim;
@@ -1111,6 +1144,54 @@ unsigned row_blocks(const EvalPlan &p) {
 bool has_pairs(const EvalPlan &p) { return p.nrows > 0 && p.N > 0; }
 
 }  // namespace
+
+// The reference table is antisymmetric in each axis (ewald_space.cc:288-383
+// fills it from one O_h wedge): the node mirrored in x, Ng-1-ix, holds
+// (-Tx, Ty, Tz), likewise in y.  Components that vanish by symmetry (Tx on
+// the plane x = 0, Ty on y = 0) hold rounding noise instead of zero
+// (|T| ~ 1e-22 against max|T| ~ 4e-4 for the 63^3 table); two such entries
+// count as equal when both are below 2^-52 max|T|.  The plane a mirror maps
+// onto itself (odd Ng) is read directly, never negated.  The fold is used
+// only when every node passes, so a folded read differs from the stored
+// value by at most 2^-51 max|T|, and only in a component that vanishes by
+// symmetry.
+static bool t3_antisymmetric_xy(const double *T, int ng) {
+    const size_t n = (size_t)ng;
+    double tmax = 0.0;
+    for (size_t i = 0; i < n * n * n * 3u; ++i) tmax = std::max(tmax, std::fabs(T[i]));
+    if (!(tmax < INFINITY)) return false;  // NaN or infinite entries: no fold
+    const double eps0 = std::ldexp(tmax, -52);
+    auto same = [eps0](double b, double a) {
+        return b == a || (std::fabs(a) <= eps0 && std::fabs(b) <= eps0);
+    };
+    auto at = [&](size_t ix, size_t iy, size_t iz) { return T + ((ix * n + iy) * n + iz) * 3u; };
+    for (size_t ix = 0; ix < n; ++ix)
+        for (size_t iy = 0; iy < n; ++iy)
+            for (size_t iz = 0; iz < n; ++iz) {
+                const double *a = at(ix, iy, iz), *bx = at(n - 1 - ix, iy, iz), *by = at(ix, n - 1 - iy, iz);
+                if (!((2 * ix + 1 == n || same(bx[0], -a[0])) && same(bx[1], a[1]) && same(bx[2], a[2])))
+                    return false;
+                if (!(same(by[0], a[0]) && (2 * iy + 1 == n || same(by[1], -a[1])) && same(by[2], a[2])))
+                    return false;
+            }
+    return true;
+}
+
+int t3_device_layout(const double *T, int ng, std::vector<double> &out) {
+    const int lo = t3_antisymmetric_xy(T, ng) ? ng / 2 : 0;
+    const size_t n = (size_t)ng, nx = (size_t)(ng - lo), nz = n + 3;
+    out.resize(t3_device_entries(ng, lo));
+    for (size_t mx = 0; mx < nx; ++mx)
+        for (size_t my = 0; my < nx; ++my) {
+            const double *src = T + (((size_t)lo + mx) * n + (size_t)lo + my) * n * 3u;
+            double *dst = out.data() + (mx * nx + my) * nz * 3u;
+            for (size_t z = 0; z < nz; ++z) {
+                const size_t iz = (z + n - 1) % n;  // layout z = node z + 1, periodic
+                for (int c = 0; c < 3; ++c) dst[z * 3 + c] = src[iz * 3 + c];
+            }
+        }
+    return lo;
+}
 
 size_t partial_bytes(const EvalPlan &p) {
     return (size_t)p.splits * 3u * (size_t)std::max<int64_t>(p.nrows, 1) * sizeof(double);

@@ -83,7 +83,8 @@ size_t footprint(const steps_force_desc &d, int64_t N) {
     size_t b = (size_t)N * (32 + 8 + 24 + 24 + 24) + 4096;
     if (make_plan(dd, N, 0, N - 1, p) == STEPS_OK) b += scratch_bytes(p);
     b += (size_t)(kdk_reduce_blocks(N) + 1) * 8;
-    if (d.topology == STEPS_TOPO_T3 && d.is_periodic >= 2) b += (size_t)d.t3_ngrid * d.t3_ngrid * d.t3_ngrid * 24;
+    if (d.topology == STEPS_TOPO_T3 && d.is_periodic >= 2 && d.t3_ngrid > 0)
+        b += t3_device_entries(d.t3_ngrid, 0) * sizeof(double);  // unfolded layout: the larger one
     if (d.topology == STEPS_TOPO_S1R2 && d.is_periodic >= 2 && d.s1r2_lookup)
         b += (size_t)d.s1r2_nrho * d.s1r2_nz * 16;
     if (d.topology == STEPS_TOPO_S1R2) b += (size_t)std::max(0, d.radial_table_size) * 8;

@@ -34,7 +34,10 @@ struct TableCopy {
     size_t n = 0;
     uint64_t fp = 0;
     DevBuf dev;
+    int lo = 0;  // T3 tables: first x/y node of the device layout
     int sync(const double *h, size_t count, hipStream_t st, const char *what);
+    // T3: upload the table in the pair kernel's layout (t3_device_layout)
+    int sync_t3(const double *h, int ng, hipStream_t st);
     const double *ptr() const { return dev.as<const double>(); }
 };
 

@@ -88,6 +88,65 @@ def test_t3_row_subset_and_shuffled_input(steps, oracle, t3_tab):
     assert np.median(e) <= 1e-12 and np.percentile(e, 99) <= 1e-10
 
 
+def _foldable(T, ng):
+    """The library's fold rule: exact antisymmetry in x and in y, except that
+    components vanishing by symmetry may hold noise below 2^-52 max|T|, and
+    the plane a mirror maps onto itself is exempt from the negated check."""
+    T4 = T.reshape(ng, ng, ng, 3)
+    eps0 = np.ldexp(np.abs(T4).max(), -52)
+
+    def same(b, a):
+        return (b == a) | ((np.abs(a) <= eps0) & (np.abs(b) <= eps0))
+
+    centre = np.zeros(ng, bool)
+    if ng % 2:
+        centre[ng // 2] = True
+    mx, my = T4[::-1], T4[:, ::-1]
+    okx = ((same(mx[..., 0], -T4[..., 0]) | centre[:, None, None]) & same(mx[..., 1], T4[..., 1])
+           & same(mx[..., 2], T4[..., 2]))
+    oky = (same(my[..., 0], T4[..., 0]) & (same(my[..., 1], -T4[..., 1]) | centre[None, :, None])
+           & same(my[..., 2], T4[..., 2]))
+    return bool(okx.all() and oky.all())
+
+
+def test_t3_tables_take_the_folded_layout(steps, t3_tab):
+    """The oracle's table and the device-built one are antisymmetric in x and
+    y to the library's rule, so the pair kernel reads them folded."""
+    ng = round((t3_tab.size // 3) ** (1 / 3))
+    assert _foldable(t3_tab, ng)
+    assert _foldable(steps.t3_table(2, L_T3), ng)
+
+
+def test_t3_unfolded_layout(steps, oracle, t3_tab):
+    """Tables without the symmetry are read unfolded.  (1) One entry that
+    vanishes by symmetry raised just above the noise bound: forces agree with
+    the folded evaluation of the original table to rounding.  (2) The table
+    perturbed by 1e-3 max|T| noise: forces agree with the oracle on that
+    same table."""
+    print("case t3 unfolded layout n_side=16", flush=True)
+    ng = round((t3_tab.size // 3) ** (1 / 3))
+    w = wl.t3_lattice(n_side=16, seed=1001)
+    d0, keep0 = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=w.L, t3_table=t3_tab)
+    F0 = steps.force_eval(d0, w.x, w.M, w.soft)
+    T1 = t3_tab.copy().reshape(ng, ng, ng, 3)
+    T1[ng // 2 - 10, ng // 2, ng // 2, 1] = 4.0 * np.ldexp(np.abs(t3_tab).max(), -52)
+    T1 = T1.reshape(-1)
+    assert not _foldable(T1, ng)
+    d1, keep1 = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=w.L, t3_table=T1)
+    F1 = steps.force_eval(d1, w.x, w.M, w.soft)
+    e = rel_err(F1, F0)
+    report("T3 unfolded (noise entry) vs folded", e)
+    assert np.median(e) <= 1e-14 and e.max() <= 1e-12
+    T2 = t3_tab + 1e-3 * np.abs(t3_tab).max() * np.random.default_rng(11).normal(size=t3_tab.size)
+    assert not _foldable(T2, ng)
+    d2, keep2 = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=w.L, t3_table=T2)
+    F2 = steps.force_eval(d2, w.x, w.M, w.soft)
+    p = oracle.params(1, is_periodic=2, L=w.L, t3_table=T2, interp_order=4)
+    e = rel_err(F2, oracle.forces(p, w.x, w.M, w.soft))
+    report("T3 perturbed table (unfolded) vs oracle", e)
+    assert np.median(e) <= 1e-12 and np.percentile(e, 99) <= 1e-9
+
+
 def test_t3_full_size_sampled_rows(steps, oracle, t3_tab):
     """The 64^3 T3 configuration at full size, checked on 96 sampled rows."""
     print("case t3 forces n_side=64 (N=262144) sampled rows", flush=True)
