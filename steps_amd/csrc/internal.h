@@ -113,11 +113,17 @@ int64_t pack_describe(const EvalPlan &p, const DeviceParticles &dp, double *part
 hipError_t launch_pair_packed(int variant, int precision, const void *dev_descs, int nseg, int64_t total_blocks,
                               hipStream_t stream);
 
-// Spatial (Morton) ordering of n sources into sorted copies jrec_s / soft_s;
-// `scratch` holds order_scratch_bytes(n) bytes (order_kernels.hip).
+// Spatial ordering of n particles into sorted copies jrec_s / soft_s: Morton
+// order over the bounding box, or, with pencil_ng > 0, z pencils over the T3
+// table's cell columns (box side pencil_L, pencil_ng cells per axis);
+// `scratch` holds order_scratch_bytes(n) bytes (order_kernels.hip).  After the
+// launch, spatial_order_perm(scratch, n)[k] is the input index of sorted
+// particle k.
 size_t order_scratch_bytes(int64_t n);
 hipError_t launch_spatial_order(const double4 *jrec, const double *soft, int64_t n, void *scratch,
-                                double4 *jrec_s, double *soft_s, hipStream_t stream);
+                                double4 *jrec_s, double *soft_s, hipStream_t stream, double pencil_L = 0.0,
+                                int pencil_ng = 0);
+const uint32_t *spatial_order_perm(const void *scratch, int64_t n);
 
 // AoS x [3n] + M [n] + soft [n] (device) -> DeviceParticles layout.
 hipError_t launch_pack(const double *x, const double *M, const double *soft, int64_t n,

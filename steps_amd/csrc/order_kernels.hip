@@ -1,16 +1,20 @@
-// order_kernels.hip -- spatial ordering of the sources for the mixed-precision
-// pair kernels.
+// order_kernels.hip -- spatial ordering of the sources (and, for T3, the rows).
 //
 // The FP32 pair arithmetic works on offsets from the centre of each 64-source
 // tile; its rounding error grows with the distance of both particles from that
-// centre, so the tiles have to be spatially compact.  The sources are put in
-// Morton order of their positions (30-bit keys over the bounding box) with a
-// stable LSD radix sort, and gathered into sorted copies that the mixed kernels
-// read; the rows keep the caller's order.  Every step is deterministic (exact
-// min/max, integer counts, stable placement) and no step uses memset or host
-// synchronisation, so the sequence can be captured in a graph.  There is no
-// counterpart in the reference (its FP32 mode is all-FP32 on unsorted input,
-// global_variables.h:26-32).
+// centre, so the tiles have to be spatially compact: the mixed paths (and the
+// FP64 R3 far-tile test) put the sources in Morton order of their positions
+// (30-bit keys over the bounding box).  The T3 Ewald pair kernel is bound by
+// its table gathers, whose cost depends on how many cache lines the 64 lanes
+// of one load touch: rows and sources go into z pencils, ordered by the table
+// cell column (x, y) and then by z, so that neighbouring lanes read
+// neighbouring nodes of the same contiguous z runs.  Both keys are sorted with
+// a stable LSD radix sort and the records gathered into sorted copies.  Every
+// step is deterministic (exact min/max, integer counts, stable placement) and
+// no step uses memset or host synchronisation, so the sequence can be captured
+// in a graph.  There is no counterpart in the reference (its FP32 mode is
+// all-FP32 on unsorted input, global_variables.h:26-32, and its kernels read
+// particles in input order).
 #include "internal.h"
 
 #include <algorithm>
@@ -110,6 +114,27 @@ __global__ __launch_bounds__(256) void morton_kernel(const double4 *__restrict__
         q[a] = (uint32_t)min(1023, max(0, (int)u));
     }
     key[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+    val[i] = (uint32_t)i;
+}
+
+// z-pencil key of each particle (30 bits): the T3 table cell column (x, y),
+// 10 bits each (Ng <= 1023), then z on 1024 levels.  Coordinates are taken
+// modulo L, so a box on [0, L) and one on [-L/2, L/2) give the same pencils
+// up to a rotation of their order.
+__device__ __forceinline__ uint32_t periodic_bin(double c, double inv_L, int nb) {
+    double u = c * inv_L;
+    u -= floor(u);
+    return (uint32_t)min(nb - 1, max(0, (int)(u * (double)nb)));
+}
+
+__global__ __launch_bounds__(256) void pencil_kernel(const double4 *__restrict__ jrec, int64_t n, double L,
+                                                     int ng, uint32_t *__restrict__ key, uint32_t *__restrict__ val) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = jrec[i];
+    const double inv_L = 1.0 / L;
+    key[i] = (periodic_bin(p.x, inv_L, ng) << 20) | (periodic_bin(p.y, inv_L, ng) << 10) |
+             periodic_bin(p.z, inv_L, 1024);
     val[i] = (uint32_t)i;
 }
 
@@ -221,8 +246,13 @@ size_t order_scratch_bytes(int64_t n) {
            a256(6 * 8) + 256;
 }
 
+const uint32_t *spatial_order_perm(const void *scratch, int64_t n) {
+    // the sort's four passes leave the permutation in val[0], after the two key arrays
+    return (const uint32_t *)((const char *)scratch + 2 * a256((size_t)std::max<int64_t>(n, 1) * 4));
+}
+
 hipError_t launch_spatial_order(const double4 *jrec, const double *soft, int64_t n, void *scratch,
-                                double4 *jrec_s, double *soft_s, hipStream_t st) {
+                                double4 *jrec_s, double *soft_s, hipStream_t st, double pencil_L, int pencil_ng) {
     if (n <= 0) return hipSuccess;
     const size_t nn = (size_t)n;
     char *p = (char *)scratch;
@@ -242,14 +272,18 @@ hipError_t launch_spatial_order(const double4 *jrec, const double *soft, int64_t
     p += a256((size_t)kBboxBlocks * 6 * 8);
     double *box = (double *)p;
 
-    const int nbb = (int)std::min<int64_t>(kBboxBlocks, (n + 255) / 256);
-    hipLaunchKernelGGL(bbox_partial_kernel, dim3(nbb), dim3(256), 0, st, jrec, n, part);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(256), 0, st, part, nbb, box);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipError_t e;
     const unsigned ng = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(morton_kernel, dim3(ng), dim3(256), 0, st, jrec, n, box, key[0], val[0]);
+    if (pencil_ng > 0) {
+        hipLaunchKernelGGL(pencil_kernel, dim3(ng), dim3(256), 0, st, jrec, n, pencil_L, pencil_ng, key[0], val[0]);
+    } else {
+        const int nbb = (int)std::min<int64_t>(kBboxBlocks, (n + 255) / 256);
+        hipLaunchKernelGGL(bbox_partial_kernel, dim3(nbb), dim3(256), 0, st, jrec, n, part);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(256), 0, st, part, nbb, box);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(morton_kernel, dim3(ng), dim3(256), 0, st, jrec, n, box, key[0], val[0]);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     int cur = 0;
     for (int shift = 0; shift < 32; shift += 8) {

@@ -903,21 +903,25 @@ __device__ __forceinline__ double radial_correction(double r, double R, const do
     return c;
 }
 
+// perm (or null): partial-sum row r belongs to input row perm[r] (rows taken
+// in spatial order; id_min is 0 then).
 __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ part,
                                                      const double4 *__restrict__ jrec,
+                                                     const uint32_t *__restrict__ perm,
                                                      const double *__restrict__ radial,
                                                      double *__restrict__ F, int64_t nrows,
                                                      int64_t id_min, int splits, int ext_mode,
                                                      double ext_coeff, double Rsim,
                                                      int radial_size) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nrows) return;
+    const int64_t r = perm ? (int64_t)perm[k] : k;  // output row
     double f[3] = {0.0, 0.0, 0.0};
     for (int s = 0; s < splits; ++s) {
         const double *p = part + (size_t)s * 3u * (size_t)nrows;
-        f[0] += p[r];
-        f[1] += p[nrows + r];
-        f[2] += p[2 * nrows + r];
+        f[0] += p[k];
+        f[1] += p[nrows + k];
+        f[2] += p[2 * nrows + k];
     }
     if (ext_mode == 3) {
         // forces_cuda.cu:698-701: mass_in_unit_sphere * x * correction(r_xy), x and y only
@@ -1054,8 +1058,13 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 bool is_mixed(const EvalPlan &p) { return p.precision == STEPS_PREC_MIXED && mixed_supported(p.variant); }
 
-// The mixed kernels and the R3 FP64 kernel read spatially ordered sources.
-bool sorted_sources(const EvalPlan &p) { return is_mixed(p) || p.variant == PV_R3; }
+// The mixed kernels and the R3 FP64 kernel read spatially ordered sources
+// (Morton order); the T3 Ewald kernel reads sources in z pencils.
+bool sorted_sources(const EvalPlan &p) { return is_mixed(p) || p.variant == PV_R3 || p.variant == PV_T3_EWALD; }
+bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
+// The T3 Ewald kernel also takes its rows in that order when it evaluates
+// every particle; the reduction scatters the sums back to input order.
+bool sorted_rows(const EvalPlan &p) { return pencil_order(p) && p.id_min == 0 && p.nrows == p.N; }
 
 size_t s1r2_f32_entries(const EvalPlan &p) {
     return (is_mixed(p) && is_s1r2_lookup(p.variant)) ? (size_t)p.s1r2_nrho * p.s1r2_nz : 0;
@@ -1122,8 +1131,8 @@ KScalars make_scalars(const EvalPlan &p) {
 PairArgs make_args(const EvalPlan &p, const DeviceParticles &dp, const ScratchView &v) {
     PairArgs a{};
     const bool sorted = sorted_sources(p);
-    a.irec = dp.jrec;
-    a.isoft = dp.soft;
+    a.irec = sorted_rows(p) ? v.jrec_s : dp.jrec;
+    a.isoft = sorted_rows(p) ? v.soft_s : dp.soft;
     a.jrec = sorted ? v.jrec_s : dp.jrec;
     a.soft = sorted ? v.soft_s : dp.soft;
     a.jrel = v.jrel;
@@ -1206,7 +1215,7 @@ size_t tile_extra_bytes(const EvalPlan &p) {
     const int64_t nt = (p.N + kTile - 1) / kTile;
     const size_t n = (size_t)std::max<int64_t>(p.N, 1);
     size_t b = 2 * align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4));
-    if (p.precision == STEPS_PREC_MIXED || p.variant == PV_R3) {
+    if (sorted_sources(p)) {
         b += align256(n * sizeof(float4));
         b += align256(s1r2_f32_entries(p) * sizeof(float2));
         b += align256(n * sizeof(double4)) + align256(n * sizeof(double));
@@ -1228,7 +1237,8 @@ hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, d
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (sorted_sources(p)) {
-        if ((e = launch_spatial_order(dp.jrec, dp.soft, p.N, v.order, v.jrec_s, v.soft_s, st)) != hipSuccess)
+        if ((e = launch_spatial_order(dp.jrec, dp.soft, p.N, v.order, v.jrec_s, v.soft_s, st,
+                                      pencil_order(p) ? p.L : 0.0, pencil_order(p) ? p.t3_ng : 0)) != hipSuccess)
             return e;
         if (is_mixed(p))
             hipLaunchKernelGGL((tiles_kernel<true>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles,
@@ -1254,8 +1264,10 @@ hipError_t launch_forces_finish(const EvalPlan &p, const DeviceParticles &dp, do
                                 hipStream_t st) {
     if (p.nrows <= 0) return hipSuccess;
     const int splits = (p.N > 0) ? p.splits : 0;
+    const uint32_t *perm =
+        (sorted_rows(p) && has_pairs(p)) ? spatial_order_perm(scratch_view(p, partials).order, p.N) : nullptr;
     hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)((p.nrows + 255) / 256)), dim3(256), 0, st, partials,
-                       dp.jrec, p.radial, F, p.nrows, p.id_min, splits, p.ext_mode, p.ext_coeff, p.Rsim,
+                       dp.jrec, perm, p.radial, F, p.nrows, p.id_min, splits, p.ext_mode, p.ext_coeff, p.Rsim,
                        p.radial_size);
     return hipGetLastError();
 }
