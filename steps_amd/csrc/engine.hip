@@ -114,6 +114,24 @@ int TableCopy::sync_t3(const double *h, int ng, hipStream_t st) {
     return STEPS_OK;
 }
 
+int TableCopy::sync_s1r2(const double *h, int nrho, int nz, hipStream_t st) {
+    const size_t count = (size_t)nrho * nz * 2;
+    if (!h || nrho < 1 || nz < 1) return STEPS_OK;
+    const uint64_t f = fingerprint(h, count);
+    if (h == host && count == n && f == fp && dev.ptr) return STEPS_OK;
+    std::vector<double> layout;
+    s1r2_device_layout(h, nrho, nz, layout);
+    int rc = dev.reserve(layout.size() * sizeof(double), "S1R2 table");
+    if (rc) return rc;
+    STEPS_HIP(hipMemcpyAsync(dev.ptr, layout.data(), layout.size() * sizeof(double), hipMemcpyHostToDevice, st),
+              "S1R2 table");
+    STEPS_HIP(hipStreamSynchronize(st), "S1R2 table");
+    host = h;
+    n = count;
+    fp = f;
+    return STEPS_OK;
+}
+
 int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_max, EvalPlan &p) {
     if (N < 0) return fail(STEPS_ERR_ARG, "N < 0");
     if (N > (int64_t)INT32_MAX * 4) return fail(STEPS_ERR_ARG, "N too large");
@@ -163,6 +181,8 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
         } else if (d.s1r2_lookup) {
             if (!d.s1r2_table || d.s1r2_nrho < 2 || d.s1r2_nz < 1)
                 return fail(STEPS_ERR_TABLE, "S1R2 Ewald table missing (or fewer than 2 rho nodes)");
+            if (s1r2_device_nodes(d.s1r2_nrho, d.s1r2_nz) >= ((size_t)1 << 28))
+                return fail(STEPS_ERR_TABLE, "S1R2 table too large (2^28 nodes at most)");
             p.variant = d.interp_order == 0 ? PV_S1R2_NGP
                         : d.interp_order == 2 ? PV_S1R2_CIC
                                               : PV_S1R2_TSC;  // forces_cuda.cu:449-451 fallback
@@ -285,7 +305,7 @@ int sim_create(const steps_force_desc &d, int64_t N, bool both_precisions, steps
     if (p.variant == PV_T3_EWALD && (rc = s->t3.sync_t3(d.t3_table, d.t3_ngrid, s->stream)))
         return cleanup(rc);
     if (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC &&
-        (rc = s->s1r2.sync(d.s1r2_table, (size_t)d.s1r2_nrho * d.s1r2_nz * 2, s->stream, "S1R2 table")))
+        (rc = s->s1r2.sync_s1r2(d.s1r2_table, d.s1r2_nrho, d.s1r2_nz, s->stream)))
         return cleanup(rc);
     if (p.ext_mode == 3 &&
         (rc = s->radial.sync(d.radial_table, (size_t)d.radial_table_size, s->stream, "radial table")))
@@ -488,7 +508,7 @@ int steps_force_eval(const steps_force_desc *d, int64_t N, const double *x, cons
     hipStream_t st = o.stream;
     if (p.variant == PV_T3_EWALD && (rc = o.t3.sync_t3(d->t3_table, d->t3_ngrid, st))) return rc;
     if (p.variant >= PV_S1R2_NGP && p.variant <= PV_S1R2_TSC &&
-        (rc = o.s1r2.sync(d->s1r2_table, (size_t)d->s1r2_nrho * d->s1r2_nz * 2, st, "S1R2 table")))
+        (rc = o.s1r2.sync_s1r2(d->s1r2_table, d->s1r2_nrho, d->s1r2_nz, st)))
         return rc;
     if (p.ext_mode == 3 && (rc = o.radial.sync(d->radial_table, (size_t)d->radial_table_size, st, "radial table")))
         return rc;

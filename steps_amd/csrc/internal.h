@@ -163,6 +163,14 @@ inline size_t t3_device_entries(int ng, int lo) {
     return (size_t)(ng - lo) * (size_t)(ng - lo) * (size_t)(ng + 3) * 3u;
 }
 constexpr int kT3MaxGrid = 1023;  // element offsets of the layout stay below 2^32
+// S1R2 table in the layout the pair kernels read.  The reference table is
+// [(ir*Nz + iz)*2 + c].  The device layout keeps rho rows with z contiguous
+// but adds ghosts: rows for rho nodes -1 .. Nrho (repeating the edge nodes,
+// the reference's clamp), each with z nodes -1 .. Nz + 1 (periodic), node
+// (ir, iz) at ((ir + 1)*(Nz + 3) + iz + 1)*2 + c, so no stencil needs a wrap
+// or a clamp.
+void s1r2_device_layout(const double *T, int nrho, int nz, std::vector<double> &out);
+inline size_t s1r2_device_nodes(int nrho, int nz) { return ((size_t)nrho + 2) * ((size_t)nz + 3); }
 
 // Table builders (table_kernels.hip); all buffers host memory.
 int build_t3_table(int is_periodic, double L, double *table, int *ngrid);

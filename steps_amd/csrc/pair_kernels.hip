@@ -228,53 +228,58 @@ __device__ __forceinline__ void t3_interp(const double *__restrict__ T, const KS
     Dz = s2;
 }
 
-// (D_rho, D_z) of node (ir, iz): one 16-byte load at a 32-bit byte offset
-// from the table base (the table has fewer than 2^28 nodes).
-__device__ __forceinline__ double2 s1r2_node(const double *__restrict__ T, int Nz, int ir, int iz) {
-    const unsigned off = ((unsigned)ir * (unsigned)Nz + (unsigned)iz) * 16u;
-    return *(const double2 *)((const char *)T + off);
+// S1R2 table in the device layout of s1r2_device_layout: one row per rho
+// node -1 .. Nrho (the ghost rows repeat the edge nodes, which is the
+// reference's clamp of the rho index), each row the z nodes -1 .. Nz + 1
+// (periodic ghosts), each node (D_rho, D_z) as 16 bytes.  Node (ir, iz) sits
+// at ((ir + 1) * (Nz + 3) + iz + 1) * 16 bytes; s1r2_row(T, k, ir) + iz + 1
+// points at it.
+__device__ __forceinline__ const double2 *s1r2_row(const double *__restrict__ T, const KScalars &k, int ir) {
+    return (const double2 *)((const char *)T + (unsigned)(ir + 1) * (unsigned)(k.nz + 3) * 16u);
+}
+
+// A z index in [lo_ok, hi_ok] needs no wrap (the ghost rows cover it); any
+// other value (a position outside the box) takes the general periodic
+// remainder, so every read stays inside the table.
+__device__ __forceinline__ int s1r2_zfix(int iz, int lo_ok, int hi_ok, int Nz) {
+    if (iz < lo_ok || iz > hi_ok) iz = imodp(iz, Nz);
+    return iz;
 }
 
 // forces_cuda.cu:188-455: NGP / CIC / TSC interpolation of (D_rho, D_z) at
-// (rho, z), rho clamped to [0, rho_max], z periodic; back to Cartesian.
+// (rho, z), rho clamped to [0, rho_max], z periodic.  Returns q = D_rho / rho
+// (0 on the axis) and D_z: the caller forms D_x = q dx, D_y = q dy.
 // Divisions by the grid spacings become multiplications by their reciprocals
 // and rho, 1/rho come from one reciprocal square root (each within an ulp or
 // two of the reference's sqrt and divides; the interpolants are continuous).
 template <int ORDER>
 __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, const KScalars &k, double dx, double dy,
-                                            double dz, double &Dx, double &Dy, double &Dz) {
-    const int Nrho = k.nrho, Nz = k.nz;
-    const double rho_max = k.rho_max;
+                                            double dz, double &q, double &Dz) {
+    const int Nz = k.nz;
     const double rxy2 = fma(dx, dx, dy * dy);
     const double inv_rho = (rxy2 > 0.0) ? rsqrt_f64(rxy2) : 0.0;
     const double rho = rxy2 * inv_rho;
     const double uz = fma(dz + k.halfL, k.inv_dzg, -0.5);
     double Dr, Dzv;
     if (ORDER == 0) {
-        double rc = rho;
-        if (rho > rho_max) rc = rho_max;
-        const double ur = rc * k.inv_drho;  // inv_drho = 0 when drho = 0
-        int ir = (int)floor(ur + 0.5);
-        const int iz = wrap_index((int)floor(uz + 0.5), Nz);
-        ir = min(max(ir, 0), Nrho - 1);
-        const double2 t = s1r2_node(T, Nz, ir, iz);
+        const int ir = (int)floor(fmin(rho, k.rho_max) * k.inv_drho + 0.5);  // in [0, Nrho-1]
+        const int iz = s1r2_zfix((int)floor(uz + 0.5), 0, Nz, Nz);
+        const double2 t = s1r2_row(T, k, ir)[iz + 1];
         Dr = t.x;
         Dzv = t.y;
     } else if (ORDER == 2) {
         // rho beyond rho_max clamps to the last node (forces_cuda.cu:225-239);
-        // min() and the clamped floor give the reference's (ir0, fr) up to the
-        // rounding of rho * (1/drho), where the interpolant is continuous
+        // at ur = Nrho-1 the upper node is the ghost copy of the last one
         const double ur = fmin(rho * k.inv_drho, k.last_r);
-        const double urf = fmin(floor(ur), k.last_r - 1.0);
+        const double urf = floor(ur);
         const double fr = ur - urf;
         const int ir0 = (int)urf;
-        const int ir1 = ir0 + 1;
         const double uzf = floor(uz);
         const double fz = uz - uzf;
-        const int iz0 = wrap_index((int)uzf, Nz);
-        const int iz1 = wrap_once(iz0 + 1, Nz);
-        const double2 t00 = s1r2_node(T, Nz, ir0, iz0), t10 = s1r2_node(T, Nz, ir1, iz0);
-        const double2 t01 = s1r2_node(T, Nz, ir0, iz1), t11 = s1r2_node(T, Nz, ir1, iz1);
+        const int iz0 = s1r2_zfix((int)uzf, -1, Nz - 1, Nz);
+        const double2 *r0 = s1r2_row(T, k, ir0) + (iz0 + 1);      // z nodes iz0, iz0 + 1
+        const double2 *r1 = s1r2_row(T, k, ir0 + 1) + (iz0 + 1);
+        const double2 t00 = r0[0], t01 = r0[1], t10 = r1[0], t11 = r1[1];
         // bilinear as two lerps in rho and one in z (same interpolant as the
         // reference's four weights, fewer operations)
         const double a0 = fma(fr, t10.x - t00.x, t00.x), b0 = fma(fr, t11.x - t01.x, t01.x);
@@ -282,30 +287,27 @@ __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, const 
         Dr = fma(fz, b0 - a0, a0);
         Dzv = fma(fz, b1 - a1, a1);
     } else {
-        const double ur = (rho > rho_max) ? (double)(Nrho - 1) : rho * k.inv_drho;
+        const double ur = fmin(rho * k.inv_drho, k.last_r);
         const double urf = floor(ur + 0.5), uzf = floor(uz + 0.5);
-        const int jr = (int)urf, jz = (int)uzf;
+        const int jr = (int)urf;  // in [0, Nrho-1]; nodes jr-1 .. jr+1 (ghosts at the ends)
+        const int jz = s1r2_zfix((int)uzf, 0, Nz, Nz);
         const double sr = ur - urf, sz = uz - uzf;
         const double wr[3] = {0.5 * (0.5 - sr) * (0.5 - sr), 0.75 - sr * sr,
                               0.5 * (0.5 + sr) * (0.5 + sr)};
         const double wz[3] = {0.5 * (0.5 - sz) * (0.5 - sz), 0.75 - sz * sz,
                               0.5 * (0.5 + sz) * (0.5 + sz)};
-        const int irs[3] = {max(jr - 1, 0), min(max(jr, 0), Nrho - 1), min(jr + 1, Nrho - 1)};
-        const int izm = wrap_index(jz, Nz);
-        const int izs[3] = {wrap_once(izm - 1, Nz), izm, wrap_once(izm + 1, Nz)};
         Dr = 0.0;
         Dzv = 0.0;
 #pragma unroll
-        for (int row = 0; row < 3; ++row) {
-            const double w0 = wr[0] * wz[row], w1 = wr[1] * wz[row], w2 = wr[2] * wz[row];
-            const double2 a = s1r2_node(T, Nz, irs[0], izs[row]), b = s1r2_node(T, Nz, irs[1], izs[row]),
-                          c = s1r2_node(T, Nz, irs[2], izs[row]);
-            Dr += w0 * a.x + w1 * b.x + w2 * c.x;
-            Dzv += w0 * a.y + w1 * b.y + w2 * c.y;
+        for (int c = 0; c < 3; ++c) {
+            const double2 *rr = s1r2_row(T, k, jr - 1 + c) + jz;  // rho node jr-1+c, z nodes jz-1 .. jz+1
+            const double w0 = wr[c] * wz[0], w1 = wr[c] * wz[1], w2 = wr[c] * wz[2];
+            const double2 a = rr[0], b = rr[1], d = rr[2];
+            Dr += w0 * a.x + w1 * b.x + w2 * d.x;
+            Dzv += w0 * a.y + w1 * b.y + w2 * d.y;
         }
     }
-    Dx = Dr * (dx * inv_rho);
-    Dy = Dr * (dy * inv_rho);
+    q = Dr * inv_rho;
     Dz = Dzv;
 }
 
@@ -384,15 +386,17 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
             fy = fma(s, dy, fy);
             fz = fma(s, dz, fz);
         } else {
-            double Dx, Dy, Dz;
+            // m (w d - D) with D_x = q dx, D_y = q dy: x and y share m (w - q)
+            double q, Dz;
             if (V == PV_S1R2_NGP)
-                s1r2_interp<0>(table, k, dx, dy, dz, Dx, Dy, Dz);
+                s1r2_interp<0>(table, k, dx, dy, dz, q, Dz);
             else if (V == PV_S1R2_CIC)
-                s1r2_interp<2>(table, k, dx, dy, dz, Dx, Dy, Dz);
+                s1r2_interp<2>(table, k, dx, dy, dz, q, Dz);
             else
-                s1r2_interp<4>(table, k, dx, dy, dz, Dx, Dy, Dz);
-            fx = fma(pj.w, fma(w, dx, -Dx), fx);
-            fy = fma(pj.w, fma(w, dy, -Dy), fy);
+                s1r2_interp<4>(table, k, dx, dy, dz, q, Dz);
+            const double sxy = pj.w * (w - q);
+            fx = fma(sxy, dx, fx);
+            fy = fma(sxy, dy, fy);
             fz = fma(pj.w, fma(w, dz, -Dz), fz);
         }
     }
@@ -658,61 +662,59 @@ __device__ __forceinline__ void body_mixed_r3(const PairArgs &a, const int bx, c
 // ---------------------------------------------------------------------------
 constexpr int s1r2_order(int V) { return V == PV_S1R2_CIC ? 2 : 4; }
 
-__device__ __forceinline__ float2 node32(const float2 *__restrict__ T, int Nz, int ir, int iz) {
-    const unsigned off = ((unsigned)ir * (unsigned)Nz + (unsigned)iz) * 8u;
-    return *(const float2 *)((const char *)T + off);
+// FP32 copy of the device layout (s1r2_row): same rows, 8-byte nodes.
+__device__ __forceinline__ const float2 *s1r2_row32(const float2 *__restrict__ T, const KScalars &k, int ir) {
+    return (const float2 *)((const char *)T + (unsigned)(ir + 1) * (unsigned)(k.nz + 3) * 8u);
 }
 
+// s1r2_interp in FP32 (CIC, TSC): returns q = D_rho / rho and D_z.
 template <int ORDER>
 __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, const KScalars &k, float dx,
-                                                float dy, float dz, float &Dx, float &Dy, float &Dz) {
-    const int Nrho = k.nrho, Nz = k.nz;
+                                                float dy, float dz, float &q, float &Dz) {
+    const int Nz = k.nz;
     const float rxy2 = fmaf(dx, dx, dy * dy);
     const float inv_rho = (rxy2 > 0.0f) ? __builtin_amdgcn_rsqf(rxy2) : 0.0f;
     const float rho = rxy2 * inv_rho;
     const float uz = fmaf(dz + k.halfLf, k.inv_dzgf, -0.5f);
+    const float lastr = (float)k.last_r;
     float Dr, Dzv;
     static_assert(ORDER == 2 || ORDER == 4, "FP32 interpolation is CIC or TSC");
     if (ORDER == 2) {
-        const float lastr = (float)k.last_r;
         const float ur = fminf(rho * k.inv_drhof, lastr);
-        const float urf = fminf(floorf(ur), lastr - 1.0f);
+        const float urf = floorf(ur);
         const float fr = ur - urf;
         const int ir0 = (int)urf;
         const float uzf = floorf(uz);
         const float fz = uz - uzf;
-        const int iz0 = wrap_index((int)uzf, Nz);
-        const int iz1 = wrap_once(iz0 + 1, Nz);
-        const float2 t00 = node32(T, Nz, ir0, iz0), t10 = node32(T, Nz, ir0 + 1, iz0);
-        const float2 t01 = node32(T, Nz, ir0, iz1), t11 = node32(T, Nz, ir0 + 1, iz1);
+        const int iz0 = s1r2_zfix((int)uzf, -1, Nz - 1, Nz);
+        const float2 *r0 = s1r2_row32(T, k, ir0) + (iz0 + 1);
+        const float2 *r1 = s1r2_row32(T, k, ir0 + 1) + (iz0 + 1);
+        const float2 t00 = r0[0], t01 = r0[1], t10 = r1[0], t11 = r1[1];
         const float a0 = fmaf(fr, t10.x - t00.x, t00.x), b0 = fmaf(fr, t11.x - t01.x, t01.x);
         const float a1 = fmaf(fr, t10.y - t00.y, t00.y), b1 = fmaf(fr, t11.y - t01.y, t01.y);
         Dr = fmaf(fz, b0 - a0, a0);
         Dzv = fmaf(fz, b1 - a1, a1);
     } else {
-        const float ur = (rho > k.rho_maxf) ? (float)(Nrho - 1) : rho * k.inv_drhof;
+        const float ur = fminf(rho * k.inv_drhof, lastr);
         const float urf = floorf(ur + 0.5f), uzf = floorf(uz + 0.5f);
-        const int jr = (int)urf, jz = (int)uzf;
+        const int jr = (int)urf;
+        const int jz = s1r2_zfix((int)uzf, 0, Nz, Nz);
         const float sr = ur - urf, sz = uz - uzf;
         const float wr[3] = {0.5f * (0.5f - sr) * (0.5f - sr), 0.75f - sr * sr, 0.5f * (0.5f + sr) * (0.5f + sr)};
         const float wz[3] = {0.5f * (0.5f - sz) * (0.5f - sz), 0.75f - sz * sz, 0.5f * (0.5f + sz) * (0.5f + sz)};
-        const int irs[3] = {max(jr - 1, 0), min(max(jr, 0), Nrho - 1), min(jr + 1, Nrho - 1)};
-        const int izm = wrap_index(jz, Nz);
-        const int izs[3] = {wrap_once(izm - 1, Nz), izm, wrap_once(izm + 1, Nz)};
         Dr = 0.0f;
         Dzv = 0.0f;
 #pragma unroll
         for (int row = 0; row < 3; ++row)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float2 t = node32(T, Nz, irs[c], izs[row]);
+                const float2 t = s1r2_row32(T, k, jr - 1 + c)[jz + row];
                 const float w = wr[c] * wz[row];
                 Dr = fmaf(w, t.x, Dr);
                 Dzv = fmaf(w, t.y, Dzv);
             }
     }
-    Dx = Dr * (dx * inv_rho);
-    Dy = Dr * (dy * inv_rho);
+    q = Dr * inv_rho;
     Dz = Dzv;
 }
 
@@ -744,10 +746,11 @@ __device__ __forceinline__ void mixed_pair_z(const KScalars &k, const float2 *__
         ay = fmaf(s, dy, ay);
         az = fmaf(s, dz, az);
     } else {
-        float Dx, Dy, Dz;
-        s1r2_interp_f32<s1r2_order(V)>(T, k, dx, dy, dz, Dx, Dy, Dz);
-        ax = fmaf(m, fmaf(y3, dx, -Dx), ax);
-        ay = fmaf(m, fmaf(y3, dy, -Dy), ay);
+        float q, Dz;
+        s1r2_interp_f32<s1r2_order(V)>(T, k, dx, dy, dz, q, Dz);
+        const float sxy = m * (y3 - q);
+        ax = fmaf(sxy, dx, ax);
+        ay = fmaf(sxy, dy, ay);
         az = fmaf(m, fmaf(y3, dz, -Dz), az);
     }
 }
@@ -1067,7 +1070,7 @@ bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
 bool sorted_rows(const EvalPlan &p) { return pencil_order(p) && p.id_min == 0 && p.nrows == p.N; }
 
 size_t s1r2_f32_entries(const EvalPlan &p) {
-    return (is_mixed(p) && is_s1r2_lookup(p.variant)) ? (size_t)p.s1r2_nrho * p.s1r2_nz : 0;
+    return (is_mixed(p) && is_s1r2_lookup(p.variant)) ? s1r2_device_nodes(p.s1r2_nrho, p.s1r2_nz) : 0;
 }
 
 // Pieces of an evaluation's scratch buffer, in the order tile_extra_bytes
@@ -1200,6 +1203,20 @@ int t3_device_layout(const double *T, int ng, std::vector<double> &out) {
             }
         }
     return lo;
+}
+
+void s1r2_device_layout(const double *T, int nrho, int nz, std::vector<double> &out) {
+    const size_t nr = (size_t)nrho + 2, nzl = (size_t)nz + 3;
+    out.resize(nr * nzl * 2u);
+    for (size_t r = 0; r < nr; ++r) {
+        const size_t ir = std::min<size_t>(r == 0 ? 0 : r - 1, (size_t)nrho - 1);  // clamped edges
+        for (size_t z = 0; z < nzl; ++z) {
+            const size_t iz = (z + (size_t)nz - 1) % (size_t)nz;  // entry z holds z node z - 1, periodic
+            const double *src = T + (ir * (size_t)nz + iz) * 2u;  // reference layout [(ir*Nz + iz)*2 + c]
+            out[(r * nzl + z) * 2u] = src[0];
+            out[(r * nzl + z) * 2u + 1] = src[1];
+        }
+    }
 }
 
 size_t partial_bytes(const EvalPlan &p) {

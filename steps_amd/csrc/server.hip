@@ -86,7 +86,7 @@ size_t footprint(const steps_force_desc &d, int64_t N) {
     if (d.topology == STEPS_TOPO_T3 && d.is_periodic >= 2 && d.t3_ngrid > 0)
         b += t3_device_entries(d.t3_ngrid, 0) * sizeof(double);  // unfolded layout: the larger one
     if (d.topology == STEPS_TOPO_S1R2 && d.is_periodic >= 2 && d.s1r2_lookup)
-        b += (size_t)d.s1r2_nrho * d.s1r2_nz * 16;
+        b += s1r2_device_nodes(std::max(d.s1r2_nrho, 1), std::max(d.s1r2_nz, 1)) * 16;
     if (d.topology == STEPS_TOPO_S1R2) b += (size_t)std::max(0, d.radial_table_size) * 8;
     return b;
 }

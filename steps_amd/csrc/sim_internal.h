@@ -36,8 +36,10 @@ struct TableCopy {
     DevBuf dev;
     int lo = 0;  // T3 tables: first x/y node of the device layout
     int sync(const double *h, size_t count, hipStream_t st, const char *what);
-    // T3: upload the table in the pair kernel's layout (t3_device_layout)
+    // T3 / S1R2: upload the table in the pair kernels' layout
+    // (t3_device_layout, s1r2_device_layout)
     int sync_t3(const double *h, int ng, hipStream_t st);
+    int sync_s1r2(const double *h, int nrho, int nz, hipStream_t st);
     const double *ptr() const { return dev.as<const double>(); }
 };
 
