@@ -124,6 +124,8 @@ hipError_t launch_spatial_order(const double4 *jrec, const double *soft, int64_t
                                 double4 *jrec_s, double *soft_s, hipStream_t stream, double pencil_L = 0.0,
                                 int pencil_ng = 0);
 const uint32_t *spatial_order_perm(const void *scratch, int64_t n);
+// ... and spatial_order_keys(scratch, n)[k] its sort key.
+const uint32_t *spatial_order_keys(const void *scratch, int64_t n);
 
 // AoS x [3n] + M [n] + soft [n] (device) -> DeviceParticles layout.
 hipError_t launch_pack(const double *x, const double *M, const double *soft, int64_t n,

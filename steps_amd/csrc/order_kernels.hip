@@ -246,6 +246,11 @@ size_t order_scratch_bytes(int64_t n) {
            a256(6 * 8) + 256;
 }
 
+const uint32_t *spatial_order_keys(const void *scratch, int64_t n) {
+    (void)n;  // the sorted keys end in key[0], at the start of the scratch
+    return (const uint32_t *)scratch;
+}
+
 const uint32_t *spatial_order_perm(const void *scratch, int64_t n) {
     // the sort's four passes leave the permutation in val[0], after the two key arrays
     return (const uint32_t *)((const char *)scratch + 2 * a256((size_t)std::max<int64_t>(n, 1) * 4));

@@ -64,6 +64,7 @@ struct PairArgs {
     const double4 *tile_ext;  // half extents of the tiles' bounding boxes (sorted sources)
     const double *table;
     const float2 *tab32;
+    const uint32_t *keys;     // pencil keys of the sorted particles (T3 Ewald), else null
     double *part;
     KScalars k;
 };
@@ -174,58 +175,74 @@ __device__ __forceinline__ int t3_axis(double d, const KScalars &k, double w[4])
 // The contraction runs z, then y, then x: Sum_i wx_i Sum_j wy_j Sum_k wz_k T.
 // The reference divides by the grid spacing; multiplying by its reciprocal
 // moves u by at most an ulp, and the interpolant is continuous in u.
-__device__ __forceinline__ void t3_interp(const double *__restrict__ T, const KScalars &k, double dx,
-                                          double dy, double dz, double &Dx, double &Dy, double &Dz) {
-    const int Ng = k.ng, lo = k.t3_lo;
-    const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3);
+struct T3Stencil {
     double wx[4], wy[4], wz[4], wxs[4], wys[4];
-    unsigned mx[4], my[4];
-    const int ix0 = t3_axis(dx, k, wx);
-    const int iy0 = t3_axis(dy, k, wy);
-    const int iz0 = t3_axis(dz, k, wz);
+    unsigned mx[4], my[4];  // layout x / y node of each stencil plane
+    int iz0;                // first z node + 1 = layout z index of the run
+};
+
+__device__ __forceinline__ void t3_stencil(const KScalars &k, double dx, double dy, double dz, T3Stencil &s) {
+    const int Ng = k.ng, lo = k.t3_lo;
+    const int ix0 = t3_axis(dx, k, s.wx);
+    const int iy0 = t3_axis(dy, k, s.wy);
+    s.iz0 = t3_axis(dz, k, s.wz);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int jx = wrap_once(ix0 - 1 + q, Ng);  // Ng >= 3
         const int jy = wrap_once(iy0 - 1 + q, Ng);
-        mx[q] = (unsigned)((jx >= lo) ? jx - lo : Ng - 1 - lo - jx);
-        my[q] = (unsigned)((jy >= lo) ? jy - lo : Ng - 1 - lo - jy);
-        wxs[q] = (jx >= lo) ? wx[q] : -wx[q];
-        wys[q] = (jy >= lo) ? wy[q] : -wy[q];
+        s.mx[q] = (unsigned)((jx >= lo) ? jx - lo : Ng - 1 - lo - jx);
+        s.my[q] = (unsigned)((jy >= lo) ? jy - lo : Ng - 1 - lo - jy);
+        s.wxs[q] = (jx >= lo) ? s.wx[q] : -s.wx[q];
+        s.wys[q] = (jy >= lo) ? s.wy[q] : -s.wy[q];
     }
+}
+
+// The staged contraction over the 16 (x, y) rows; row(i, j) returns the
+// address of the stencil's twelve z-run doubles in table row (mx[i], my[j]).
+template <class Row>
+__device__ __forceinline__ void t3_contract(const T3Stencil &s, Row row, double &Dx, double &Dy, double &Dz) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         double u0 = 0.0, u1 = 0.0, u2 = 0.0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            // z node iz0-1+q sits at layout index iz0+q (one ghost below)
-            const unsigned e = ((mx[i] * NY + my[j]) * NZ + (unsigned)iz0) * 3u;
-            const double2u *row = (const double2u *)(T + e);
+            const double2u *r = (const double2u *)row(i, j);
             double v[12];
 #pragma unroll
             for (int h = 0; h < 6; ++h) {
-                const double2u a = row[h];
+                const double2u a = r[h];
                 v[2 * h] = a.x;
                 v[2 * h + 1] = a.y;
             }
-            double t0 = wz[0] * v[0], t1 = wz[0] * v[1], t2 = wz[0] * v[2];
+            double t0 = s.wz[0] * v[0], t1 = s.wz[0] * v[1], t2 = s.wz[0] * v[2];
 #pragma unroll
             for (int q = 1; q < 4; ++q) {
-                t0 = fma(wz[q], v[3 * q], t0);
-                t1 = fma(wz[q], v[3 * q + 1], t1);
-                t2 = fma(wz[q], v[3 * q + 2], t2);
+                t0 = fma(s.wz[q], v[3 * q], t0);
+                t1 = fma(s.wz[q], v[3 * q + 1], t1);
+                t2 = fma(s.wz[q], v[3 * q + 2], t2);
             }
-            u0 = fma(wy[j], t0, u0);
-            u1 = fma(wys[j], t1, u1);
-            u2 = fma(wy[j], t2, u2);
+            u0 = fma(s.wy[j], t0, u0);
+            u1 = fma(s.wys[j], t1, u1);
+            u2 = fma(s.wy[j], t2, u2);
         }
-        s0 = fma(wxs[i], u0, s0);
-        s1 = fma(wx[i], u1, s1);
-        s2 = fma(wx[i], u2, s2);
+        s0 = fma(s.wxs[i], u0, s0);
+        s1 = fma(s.wx[i], u1, s1);
+        s2 = fma(s.wx[i], u2, s2);
     }
     Dx = s0;
     Dy = s1;
     Dz = s2;
+}
+
+__device__ __forceinline__ void t3_interp(const double *__restrict__ T, const KScalars &k, double dx,
+                                          double dy, double dz, double &Dx, double &Dy, double &Dz) {
+    T3Stencil s;
+    t3_stencil(k, dx, dy, dz, s);
+    const unsigned NY = (unsigned)(k.ng - k.t3_lo), NZ = (unsigned)(k.ng + 3);
+    // z node iz0-1+q sits at layout index iz0+q (one ghost below)
+    t3_contract(s, [&](int i, int j) { return T + ((s.mx[i] * NY + s.my[j]) * NZ + (unsigned)s.iz0) * 3u; },
+                Dx, Dy, Dz);
 }
 
 // S1R2 table in the device layout of s1r2_device_layout: one row per rho
@@ -855,11 +872,156 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
 // evaluations of one variant packed into one launch (flat grid, each
 // workgroup finds its evaluation among at most kMaxPack segments)
 // ---------------------------------------------------------------------------
-enum BodyKind : int { BK_F64 = 0, BK_MIXED_R3 = 1, BK_MIXED_Z = 2 };
+// ---------------------------------------------------------------------------
+// T3 Ewald pair kernel with a table window in LDS.  Rows and sources are in z
+// pencils (order_kernels.hip: key = cell column (x, y), then z), so a
+// workgroup's 256 rows cover a few adjacent cell columns and one run of
+// sources covers one column.  The table rows that any of their stencils can
+// reach then form a small (x, y) window of full z runs (5 x 9 rows of the
+// folded 63^3 table: 71 KB).  At each change of source column the workgroup
+// copies that window into LDS between two barriers (the source loop is the
+// same for every wave, so both barriers are reached by all); each pair then
+// reads its 16 stencil rows from LDS, which feeds 128 B per clock against the
+// 64 B per clock of the vector-L1 path the global reads take.  A lane whose
+// stencil leaves the window (a coordinate rounded across a cell edge) reads
+// that pair from global memory, and a window that does not fit (workgroups
+// whose rows wrap around the box) leaves the whole run on the global path, so
+// every read stays in bounds.  Same arithmetic on the same values as the
+// global kernel: the sums are bitwise equal.
+// ---------------------------------------------------------------------------
+constexpr int kT3WinDoubles = 9216;  // 72 KB of LDS per workgroup: two workgroups per CU
+
+// Layout nodes one axis of the window needs: sources in cell column s, rows
+// in columns [bmin, bmax].  d / h lies in (s - b - 1, s - b + 1) (mod Ng), so
+// the stencils' unfolded nodes lie in [s - bmax + (Ng-1)/2 - 2,
+// s - bmin + Ng/2 + 2] (mod Ng); their images in the folded layout are a
+// contiguous range [f0, f0 + nf).  False when the range is too wide to be one.
+__device__ __forceinline__ bool t3_win_axis(int s, int bmin, int bmax, int Ng, int lo, int &f0, int &nf) {
+    const int nlo = (s - bmax) + (Ng - 1) / 2 - 2;
+    const int nhi = (s - bmin) + Ng / 2 + 2;
+    if (nhi - nlo >= Ng / 2) return false;
+    int mn = Ng, mxv = -1;
+    for (int n = nlo; n <= nhi; ++n) {
+        const int jj = imodp(n, Ng);
+        const int m = (jj >= lo) ? jj - lo : Ng - 1 - lo - jj;
+        mn = min(mn, m);
+        mxv = max(mxv, m);
+    }
+    f0 = mn;
+    nf = mxv - mn + 1;
+    return true;
+}
+
+__device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, const int split) {
+    __shared__ double win[kT3WinDoubles];
+    __shared__ int brange[4];
+    const double4 *__restrict__ irec = a.irec;
+    const double *__restrict__ isoft = a.isoft;
+    const double4 *__restrict__ jrec = a.jrec;
+    const double *__restrict__ soft = a.soft;
+    const double4 *__restrict__ tiles = a.tiles;
+    const double *__restrict__ table = a.table;
+    const uint32_t *__restrict__ keys = a.keys;
+    KScalars k = a.k;
+    k.c15 = in_vgpr(a.k.c15);
+    const int tid = threadIdx.x;
+    const int64_t row_base = (int64_t)bx * kBlock;
+    const int64_t j0 = (int64_t)split * k.jchunk;  // multiple of kTile
+    const int64_t j1 = min(k.N, j0 + k.jchunk);
+    // rows are the sorted particles (id_min = 0, nrows = N)
+    const int64_t r = min(row_base + tid, k.nrows - 1);
+    const double4 p = irec[r];
+    const double xi = p.x, yi = p.y, zi = p.z, hi = isoft[r];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const uint32_t kr = keys[r];
+    if (tid == 0) {
+        brange[0] = INT32_MAX;
+        brange[1] = -1;
+        brange[2] = INT32_MAX;
+        brange[3] = -1;
+    }
+    __syncthreads();
+    atomicMin(&brange[0], (int)(kr >> 20));
+    atomicMax(&brange[1], (int)(kr >> 20));
+    atomicMin(&brange[2], (int)((kr >> 10) & 1023u));
+    atomicMax(&brange[3], (int)((kr >> 10) & 1023u));
+    __syncthreads();
+    const int bxmin = brange[0], bxmax = brange[1], bymin = brange[2], bymax = brange[3];
+    const int Ng = k.ng, lo = k.t3_lo;
+    const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3), NZ3 = 3u * NZ;
+
+    int cur = -1;  // cell column of the sources the window was built for
+    bool wok = false;
+    int fx0 = 0, nxw = 0, fy0 = 0, nyw = 0;
+    for (int64_t t0 = j0; t0 < j1; t0 += kTile) {
+        const double b = hi + tiles[t0 / kTile].w;
+        const double B = b * b;
+        const int64_t tend = min(j1, t0 + kTile);
+        for (int64_t j = t0; j < tend; ++j) {
+            const int pk = (int)(keys[j] >> 10);
+            if (pk != cur) {
+                cur = pk;
+                wok = t3_win_axis(pk >> 10, bxmin, bxmax, Ng, lo, fx0, nxw) &&
+                      t3_win_axis(pk & 1023, bymin, bymax, Ng, lo, fy0, nyw) &&
+                      (int64_t)nxw * nyw * NZ3 <= kT3WinDoubles;
+                __syncthreads();  // every wave is done with the previous window
+                if (wok) {
+                    for (int wa = 0; wa < nxw; ++wa)
+                        for (int wc = 0; wc < nyw; ++wc) {
+                            const double *src = table + ((unsigned)(fx0 + wa) * NY + (unsigned)(fy0 + wc)) * NZ3;
+                            double *dst = win + (unsigned)(wa * nyw + wc) * NZ3;
+                            for (unsigned o = tid; o < NZ3; o += kBlock) dst[o] = src[o];
+                        }
+                }
+                __syncthreads();  // the window is complete
+            }
+            const double4 pj = jrec[j];
+            double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+            // forces_cuda.cu:622-627: d - L*d/|d| when |d| > L/2
+            if (fabs(dx) > k.halfL) dx -= copysign(k.L, dx);
+            if (fabs(dy) > k.halfL) dy -= copysign(k.L, dy);
+            if (fabs(dz) > k.halfL) dz -= copysign(k.L, dz);
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            const double w = pair_w_tile(r2, B, hi, soft, j, k);
+            T3Stencil st;
+            t3_stencil(k, dx, dy, dz, st);
+            bool in = wok;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                in = in && (st.mx[q] - (unsigned)fx0 < (unsigned)nxw) && (st.my[q] - (unsigned)fy0 < (unsigned)nyw);
+            double Dx, Dy, Dz;
+            if (in)
+                t3_contract(
+                    st,
+                    [&](int i, int jj) {
+                        return win + ((st.mx[i] - (unsigned)fx0) * (unsigned)nyw + (st.my[jj] - (unsigned)fy0)) * NZ3 +
+                               (unsigned)st.iz0 * 3u;
+                    },
+                    Dx, Dy, Dz);
+            else
+                t3_contract(
+                    st, [&](int i, int jj) { return table + ((st.mx[i] * NY + st.my[jj]) * NZ + (unsigned)st.iz0) * 3u; },
+                    Dx, Dy, Dz);
+            fx = fma(pj.w, fma(w, dx, -Dx), fx);
+            fy = fma(pj.w, fma(w, dy, -Dy), fy);
+            fz = fma(pj.w, fma(w, dz, -Dz), fz);
+        }
+    }
+    const int64_t rr = row_base + tid;
+    if (rr < k.nrows) {
+        double *out = a.part + (size_t)split * 3u * (size_t)k.nrows;
+        out[rr] = fx;
+        out[k.nrows + rr] = fy;
+        out[2 * k.nrows + rr] = fz;
+    }
+}
+
+enum BodyKind : int { BK_F64 = 0, BK_MIXED_R3 = 1, BK_MIXED_Z = 2, BK_T3_WIN = 3 };
 
 template <int BK, int V, int NI, int UNR>
 __device__ __forceinline__ void run_body(const PairArgs &a, int bx, int by) {
     if (BK == BK_F64) body_f64<V, NI, UNR>(a, bx, by);
+    else if (BK == BK_T3_WIN) body_t3_win(a, bx, by);
     else if (BK == BK_MIXED_R3) body_mixed_r3<NI, UNR>(a, bx, by);
     else body_mixed_z<V, NI, UNR>(a, bx, by);
 }
@@ -1021,7 +1183,7 @@ constexpr int unroll_of(int v, int precision) {
 template <int BK, int V>
 hipError_t launch_pair_t(const PairArgs *a, const PackSeg *segs, int nseg, unsigned gx, unsigned gy,
                          hipStream_t st) {
-    constexpr int prec = (BK == BK_F64) ? STEPS_PREC_FP64 : STEPS_PREC_MIXED;
+    constexpr int prec = (BK == BK_F64 || BK == BK_T3_WIN) ? STEPS_PREC_FP64 : STEPS_PREC_MIXED;
     constexpr int NI = rows_per_lane(V, prec);
     constexpr int UNR = unroll_of(V, prec);
     if (segs)
@@ -1068,6 +1230,11 @@ bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
 // The T3 Ewald kernel also takes its rows in that order when it evaluates
 // every particle; the reduction scatters the sums back to input order.
 bool sorted_rows(const EvalPlan &p) { return pencil_order(p) && p.id_min == 0 && p.nrows == p.N; }
+// The LDS-window T3 kernel needs rows in pencils and room for a 5 x 5 window
+// of full z runs (Ng <= 119); packed launches keep the global kernel.
+bool t3_window(const EvalPlan &p) {
+    return sorted_rows(p) && !is_mixed(p) && (int64_t)(p.t3_ng + 3) * 3 * 25 <= kT3WinDoubles;
+}
 
 size_t s1r2_f32_entries(const EvalPlan &p) {
     return (is_mixed(p) && is_s1r2_lookup(p.variant)) ? s1r2_device_nodes(p.s1r2_nrho, p.s1r2_nz) : 0;
@@ -1143,6 +1310,7 @@ PairArgs make_args(const EvalPlan &p, const DeviceParticles &dp, const ScratchVi
     a.tile_ext = v.tile_ext;
     a.table = (p.variant == PV_T3_EWALD) ? p.t3 : (is_s1r2_lookup(p.variant) ? p.s1r2 : nullptr);
     a.tab32 = v.tab32;
+    a.keys = (sorted && pencil_order(p)) ? spatial_order_keys(v.order, p.N) : nullptr;
     a.part = v.part;
     a.k = make_scalars(p);
     return a;
@@ -1273,6 +1441,7 @@ hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, d
 hipError_t launch_forces_pairs(const EvalPlan &p, const DeviceParticles &dp, double *partials, hipStream_t st) {
     if (!has_pairs(p)) return hipSuccess;
     const PairArgs a = make_args(p, dp, scratch_view(p, partials));
+    if (t3_window(p)) return launch_pair_t<BK_T3_WIN, PV_T3_EWALD>(&a, nullptr, 0, row_blocks(p), (unsigned)p.splits, st);
     return launch_pair(p.variant, is_mixed(p) ? STEPS_PREC_MIXED : STEPS_PREC_FP64, &a, nullptr, 0, row_blocks(p),
                        (unsigned)p.splits, st);
 }
