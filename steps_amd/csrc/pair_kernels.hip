@@ -197,24 +197,19 @@ __device__ __forceinline__ void t3_stencil(const KScalars &k, double dx, double 
     }
 }
 
-// The staged contraction over the 16 (x, y) rows; row(i, j) returns the
-// address of the stencil's twelve z-run doubles in table row (mx[i], my[j]).
-template <class Row>
-__device__ __forceinline__ void t3_contract(const T3Stencil &s, Row row, double &Dx, double &Dy, double &Dz) {
+// The staged contraction over the 16 (x, y) rows; load(i, j, v) fills v with
+// the stencil's twelve z-run values of table row (mx[i], my[j]), node-major
+// (v[3q + c] = component c of z node q).
+template <class Load>
+__device__ __forceinline__ void t3_contract(const T3Stencil &s, Load load, double &Dx, double &Dy, double &Dz) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         double u0 = 0.0, u1 = 0.0, u2 = 0.0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const double2u *r = (const double2u *)row(i, j);
             double v[12];
-#pragma unroll
-            for (int h = 0; h < 6; ++h) {
-                const double2u a = r[h];
-                v[2 * h] = a.x;
-                v[2 * h + 1] = a.y;
-            }
+            load(i, j, v);
             double t0 = s.wz[0] * v[0], t1 = s.wz[0] * v[1], t2 = s.wz[0] * v[2];
 #pragma unroll
             for (int q = 1; q < 4; ++q) {
@@ -235,14 +230,26 @@ __device__ __forceinline__ void t3_contract(const T3Stencil &s, Row row, double 
     Dz = s2;
 }
 
+// Twelve contiguous doubles (a z run of the device layout) as six 16-byte loads.
+__device__ __forceinline__ void t3_load_aos(const double *p, double *v) {
+    const double2u *r = (const double2u *)p;
+#pragma unroll
+    for (int h = 0; h < 6; ++h) {
+        const double2u a = r[h];
+        v[2 * h] = a.x;
+        v[2 * h + 1] = a.y;
+    }
+}
+
 __device__ __forceinline__ void t3_interp(const double *__restrict__ T, const KScalars &k, double dx,
                                           double dy, double dz, double &Dx, double &Dy, double &Dz) {
     T3Stencil s;
     t3_stencil(k, dx, dy, dz, s);
     const unsigned NY = (unsigned)(k.ng - k.t3_lo), NZ = (unsigned)(k.ng + 3);
     // z node iz0-1+q sits at layout index iz0+q (one ghost below)
-    t3_contract(s, [&](int i, int j) { return T + ((s.mx[i] * NY + s.my[j]) * NZ + (unsigned)s.iz0) * 3u; },
-                Dx, Dy, Dz);
+    t3_contract(
+        s, [&](int i, int j, double *v) { t3_load_aos(T + ((s.mx[i] * NY + s.my[j]) * NZ + (unsigned)s.iz0) * 3u, v); },
+        Dx, Dy, Dz);
 }
 
 // S1R2 table in the device layout of s1r2_device_layout: one row per rho
@@ -889,7 +896,15 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
 // every read stays in bounds.  Same arithmetic on the same values as the
 // global kernel: the sums are bitwise equal.
 // ---------------------------------------------------------------------------
-constexpr int kT3WinDoubles = 9216;  // 72 KB of LDS per workgroup: two workgroups per CU
+constexpr int kT3WinDoubles = 10080;  // 78.75 KB of LDS per workgroup: two workgroups per CU
+// A window row holds the (x, y) components of its NZ z nodes as 16-byte pairs
+// (2 NZ doubles), then their z components (NZ doubles); rows start every
+// 256 bytes.  A stencil row is then four aligned 16-byte reads for (x, y) and
+// two 8-byte-aligned pair reads for z.  Measured at 64^3: node-major rows
+// (24 bytes a node, six unaligned pair reads) 21.3 G pairs/s, with bank-
+// conflict stalls on 57% of LDS cycles; three component runs, 20.9; this
+// layout, 25.5.
+__host__ __device__ constexpr unsigned t3_win_stride(unsigned nz3) { return (nz3 + 31u) & ~31u; }
 
 // Layout nodes one axis of the window needs: sources in cell column s, rows
 // in columns [bmin, bmax].  d / h lies in (s - b - 1, s - b + 1) (mod Ng), so
@@ -913,7 +928,7 @@ __device__ __forceinline__ bool t3_win_axis(int s, int bmin, int bmax, int Ng, i
 }
 
 __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, const int split) {
-    __shared__ double win[kT3WinDoubles];
+    __shared__ __align__(16) double win[kT3WinDoubles];
     __shared__ int brange[4];
     const double4 *__restrict__ irec = a.irec;
     const double *__restrict__ isoft = a.isoft;
@@ -948,7 +963,7 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
     __syncthreads();
     const int bxmin = brange[0], bxmax = brange[1], bymin = brange[2], bymax = brange[3];
     const int Ng = k.ng, lo = k.t3_lo;
-    const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3), NZ3 = 3u * NZ;
+    const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3), NZ3 = 3u * NZ, WS = t3_win_stride(NZ3);
 
     int cur = -1;  // cell column of the sources the window was built for
     bool wok = false;
@@ -963,14 +978,17 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
                 cur = pk;
                 wok = t3_win_axis(pk >> 10, bxmin, bxmax, Ng, lo, fx0, nxw) &&
                       t3_win_axis(pk & 1023, bymin, bymax, Ng, lo, fy0, nyw) &&
-                      (int64_t)nxw * nyw * NZ3 <= kT3WinDoubles;
+                      (int64_t)nxw * nyw * WS <= kT3WinDoubles;
                 __syncthreads();  // every wave is done with the previous window
                 if (wok) {
                     for (int wa = 0; wa < nxw; ++wa)
                         for (int wc = 0; wc < nyw; ++wc) {
                             const double *src = table + ((unsigned)(fx0 + wa) * NY + (unsigned)(fy0 + wc)) * NZ3;
-                            double *dst = win + (unsigned)(wa * nyw + wc) * NZ3;
-                            for (unsigned o = tid; o < NZ3; o += kBlock) dst[o] = src[o];
+                            double *dst = win + (unsigned)(wa * nyw + wc) * WS;
+                            for (unsigned o = tid; o < NZ3; o += kBlock) {
+                                const unsigned z = o / 3u, c = o - 3u * z;
+                                dst[c < 2u ? 2u * z + c : 2u * NZ + z] = src[o];
+                            }
                         }
                 }
                 __syncthreads();  // the window is complete
@@ -993,14 +1011,30 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
             if (in)
                 t3_contract(
                     st,
-                    [&](int i, int jj) {
-                        return win + ((st.mx[i] - (unsigned)fx0) * (unsigned)nyw + (st.my[jj] - (unsigned)fy0)) * NZ3 +
-                               (unsigned)st.iz0 * 3u;
+                    [&](int i, int jj, double *v) {
+                        const double *w0 =
+                            win + ((st.mx[i] - (unsigned)fx0) * (unsigned)nyw + (st.my[jj] - (unsigned)fy0)) * WS;
+                        const double2 *xy = (const double2 *)w0 + st.iz0;  // 16-byte aligned (x, y) nodes
+                        const double2u *zz = (const double2u *)(w0 + 2u * NZ + (unsigned)st.iz0);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const double2 a = xy[q];
+                            v[3 * q] = a.x;
+                            v[3 * q + 1] = a.y;
+                        }
+                        const double2u c0 = zz[0], c1 = zz[1];
+                        v[2] = c0.x;
+                        v[5] = c0.y;
+                        v[8] = c1.x;
+                        v[11] = c1.y;
                     },
                     Dx, Dy, Dz);
             else
                 t3_contract(
-                    st, [&](int i, int jj) { return table + ((st.mx[i] * NY + st.my[jj]) * NZ + (unsigned)st.iz0) * 3u; },
+                    st,
+                    [&](int i, int jj, double *v) {
+                        t3_load_aos(table + ((st.mx[i] * NY + st.my[jj]) * NZ + (unsigned)st.iz0) * 3u, v);
+                    },
                     Dx, Dy, Dz);
             fx = fma(pj.w, fma(w, dx, -Dx), fx);
             fy = fma(pj.w, fma(w, dy, -Dy), fy);
@@ -1233,7 +1267,7 @@ bool sorted_rows(const EvalPlan &p) { return pencil_order(p) && p.id_min == 0 &&
 // The LDS-window T3 kernel needs rows in pencils and room for a 5 x 5 window
 // of full z runs (Ng <= 119); packed launches keep the global kernel.
 bool t3_window(const EvalPlan &p) {
-    return sorted_rows(p) && !is_mixed(p) && (int64_t)(p.t3_ng + 3) * 3 * 25 <= kT3WinDoubles;
+    return sorted_rows(p) && !is_mixed(p) && (int64_t)t3_win_stride(3u * (unsigned)(p.t3_ng + 3)) * 25 <= kT3WinDoubles;
 }
 
 size_t s1r2_f32_entries(const EvalPlan &p) {
