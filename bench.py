@@ -47,9 +47,10 @@ PEAK_FP32_VECTOR_TFLOPS = 157.3   # MI355X published FP32 vector peak (packed)
 #   T3 tricubic 555 = pair 13 + grid map 12 + weights 54 + tensor 464 + combine 12
 FLOP_PER_PAIR = {"t3_ewald": 555, "s1r2_ngp": 39, "s1r2_cic": 61, "s1r2_tsc": 104}
 T3_TAP_BYTES_PER_PAIR = 64 * 3 * 8   # 4^3 table nodes x 3 components x FP64
-# Vector L1 data path that every table tap passes through: 64 B per clock per
-# CU (one 16-byte lane load per four lanes a clock) x 256 CUs x 2.4 GHz.
-L1_PEAK_BYTES_PER_S = 64 * 256 * 2.4e9
+# The T3 kernel reads its table taps from a window staged in LDS (128 B per
+# clock per CU x 256 CUs x 2.4 GHz); taps outside the window go through the
+# vector L1 (64 B per clock per CU).
+LDS_PEAK_BYTES_PER_S = 128 * 256 * 2.4e9
 
 WORKLOADS = {
     # name: (generator kwargs, description)
@@ -207,7 +208,7 @@ def variants_leg(reps: int) -> dict:
              "frac_fp64_peak": flop * pairs / (pair_ms * 1e-3) / 1e12 / PEAK_FP64_VECTOR_TFLOPS}
         if name == "t3_ewald":
             r["tap_bytes_per_s"] = T3_TAP_BYTES_PER_PAIR * pairs / (pair_ms * 1e-3)
-            r["tap_frac_of_l1_peak"] = r["tap_bytes_per_s"] / L1_PEAK_BYTES_PER_S
+            r["tap_frac_of_lds_peak"] = r["tap_bytes_per_s"] / LDS_PEAK_BYTES_PER_S
         else:
             # FP32-compensated path on the same state: speed and error against FP64
             sim.set_precision(steps_amd.PREC_MIXED)
