@@ -118,7 +118,8 @@ __global__ __launch_bounds__(256) void morton_kernel(const double4 *__restrict__
 }
 
 // z-pencil key of each particle (30 bits): the T3 table cell column (x, y),
-// 10 bits each (Ng <= 1023), then z on 1024 levels.  Coordinates are taken
+// 10 bits each (Ng <= 1023; y reversed in odd x columns), then z on 1024
+// levels.  Coordinates are taken
 // modulo L, so a box on [0, L) and one on [-L/2, L/2) give the same pencils
 // up to a rotation of their order.
 __device__ __forceinline__ uint32_t periodic_bin(double c, double inv_L, int nb) {
@@ -133,8 +134,10 @@ __global__ __launch_bounds__(256) void pencil_kernel(const double4 *__restrict__
     if (i >= n) return;
     const double4 p = jrec[i];
     const double inv_L = 1.0 / L;
-    key[i] = (periodic_bin(p.x, inv_L, ng) << 20) | (periodic_bin(p.y, inv_L, ng) << 10) |
-             periodic_bin(p.z, inv_L, 1024);
+    const uint32_t bx = periodic_bin(p.x, inv_L, ng), by = periodic_bin(p.y, inv_L, ng);
+    // serpentine: y runs backwards in odd x columns, so consecutive pencils
+    // stay neighbours across a column change
+    key[i] = (bx << 20) | (((bx & 1u) ? (uint32_t)(ng - 1) - by : by) << 10) | periodic_bin(p.z, inv_L, 1024);
     val[i] = (uint32_t)i;
 }
 

@@ -885,7 +885,8 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
 // workgroup's 256 rows cover a few adjacent cell columns and one run of
 // sources covers one column.  The table rows that any of their stencils can
 // reach then form a small (x, y) window of full z runs (5 x 9 rows of the
-// folded 63^3 table: 71 KB).  At each change of source column the workgroup
+// folded 63^3 table: 70 KB; serpentine pencils keep the workgroups that
+// cross a column change to 6 x 8).  At each change of source column the workgroup
 // copies that window into LDS between two barriers (the source loop is the
 // same for every wave, so both barriers are reached by all); each pair then
 // reads its 16 stencil rows from LDS, which feeds 128 B per clock against the
@@ -897,14 +898,17 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
 // global kernel: the sums are bitwise equal.
 // ---------------------------------------------------------------------------
 constexpr int kT3WinDoubles = 10080;  // 78.75 KB of LDS per workgroup: two workgroups per CU
+                                      // (a 6 x 8 window of 63^3 rows)
 // A window row holds the (x, y) components of its NZ z nodes as 16-byte pairs
-// (2 NZ doubles), then their z components (NZ doubles); rows start every
-// 256 bytes.  A stencil row is then four aligned 16-byte reads for (x, y) and
-// two 8-byte-aligned pair reads for z.  Measured at 64^3: node-major rows
-// (24 bytes a node, six unaligned pair reads) 21.3 G pairs/s, with bank-
-// conflict stalls on 57% of LDS cycles; three component runs, 20.9; this
-// layout, 25.5.
-__host__ __device__ constexpr unsigned t3_win_stride(unsigned nz3) { return (nz3 + 31u) & ~31u; }
+// (2 NZ doubles), then their z components (NZ doubles); rows start on 16-byte
+// boundaries.  A stencil row is then four aligned 16-byte reads for (x, y)
+// and two 8-byte-aligned pair reads for z.  Measured at 64^3 (row-major
+// pencils): node-major rows (24 bytes a node, six unaligned pair reads) 21.3
+// G pairs/s, with bank-conflict stalls on 57% of LDS cycles; three component
+// runs, 20.9; this layout, 26.7.  Rows padded to 256 bytes (so that a change
+// of row keeps the bank pattern) fit fewer windows: 27.9 against 30.2 with
+// serpentine pencils.
+__host__ __device__ constexpr unsigned t3_win_stride(unsigned nz3) { return (nz3 + 1u) & ~1u; }
 
 // Layout nodes one axis of the window needs: sources in cell column s, rows
 // in columns [bmin, bmax].  d / h lies in (s - b - 1, s - b + 1) (mod Ng), so
@@ -925,6 +929,12 @@ __device__ __forceinline__ bool t3_win_axis(int s, int bmin, int bmax, int Ng, i
     f0 = mn;
     nf = mxv - mn + 1;
     return true;
+}
+
+// Cell column y of a pencil key (order_kernels.hip reverses y in odd x columns).
+__device__ __forceinline__ int pencil_y(uint32_t key, int Ng) {
+    const int by = (int)((key >> 10) & 1023u);
+    return ((key >> 20) & 1u) ? Ng - 1 - by : by;
 }
 
 __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, const int split) {
@@ -956,10 +966,11 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
         brange[3] = -1;
     }
     __syncthreads();
-    atomicMin(&brange[0], (int)(kr >> 20));
-    atomicMax(&brange[1], (int)(kr >> 20));
-    atomicMin(&brange[2], (int)((kr >> 10) & 1023u));
-    atomicMax(&brange[3], (int)((kr >> 10) & 1023u));
+    const int rbx = (int)(kr >> 20), rby = pencil_y(kr, k.ng);
+    atomicMin(&brange[0], rbx);
+    atomicMax(&brange[1], rbx);
+    atomicMin(&brange[2], rby);
+    atomicMax(&brange[3], rby);
     __syncthreads();
     const int bxmin = brange[0], bxmax = brange[1], bymin = brange[2], bymax = brange[3];
     const int Ng = k.ng, lo = k.t3_lo;
@@ -973,11 +984,12 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
         const double B = b * b;
         const int64_t tend = min(j1, t0 + kTile);
         for (int64_t j = t0; j < tend; ++j) {
-            const int pk = (int)(keys[j] >> 10);
+            const uint32_t kj = keys[j];
+            const int pk = (int)(kj >> 10);
             if (pk != cur) {
                 cur = pk;
                 wok = t3_win_axis(pk >> 10, bxmin, bxmax, Ng, lo, fx0, nxw) &&
-                      t3_win_axis(pk & 1023, bymin, bymax, Ng, lo, fy0, nyw) &&
+                      t3_win_axis(pencil_y(kj, Ng), bymin, bymax, Ng, lo, fy0, nyw) &&
                       (int64_t)nxw * nyw * WS <= kT3WinDoubles;
                 __syncthreads();  // every wave is done with the previous window
                 if (wok) {
