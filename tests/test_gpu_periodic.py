@@ -147,6 +147,68 @@ def test_t3_unfolded_layout(steps, oracle, t3_tab):
     assert np.median(e) <= 1e-12 and np.percentile(e, 99) <= 1e-9
 
 
+def _antisymmetric_table(ng, seed):
+    """A random table with the reference table's symmetry, exactly: Tx odd
+    in x and even in y, Ty even in x and odd in y, Tz even in both."""
+    R = np.random.default_rng(seed).normal(size=(ng, ng, ng, 3)) * 1e-4
+    A, B, C = R[..., 0], R[..., 1], R[..., 2]
+    A = A - A[::-1]
+    A = A + A[:, ::-1]
+    B = B + B[::-1]
+    B = B - B[:, ::-1]
+    C = C + C[::-1]
+    C = C + C[:, ::-1]
+    return np.ascontiguousarray(np.stack([A, B, C], axis=-1).reshape(-1))
+
+
+@pytest.mark.parametrize("ng", [16, 15])
+def test_t3_synthetic_tables_even_and_odd_grid(steps, oracle, ng):
+    """Folded tables of even and odd Ng (the table window's node range
+    differs between the two) and an unfolded one, against the oracle."""
+    print(f"case t3 synthetic table ng={ng}", flush=True)
+    w = wl.t3_lattice(n_side=16, seed=1001)
+    T = _antisymmetric_table(ng, 17 + ng)
+    assert _foldable(T, ng)
+    Tr = np.random.default_rng(ng).normal(size=ng ** 3 * 3) * 1e-4
+    assert not _foldable(Tr, ng)
+    for name, tab in (("folded", T), ("unfolded", Tr)):
+        d, keep = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=w.L, t3_table=tab)
+        F = steps.force_eval(d, w.x, w.M, w.soft)
+        p = oracle.params(1, is_periodic=2, L=w.L, t3_table=tab, interp_order=4)
+        e = rel_err(F, oracle.forces(p, w.x, w.M, w.soft))
+        report(f"T3 ng={ng} {name} table", e)
+        assert np.median(e) <= 1e-12 and np.percentile(e, 99) <= 1e-9
+
+
+def test_t3_clustered_and_outside_box_input(steps, oracle, t3_tab):
+    """Inputs the pencil ordering and the table window treat specially: a
+    tight cluster plus scattered particles (many pencils of one or two
+    particles, so windows change at almost every source), positions shifted
+    by whole box lengths (periodic pencil bins), and N below one workgroup."""
+    print("case t3 clustered / outside-box / small N", flush=True)
+    L = L_T3
+    rng = np.random.default_rng(23)
+    n = 3000
+    x = np.concatenate([L * (0.3 + 0.02 * rng.normal(size=(2400, 3))), rng.uniform(0, L, size=(n - 2400, 3))])
+    x = np.mod(x, L)
+    M = np.full(n, 1.0)
+    h = np.full(n, 0.01)
+    for label, xx in (("clustered", x), ("shifted by whole boxes", x + L * rng.integers(-2, 3, size=(n, 3)))):
+        d, keep = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=L, t3_table=t3_tab)
+        F = steps.force_eval(d, xx, M, h)
+        p = oracle.params(1, is_periodic=2, L=L, t3_table=t3_tab, interp_order=4)
+        e = rel_err(F, oracle.forces(p, xx, M, h))
+        report(f"T3 {label}", e)
+        assert np.median(e) <= 1e-12 and np.percentile(e, 99) <= 1e-9
+    xs = x[:100]
+    d, keep = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=L, t3_table=t3_tab)
+    F = steps.force_eval(d, xs, M[:100], h[:100])
+    p = oracle.params(1, is_periodic=2, L=L, t3_table=t3_tab, interp_order=4)
+    e = rel_err(F, oracle.forces(p, xs, M[:100], h[:100]))
+    report("T3 N=100", e)
+    assert np.median(e) <= 1e-12
+
+
 def test_t3_full_size_sampled_rows(steps, oracle, t3_tab):
     """The 64^3 T3 configuration at full size, checked on 96 sampled rows."""
     print("case t3 forces n_side=64 (N=262144) sampled rows", flush=True)
