@@ -230,6 +230,44 @@ __device__ __forceinline__ void t3_contract(const T3Stencil &s, Load load, doubl
     Dz = s2;
 }
 
+// The same contraction reading the LDS window: (x, y) of the four z nodes as
+// aligned 16-byte pairs, and the z components of six nodes from the even
+// start a = iz0 & ~1 as three aligned 16-byte pairs (an 8-byte-aligned pair
+// read costs ~2.8x an aligned one in LDS, probed).  wz5 carries the four
+// weights at the nodes' places and 0 at the extra one, so each FMA chain is
+// the four-node chain (a zero-weight term adds a signed zero at most).
+template <class Load>
+__device__ __forceinline__ void t3_contract_win(const T3Stencil &s, const double wz5[5], Load load, double &Dx,
+                                                double &Dy, double &Dz) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            double vxy[8], vz[6];
+            load(i, j, vxy, vz);
+            double t0 = s.wz[0] * vxy[0], t1 = s.wz[0] * vxy[1], t2 = wz5[0] * vz[0];
+#pragma unroll
+            for (int q = 1; q < 4; ++q) {
+                t0 = fma(s.wz[q], vxy[2 * q], t0);
+                t1 = fma(s.wz[q], vxy[2 * q + 1], t1);
+            }
+#pragma unroll
+            for (int q = 1; q < 5; ++q) t2 = fma(wz5[q], vz[q], t2);
+            u0 = fma(s.wy[j], t0, u0);
+            u1 = fma(s.wys[j], t1, u1);
+            u2 = fma(s.wy[j], t2, u2);
+        }
+        s0 = fma(s.wxs[i], u0, s0);
+        s1 = fma(s.wx[i], u1, s1);
+        s2 = fma(s.wx[i], u2, s2);
+    }
+    Dx = s0;
+    Dy = s1;
+    Dz = s2;
+}
+
 // Twelve contiguous doubles (a z run of the device layout) as six 16-byte loads.
 __device__ __forceinline__ void t3_load_aos(const double *p, double *v) {
     const double2u *r = (const double2u *)p;
@@ -900,9 +938,11 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
 constexpr int kT3WinDoubles = 10080;  // 78.75 KB of LDS per workgroup: two workgroups per CU
                                       // (a 6 x 8 window of 63^3 rows)
 // A window row holds the (x, y) components of its NZ z nodes as 16-byte pairs
-// (2 NZ doubles), then their z components (NZ doubles); rows start on 16-byte
-// boundaries.  A stencil row is then four aligned 16-byte reads for (x, y)
-// and two 8-byte-aligned pair reads for z.  Measured at 64^3 (row-major
+// (2 NZ doubles), then their z components (NZ doubles and two zeros, so that
+// six nodes from an even start stay inside); rows start on 16-byte
+// boundaries.  A stencil row is then seven aligned 16-byte reads
+// (t3_contract_win).  Earlier forms: four aligned reads for (x, y) and two
+// 8-byte-aligned pair reads for z, 30.2 G pairs/s.  Measured at 64^3 (row-major
 // pencils): node-major rows (24 bytes a node, six unaligned pair reads) 21.3
 // G pairs/s, with bank-conflict stalls on 57% of LDS cycles; three component
 // runs, 20.9; this layout, 26.7.  Rows padded to 256 bytes (so that a change
@@ -974,7 +1014,7 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
     __syncthreads();
     const int bxmin = brange[0], bxmax = brange[1], bymin = brange[2], bymax = brange[3];
     const int Ng = k.ng, lo = k.t3_lo;
-    const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3), NZ3 = 3u * NZ, WS = t3_win_stride(NZ3);
+    const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3), NZ3 = 3u * NZ, WS = t3_win_stride(NZ3 + 2u);
 
     int cur = -1;  // cell column of the sources the window was built for
     bool wok = false;
@@ -997,9 +1037,13 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
                         for (int wc = 0; wc < nyw; ++wc) {
                             const double *src = table + ((unsigned)(fx0 + wa) * NY + (unsigned)(fy0 + wc)) * NZ3;
                             double *dst = win + (unsigned)(wa * nyw + wc) * WS;
-                            for (unsigned o = tid; o < NZ3; o += kBlock) {
-                                const unsigned z = o / 3u, c = o - 3u * z;
-                                dst[c < 2u ? 2u * z + c : 2u * NZ + z] = src[o];
+                            for (unsigned o = tid; o < NZ3 + 2u; o += kBlock) {
+                                if (o < NZ3) {
+                                    const unsigned z = o / 3u, c = o - 3u * z;
+                                    dst[c < 2u ? 2u * z + c : 2u * NZ + z] = src[o];
+                                } else {
+                                    dst[o] = 0.0;  // the two pad entries after the z run
+                                }
                             }
                         }
                 }
@@ -1020,27 +1064,33 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
             for (int q = 0; q < 4; ++q)
                 in = in && (st.mx[q] - (unsigned)fx0 < (unsigned)nxw) && (st.my[q] - (unsigned)fy0 < (unsigned)nyw);
             double Dx, Dy, Dz;
-            if (in)
-                t3_contract(
-                    st,
-                    [&](int i, int jj, double *v) {
+            if (in) {
+                const bool odd = (st.iz0 & 1) != 0;
+                const double wz5[5] = {odd ? 0.0 : st.wz[0], odd ? st.wz[0] : st.wz[1], odd ? st.wz[1] : st.wz[2],
+                                       odd ? st.wz[2] : st.wz[3], odd ? st.wz[3] : 0.0};
+                const unsigned za = 2u * NZ + ((unsigned)st.iz0 & ~1u);
+                t3_contract_win(
+                    st, wz5,
+                    [&](int i, int jj, double *vxy, double *vz) {
                         const double *w0 =
                             win + ((st.mx[i] - (unsigned)fx0) * (unsigned)nyw + (st.my[jj] - (unsigned)fy0)) * WS;
                         const double2 *xy = (const double2 *)w0 + st.iz0;  // 16-byte aligned (x, y) nodes
-                        const double2u *zz = (const double2u *)(w0 + 2u * NZ + (unsigned)st.iz0);
+                        const double2 *zz = (const double2 *)(w0 + za);    // 16-byte aligned z pairs
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             const double2 a = xy[q];
-                            v[3 * q] = a.x;
-                            v[3 * q + 1] = a.y;
+                            vxy[2 * q] = a.x;
+                            vxy[2 * q + 1] = a.y;
                         }
-                        const double2u c0 = zz[0], c1 = zz[1];
-                        v[2] = c0.x;
-                        v[5] = c0.y;
-                        v[8] = c1.x;
-                        v[11] = c1.y;
+#pragma unroll
+                        for (int h = 0; h < 3; ++h) {
+                            const double2 b = zz[h];
+                            vz[2 * h] = b.x;
+                            vz[2 * h + 1] = b.y;
+                        }
                     },
                     Dx, Dy, Dz);
+            }
             else
                 t3_contract(
                     st,
@@ -1279,7 +1329,7 @@ bool sorted_rows(const EvalPlan &p) { return pencil_order(p) && p.id_min == 0 &&
 // The LDS-window T3 kernel needs rows in pencils and room for a 5 x 5 window
 // of full z runs (Ng <= 119); packed launches keep the global kernel.
 bool t3_window(const EvalPlan &p) {
-    return sorted_rows(p) && !is_mixed(p) && (int64_t)t3_win_stride(3u * (unsigned)(p.t3_ng + 3)) * 25 <= kT3WinDoubles;
+    return sorted_rows(p) && !is_mixed(p) && (int64_t)t3_win_stride(3u * (unsigned)(p.t3_ng + 3) + 2u) * 25 <= kT3WinDoubles;
 }
 
 size_t s1r2_f32_entries(const EvalPlan &p) {
