@@ -245,8 +245,9 @@ def main() -> int:
     if args.copy_calibration:
         return copy_calibration()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build runs the full N x N interaction on a single "
-              "MI355X; multi-GPU sharding is design only (see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the engine runs the full N x N interaction on a single "
+              "MI355X; sharding across GPUs (row ranges per GPU, positions all-gathered each step) "
+              "is not implemented", file=sys.stderr)
         return 2
 
     if args.only == "variants":
