@@ -225,6 +225,20 @@ def variants_leg(reps: int) -> dict:
     return out
 
 
+def host_path_leg(w, desc, calls: int) -> dict:
+    """The host-buffer entry point (steps_force_eval, the reference routine's
+    boundary): host positions, masses and softening lengths in, forces out,
+    PCIe copies included; the first call allocates and uploads, then `calls`
+    calls are timed with the wall clock."""
+    steps_amd.force_eval(desc, w.x, w.M, w.soft)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        steps_amd.force_eval(desc, w.x, w.M, w.soft)
+    dt = (time.perf_counter() - t0) / calls
+    return {"N": w.N, "calls": calls, "ms_per_call": dt * 1e3,
+            "g_pairs_per_s": float(w.N) ** 2 / dt / 1e9}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,6 +249,8 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the T3 / S1xR2 variant leg")
+    ap.add_argument("--no-host-path", action="store_true",
+                    help="skip the host-buffer entry-point leg (profiling runs: its N = 16384 launches share kernel names)")
     ap.add_argument("--serving-requests", type=int, default=160,
                     help="length of the seeded Poisson request stream of the serving leg")
     ap.add_argument("--only", choices=["variants", "serving"],
@@ -293,6 +309,13 @@ def main() -> int:
                      "traffic": load_traffic(f"r3_fp64_n{N}"),
                      "flop_per_pair": FLOP_PER_PAIR_R3, "kernel_ms": pair_ms, "eval_ms": total_ms},
     }
+
+    # host-buffer entry point (PCIe included): never `value`
+    if not args.no_host_path:
+        w0 = wl.r3_compactified(**WORKLOADS["r3_16k"])
+        d0, k0 = steps_amd.make_desc(steps_amd.TOPO_R3, precision=steps_amd.PREC_FP64, cosmology=w0.cosmology,
+                                     comoving=w0.comoving, mass_in_unit_sphere=w0.mass_in_unit_sphere)
+        out["host_path"] = {"r3_headline": host_path_leg(w, d64, 1), "r3_16k": host_path_leg(w0, d0, 20)}
 
     if not args.no_mixed:
         dmx, kmx = steps_amd.make_desc(steps_amd.TOPO_R3, precision=steps_amd.PREC_MIXED, **kw)
