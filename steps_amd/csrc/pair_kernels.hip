@@ -1319,9 +1319,15 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 bool is_mixed(const EvalPlan &p) { return p.precision == STEPS_PREC_MIXED && mixed_supported(p.variant); }
 
-// The mixed kernels and the R3 FP64 kernel read spatially ordered sources
-// (Morton order); the T3 Ewald kernel reads sources in z pencils.
-bool sorted_sources(const EvalPlan &p) { return is_mixed(p) || p.variant == PV_R3 || p.variant == PV_T3_EWALD; }
+// The mixed kernels read spatially ordered sources (Morton order; their FP32
+// offsets need compact tiles), the R3 FP64 kernel from N = 65536 on (its
+// far-tile test gains 7% at N = 2^21; below, the ~16 launches of the ordering
+// cost more than the pair kernel saves), the T3 Ewald kernel sources in z
+// pencils.
+constexpr int64_t kR3SortMinN = 65536;
+bool sorted_sources(const EvalPlan &p) {
+    return is_mixed(p) || (p.variant == PV_R3 && p.N >= kR3SortMinN) || p.variant == PV_T3_EWALD;
+}
 bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
 // The T3 Ewald kernel also takes its rows in that order when it evaluates
 // every particle; the reduction scatters the sums back to input order.
@@ -1528,8 +1534,10 @@ hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, d
             hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles,
                                v.jrel, v.tile_ext, p.N);
     } else {
+        // tile extents too: the R3 kernel's far-tile test reads them (on unsorted
+        // tiles it rarely passes, and the per-pair loop runs)
         hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles, v.jrel,
-                           (double4 *)nullptr, p.N);
+                           v.tile_ext, p.N);
     }
     return hipGetLastError();
 }
