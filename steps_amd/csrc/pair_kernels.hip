@@ -519,7 +519,11 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
         fx[q] = fy[q] = fz[q] = 0.0;
     }
 
-    for (int64_t t0 = j0; t0 < j1; t0 += kTile) {
+    // loop bounds in 32 bits, relative to the tile: the loop tests stay on the
+    // SALU (a 64-bit signed compare would take two VALU instructions per trip)
+    const int ntiles = (int)((j1 - j0 + kTile - 1) / kTile);
+    for (int ti = 0; ti < ntiles; ++ti) {
+        const int64_t t0 = j0 + (int64_t)ti * kTile;
         const double4 tc = tiles[t0 / kTile];
         const double hmax = tc.w;
         double B[NI];
@@ -528,8 +532,9 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
             const double b = hi[q] + hmax;
             B[q] = b * b;
         }
-        const int64_t tend = min(j1, t0 + kTile);
-        int64_t j = t0;
+        const int cnt = (int)min((int64_t)kTile, j1 - t0);  // sources in this tile
+        const double4 *__restrict__ tj = jrec + t0;
+        int u0 = 0;
         if (V == PV_R3) {
             // R3 sources are spatially ordered: when the tile's bounding box
             // lies outside every lane's softening bound (B widened by 1e-9
@@ -539,39 +544,39 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
 #pragma unroll
             for (int q = 0; q < NI; ++q) far = far && box_dist2(xi[q], yi[q], zi[q], tc, te) > B[q] * (1.0 + 1e-9);
             if (__all(far)) {
-                for (; j + UNR <= tend; j += UNR) {
+                for (; u0 + UNR <= cnt; u0 += UNR) {
                     double4 pj[UNR];
 #pragma unroll
-                    for (int u = 0; u < UNR; ++u) pj[u] = jrec[j + u];
+                    for (int u = 0; u < UNR; ++u) pj[u] = tj[u0 + u];
 #pragma unroll
                     for (int u = 0; u < UNR; ++u)
 #pragma unroll
                         for (int q = 0; q < NI; ++q) pair_r3_far(k, pj[u], xi[q], yi[q], zi[q], fx[q], fy[q], fz[q]);
                 }
-                for (; j < tend; ++j) {
-                    const double4 pj = jrec[j];
+                for (; u0 < cnt; ++u0) {
+                    const double4 pj = tj[u0];
 #pragma unroll
                     for (int q = 0; q < NI; ++q) pair_r3_far(k, pj, xi[q], yi[q], zi[q], fx[q], fy[q], fz[q]);
                 }
                 continue;
             }
         }
-        for (; j + UNR <= tend; j += UNR) {
+        for (; u0 + UNR <= cnt; u0 += UNR) {
             double4 pj[UNR];
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) pj[u] = jrec[j + u];
+            for (int u = 0; u < UNR; ++u) pj[u] = tj[u0 + u];
 #pragma unroll
             for (int u = 0; u < UNR; ++u)
 #pragma unroll
                 for (int q = 0; q < NI; ++q)
-                    pair_f64<V>(k, table, soft, j + u, pj[u], xi[q], yi[q], zi[q], hi[q], B[q], fx[q],
+                    pair_f64<V>(k, table, soft, t0 + u0 + u, pj[u], xi[q], yi[q], zi[q], hi[q], B[q], fx[q],
                                 fy[q], fz[q]);
         }
-        for (; j < tend; ++j) {
-            const double4 pj = jrec[j];
+        for (; u0 < cnt; ++u0) {
+            const double4 pj = tj[u0];
 #pragma unroll
             for (int q = 0; q < NI; ++q)
-                pair_f64<V>(k, table, soft, j, pj, xi[q], yi[q], zi[q], hi[q], B[q], fx[q], fy[q], fz[q]);
+                pair_f64<V>(k, table, soft, t0 + u0, pj, xi[q], yi[q], zi[q], hi[q], B[q], fx[q], fy[q], fz[q]);
         }
     }
 
@@ -640,7 +645,9 @@ __device__ __forceinline__ void body_mixed_r3(const PairArgs &a, const int bx, c
         fx[q] = fy[q] = fz[q] = 0.0;
     }
 
-    for (int64_t t0 = j0; t0 < j1; t0 += kTile) {
+    const int ntiles = (int)((j1 - j0 + kTile - 1) / kTile);  // 32-bit loop bounds: SALU tests
+    for (int ti = 0; ti < ntiles; ++ti) {
+        const int64_t t0 = j0 + (int64_t)ti * kTile;
         const double4 c = tiles[t0 / kTile];
         float xr[NI], yr[NI], zr[NI], B[NI], ax[NI], ay[NI], az[NI];
 #pragma unroll
@@ -655,19 +662,21 @@ __device__ __forceinline__ void body_mixed_r3(const PairArgs &a, const int bx, c
             ax[q] = ay[q] = az[q] = 0.0f;
         }
         const int64_t tend = min(j1, t0 + kTile);
+        const int cnt = (int)(tend - t0);
+        const float4 *__restrict__ tr = jrel + t0;
         bool near = false;
-        int64_t j = t0;
-        for (; j + UNR <= tend; j += UNR) {
+        int j = 0;
+        for (; j + UNR <= cnt; j += UNR) {
             float4 pj[UNR];
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) pj[u] = jrel[j + u];
+            for (int u = 0; u < UNR; ++u) pj[u] = tr[j + u];
 #pragma unroll
             for (int u = 0; u < UNR; ++u)
 #pragma unroll
                 for (int q = 0; q < NI; ++q) mixed_pair(pj[u], xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q]);
         }
-        for (; j < tend; ++j) {
-            const float4 pj = jrel[j];
+        for (; j < cnt; ++j) {
+            const float4 pj = tr[j];
 #pragma unroll
             for (int q = 0; q < NI; ++q) mixed_pair(pj, xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q]);
         }
@@ -847,7 +856,9 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
         fx[q] = fy[q] = fz[q] = 0.0;
     }
 
-    for (int64_t t0 = j0; t0 < j1; t0 += kTile) {
+    const int ntiles = (int)((j1 - j0 + kTile - 1) / kTile);  // 32-bit loop bounds: SALU tests
+    for (int ti = 0; ti < ntiles; ++ti) {
+        const int64_t t0 = j0 + (int64_t)ti * kTile;
         const double4 c = tiles[t0 / kTile];
         float xr[NI], yr[NI], zr[NI], B[NI], ax[NI], ay[NI], az[NI];
 #pragma unroll
@@ -860,20 +871,22 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
             ax[q] = ay[q] = az[q] = 0.0f;
         }
         const int64_t tend = min(j1, t0 + kTile);
+        const int cnt = (int)(tend - t0);
+        const float4 *__restrict__ tr = jrel + t0;
         bool near = false;
-        int64_t j = t0;
-        for (; j + UNR <= tend; j += UNR) {
+        int j = 0;
+        for (; j + UNR <= cnt; j += UNR) {
             float4 pj[UNR];
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) pj[u] = jrel[j + u];
+            for (int u = 0; u < UNR; ++u) pj[u] = tr[j + u];
 #pragma unroll
             for (int u = 0; u < UNR; ++u)
 #pragma unroll
                 for (int q = 0; q < NI; ++q)
                     mixed_pair_z<V>(k, tab32, pj[u], xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q]);
         }
-        for (; j < tend; ++j) {
-            const float4 pj = jrel[j];
+        for (; j < cnt; ++j) {
+            const float4 pj = tr[j];
 #pragma unroll
             for (int q = 0; q < NI; ++q)
                 mixed_pair_z<V>(k, tab32, pj, xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q]);
@@ -1019,11 +1032,14 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
     int cur = -1;  // cell column of the sources the window was built for
     bool wok = false;
     int fx0 = 0, nxw = 0, fy0 = 0, nyw = 0;
-    for (int64_t t0 = j0; t0 < j1; t0 += kTile) {
+    const int ntiles = (int)((j1 - j0 + kTile - 1) / kTile);  // 32-bit loop bounds: SALU tests
+    for (int ti = 0; ti < ntiles; ++ti) {
+        const int64_t t0 = j0 + (int64_t)ti * kTile;
         const double b = hi + tiles[t0 / kTile].w;
         const double B = b * b;
-        const int64_t tend = min(j1, t0 + kTile);
-        for (int64_t j = t0; j < tend; ++j) {
+        const int cnt = (int)min((int64_t)kTile, j1 - t0);
+        for (int u = 0; u < cnt; ++u) {
+            const int64_t j = t0 + u;
             const uint32_t kj = keys[j];
             const int pk = (int)(kj >> 10);
             if (pk != cur) {
