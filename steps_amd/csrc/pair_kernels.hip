@@ -48,7 +48,7 @@ struct KScalars {
     double inv_gs, inv_drho, inv_dzg;
     double last_r;  // S1R2: index of the last rho node, Nrho - 1 (Nrho >= 2)
     // FP32 copies for the mixed-precision S1R2 pair
-    float Lf, halfLf, rho_maxf, inv_drhof, inv_dzgf;
+    float Lf, halfLf, rho_maxf, inv_drhof, inv_dzgf, last_rf;
 };
 
 // Everything one evaluation's pair kernel reads: rows (caller order),
@@ -532,7 +532,7 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
             const double b = hi[q] + hmax;
             B[q] = b * b;
         }
-        const int cnt = (int)min((int64_t)kTile, j1 - t0);  // sources in this tile
+        const int cnt = (ti + 1 < ntiles) ? kTile : (int)(j1 - t0);  // sources in this tile
         const double4 *__restrict__ tj = jrec + t0;
         int u0 = 0;
         if (V == PV_R3) {
@@ -747,7 +747,7 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
     const float inv_rho = (rxy2 > 0.0f) ? __builtin_amdgcn_rsqf(rxy2) : 0.0f;
     const float rho = rxy2 * inv_rho;
     const float uz = fmaf(dz + k.halfLf, k.inv_dzgf, -0.5f);
-    const float lastr = (float)k.last_r;
+    const float lastr = k.last_rf;
     float Dr, Dzv;
     static_assert(ORDER == 2 || ORDER == 4, "FP32 interpolation is CIC or TSC");
     if (ORDER == 2) {
@@ -1413,6 +1413,7 @@ KScalars make_scalars(const EvalPlan &p) {
     k.rho_maxf = (float)k.rho_max;
     k.inv_drhof = (float)k.inv_drho;
     k.inv_dzgf = (float)k.inv_dzg;
+    k.last_rf = (float)k.last_r;
     return k;
 }
 
