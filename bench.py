@@ -180,9 +180,11 @@ def serving_leg(nreq: int) -> dict:
 
 def variants_leg(reps: int) -> dict:
     """The periodic force variants on their measurement configurations: T3
-    with the tricubic Ewald table (64^3 perturbed lattice, seed 1001) and
-    S1xR2 with CIC and TSC table interpolation (N = 524288, seed 1002); FP64,
-    one warm-up, then `reps` evaluations timed with HIP events."""
+    with the tricubic Ewald table and with the nearest image only (64^3
+    perturbed lattice, seed 1001); S1xR2 with CIC and TSC table
+    interpolation, with the nearest z-image and with the image sum
+    (N = 524288, seed 1002); FP64, plus the mixed path where one exists; one
+    warm-up, then `reps` evaluations timed with HIP events."""
     out = {}
     t3 = wl.t3_lattice()
     T = steps_amd.t3_table(2, t3.L)
@@ -195,21 +197,38 @@ def variants_leg(reps: int) -> dict:
                                      comoving=1, mass_in_unit_sphere=c.mass_in_unit_sphere, s1r2_lookup=1,
                                      interp_order=order, s1r2_table=Ts, s1r2_shape=shape)
         cases.append((name, c, ds, [Ts, ks]))
+    # the other modes of the same rows: nearest image only (IS_PERIODIC = 1)
+    # and the S1xR2 image sum without a table; both use the cylindrical
+    # radial correction table with the reference's parameters
+    # (main.cc:1302 and :1282: R = Rsim, Lz = L/2 or (P + 0.6) L, 1000 nodes)
+    d1, k1 = steps_amd.make_desc(steps_amd.TOPO_T3, is_periodic=1, L=t3.L)
+    cases.append(("t3_nearest", t3, d1, [k1]))
+    rad1 = steps_amd.radial_table(c.Rsim, 0.5 * c.L, 1000, 1000)
+    dn, kn = steps_amd.make_desc(steps_amd.TOPO_S1R2, is_periodic=1, L=c.L, Rsim=c.Rsim, cosmology=1, comoving=1,
+                                 mass_in_unit_sphere=c.mass_in_unit_sphere, radial_table=rad1)
+    cases.append(("s1r2_nearest", c, dn, [rad1, kn]))
+    rad2 = steps_amd.radial_table(c.Rsim, c.L * (2 + 1 - 0.4), 1000, 1000)
+    di, ki = steps_amd.make_desc(steps_amd.TOPO_S1R2, is_periodic=2, L=c.L, Rsim=c.Rsim, cosmology=1, comoving=1,
+                                 mass_in_unit_sphere=c.mass_in_unit_sphere, s1r2_lookup=0, radial_table=rad2)
+    cases.append(("s1r2_images", c, di, [rad2, ki]))
     for name, w, desc, _keep in cases:
         sim = steps_amd.Sim(desc, w.N)
         sim.set_state(w.x, w.M, w.soft, w.v)
         wall, pair_ms, total_ms = time_path(sim, reps, 1)
         F64 = sim.get_forces()
         pairs = float(w.N) ** 2
-        flop = FLOP_PER_PAIR[name]
         r = {"N": w.N, "kernel_ms": pair_ms, "eval_ms": total_ms,
-             "g_pairs_per_s": pairs / (pair_ms * 1e-3) / 1e9,
-             "flop_per_pair": flop, "achieved_tflops": flop * pairs / (pair_ms * 1e-3) / 1e12,
-             "frac_fp64_peak": flop * pairs / (pair_ms * 1e-3) / 1e12 / PEAK_FP64_VECTOR_TFLOPS}
+             "g_pairs_per_s": pairs / (pair_ms * 1e-3) / 1e9}
+        flop = FLOP_PER_PAIR.get(name)
+        if flop is not None:
+            r.update({"flop_per_pair": flop, "achieved_tflops": flop * pairs / (pair_ms * 1e-3) / 1e12,
+                      "frac_fp64_peak": flop * pairs / (pair_ms * 1e-3) / 1e12 / PEAK_FP64_VECTOR_TFLOPS})
+        if name == "s1r2_images":
+            r["images_per_pair"] = 2 * 3 + 1  # m in [-(P+1), P+1], P = IS_PERIODIC = 2 (forces_cuda.cu:659-660)
         if name == "t3_ewald":
             r["tap_bytes_per_s"] = T3_TAP_BYTES_PER_PAIR * pairs / (pair_ms * 1e-3)
             r["tap_frac_of_lds_peak"] = r["tap_bytes_per_s"] / LDS_PEAK_BYTES_PER_S
-        else:
+        if name in ("s1r2_cic", "s1r2_tsc", "s1r2_nearest"):
             # FP32-compensated path on the same state: speed and error against FP64
             sim.set_precision(steps_amd.PREC_MIXED)
             _, pair_m, _ = time_path(sim, reps, 1)
