@@ -92,3 +92,33 @@ def test_r3_mixed_precision_error(steps, oracle):
         e = rel_err(Fmx, F64)
         report(f"R3 mixed vs fp64 N=16384 cosmology={cosmology}", e)
         assert np.percentile(e, 99) <= 1e-5
+
+
+def test_r3_full_size_2m(steps, oracle):
+    """The N = 2^21 compactified R^3 configuration (seed 1003) at full size:
+    64 sampled rows against the oracle; Newton's third law over all rows
+    (external term off, sum_i m_i F_i = 0 up to rounding); the mixed path's
+    pair-sum error against FP64; two evaluations bitwise equal."""
+    print("case r3 N=2^21 full size", flush=True)
+    w = wl.r3_compactified(N=2 ** 21, seed=1003)
+    d, keep, p = r3_descs(steps, oracle, w, cosmology=0)
+    sim = steps.Sim(d, w.N)
+    sim.set_state(w.x, w.M, w.soft)
+    sim.forces()
+    F = sim.get_forces()
+    sim.forces()
+    assert np.array_equal(F, sim.get_forces())
+    rows = np.linspace(0, w.N - 1, 64).astype(np.int32)
+    e = rel_err(F[rows], oracle.forces_rows(p, w.x, w.M, w.soft, rows))
+    report("R3 N=2^21 sampled rows", e)
+    assert np.median(e) <= 1e-12 and e.max() <= 1e-10
+    mF = w.M[:, None] * F
+    net = np.abs(mF.sum(axis=0)).max() / np.abs(mF).sum(axis=0).max()
+    print(f"[R3 N=2^21] |sum m F| / sum |m F| = {net:.3e}", flush=True)
+    assert net <= 1e-12
+    sim.set_precision(steps.PREC_MIXED)
+    sim.forces()
+    em = rel_err(sim.get_forces(), F)
+    report("R3 N=2^21 mixed pair sum vs fp64", em)
+    assert np.percentile(em, 99) <= 1e-5
+    sim.close()
