@@ -227,6 +227,11 @@ def test_t3_full_size_sampled_rows(steps, oracle, t3_tab):
     e = rel_err(F[rows], Fo)
     report("T3 N=262144 sampled", e)
     assert np.median(e) <= 1e-12
+    # no external term and an antisymmetric correction: sum_i m_i F_i = 0
+    mF = w.M[:, None] * F
+    net = np.abs(mF.sum(axis=0)).max() / np.abs(mF).sum(axis=0).max()
+    print(f"[T3 N=262144] |sum m F| / sum |m F| = {net:.3e}", flush=True)
+    assert net <= 1e-12
 
 
 @pytest.fixture(scope="module")
