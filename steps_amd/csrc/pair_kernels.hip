@@ -481,6 +481,14 @@ __device__ __forceinline__ void pair_r3_far(const KScalars &k, const double4 pj,
     fz = fma(s, dz, fz);
 }
 
+// Every displacement from x to a source of the tile (centre c, half extent
+// e on this axis) has |d| <= |x - c| + e; below L/2 (with a 1e-12 margin for
+// rounding) the nearest-image test (|d| > L/2, forces_cuda.cu:622-627 and
+// :833-836) cannot fire.
+__device__ __forceinline__ bool no_wrap(double x, double c, double e, const KScalars &k) {
+    return fabs(x - c) + e < k.halfL * (1.0 - 1e-12);
+}
+
 // Squared distance from a point to an axis-aligned box (centre c, half
 // extents e); every source of the tile is at least this far from the point.
 __device__ __forceinline__ double box_dist2(double x, double y, double z, const double4 &c, const double4 &e) {
@@ -535,14 +543,22 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
         const int cnt = (ti + 1 < ntiles) ? kTile : (int)(j1 - t0);  // sources in this tile
         const double4 *__restrict__ tj = jrec + t0;
         int u0 = 0;
-        if (V == PV_R3) {
-            // R3 sources are spatially ordered: when the tile's bounding box
-            // lies outside every lane's softening bound (B widened by 1e-9
-            // for rounding), its pairs skip the per-pair test
+        if (V == PV_R3 || V == PV_T3_NEAREST || V == PV_S1R2_NEAREST) {
+            // Spatially ordered sources: when the tile's bounding box lies
+            // outside every lane's softening bound (B widened by 1e-9 for
+            // rounding) and, for the periodic variants, within half a box of
+            // every lane on each wrapped axis (so no displacement wraps), its
+            // pairs are the plain R3 pair: same arithmetic, no per-pair tests
             const double4 te = ext[t0 / kTile];
             bool far = true;
 #pragma unroll
-            for (int q = 0; q < NI; ++q) far = far && box_dist2(xi[q], yi[q], zi[q], tc, te) > B[q] * (1.0 + 1e-9);
+            for (int q = 0; q < NI; ++q) {
+                far = far && box_dist2(xi[q], yi[q], zi[q], tc, te) > B[q] * (1.0 + 1e-9);
+                if (V == PV_T3_NEAREST)
+                    far = far && no_wrap(xi[q], tc.x, te.x, k) && no_wrap(yi[q], tc.y, te.y, k) &&
+                          no_wrap(zi[q], tc.z, te.z, k);
+                if (V == PV_S1R2_NEAREST) far = far && no_wrap(zi[q], tc.z, te.z, k);
+            }
             if (__all(far)) {
                 for (; u0 + UNR <= cnt; u0 += UNR) {
                     double4 pj[UNR];
@@ -1141,7 +1157,8 @@ __device__ __forceinline__ void run_body(const PairArgs &a, int bx, int by) {
 // The mixed R3 kernel is held to 80 VGPRs (6 waves per SIMD; no spills),
 // 1% faster than the 84 the compiler picks on its own (measured at N = 2^21).
 template <int BK, int V, int NI, int UNR>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BK == BK_MIXED_R3 ? 6 : 1)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
+    (BK == BK_MIXED_R3 || (BK == BK_F64 && (V == PV_T3_NEAREST || V == PV_S1R2_NEAREST))) ? 6 : 1)))
 void pair_kernel(const PairArgs a) {
     run_body<BK, V, NI, UNR>(a, blockIdx.x, blockIdx.y);
 }
@@ -1336,22 +1353,30 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 bool is_mixed(const EvalPlan &p) { return p.precision == STEPS_PREC_MIXED && mixed_supported(p.variant); }
 
 // The mixed kernels read spatially ordered sources (Morton order; their FP32
-// offsets need compact tiles), the R3 FP64 kernel from N = 65536 on (its
-// far-tile test gains 7% at N = 2^21; below, the ~16 launches of the ordering
-// cost more than the pair kernel saves), the T3 Ewald kernel sources in z
-// pencils.
+// offsets need compact tiles), the R3 and nearest-image FP64 kernels from
+// N = 65536 on (their far-tile test gains 7% for R3 at N = 2^21; below, the
+// ~16 launches of the ordering cost more than the pair kernel saves), the T3
+// Ewald kernel sources in z pencils.
 constexpr int64_t kR3SortMinN = 65536;
 bool sorted_sources(const EvalPlan &p) {
-    return is_mixed(p) || (p.variant == PV_R3 && p.N >= kR3SortMinN) || p.variant == PV_T3_EWALD;
+    const bool far_tiles = p.variant == PV_R3 || p.variant == PV_T3_NEAREST || p.variant == PV_S1R2_NEAREST;
+    return is_mixed(p) || (far_tiles && p.N >= kR3SortMinN) || p.variant == PV_T3_EWALD;
 }
 bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
 // The T3 Ewald kernel also takes its rows in that order when it evaluates
-// every particle; the reduction scatters the sums back to input order.
-bool sorted_rows(const EvalPlan &p) { return pencil_order(p) && p.id_min == 0 && p.nrows == p.N; }
+// every particle, and so do the nearest-image FP64 kernels (Morton order: a
+// wave's rows then sit in one small region, so whole tiles pass the far /
+// no-wrap test; in input order a wave of a lattice spans the box); the
+// reduction scatters the sums back to input order.
+bool sorted_rows(const EvalPlan &p) {
+    const bool nearest = !is_mixed(p) && (p.variant == PV_T3_NEAREST || p.variant == PV_S1R2_NEAREST);
+    return (pencil_order(p) || nearest) && sorted_sources(p) && p.id_min == 0 && p.nrows == p.N;
+}
 // The LDS-window T3 kernel needs rows in pencils and room for a 5 x 5 window
 // of full z runs (Ng <= 119); packed launches keep the global kernel.
 bool t3_window(const EvalPlan &p) {
-    return sorted_rows(p) && !is_mixed(p) && (int64_t)t3_win_stride(3u * (unsigned)(p.t3_ng + 3) + 2u) * 25 <= kT3WinDoubles;
+    return p.variant == PV_T3_EWALD && sorted_rows(p) && !is_mixed(p) &&
+           (int64_t)t3_win_stride(3u * (unsigned)(p.t3_ng + 3) + 2u) * 25 <= kT3WinDoubles;
 }
 
 size_t s1r2_f32_entries(const EvalPlan &p) {
