@@ -481,12 +481,38 @@ __device__ __forceinline__ void pair_r3_far(const KScalars &k, const double4 pj,
     fz = fma(s, dz, fz);
 }
 
-// Every displacement from x to a source of the tile (centre c, half extent
-// e on this axis) has |d| <= |x - c| + e; below L/2 (with a 1e-12 margin for
-// rounding) the nearest-image test (|d| > L/2, forces_cuda.cu:622-627 and
-// :833-836) cannot fire.
-__device__ __forceinline__ bool no_wrap(double x, double c, double e, const KScalars &k) {
-    return fabs(x - c) + e < k.halfL * (1.0 - 1e-12);
+// The nearest-image test (|d| > L/2 -> d -/+ L, forces_cuda.cu:622-627 and
+// :833-836) for every source of a tile at once: the displacements from x to
+// the tile (centre c, half extent e on this axis) lie in [c - e - x, c + e - x].
+// All inside (-L/2, L/2): shift 0; all above L/2: every one wraps to d - L,
+// shift -L; all below -L/2: shift +L (margins of 1e-12 L/2 cover rounding).
+// False when the tile straddles a wrap.  d + shift is the reference's d -/+ L
+// to the bit.
+__device__ __forceinline__ bool tile_shift(double x, double c, double e, const KScalars &k, double &s) {
+    const double m = k.halfL * 1e-12;
+    s = 0.0;
+    if (fabs(x - c) + e < k.halfL - m) return true;
+    if (c - e - x > k.halfL + m) {
+        s = -k.L;
+        return true;
+    }
+    if (c + e - x < -k.halfL - m) {
+        s = k.L;
+        return true;
+    }
+    return false;
+}
+
+// The R3 pair of a far tile with per-axis nearest-image shifts.
+__device__ __forceinline__ void pair_far_shift(const KScalars &k, const double4 pj, const double xi, const double yi,
+                                               const double zi, const double sx, const double sy, const double sz,
+                                               double &fx, double &fy, double &fz) {
+    const double dx = (pj.x - xi) + sx, dy = (pj.y - yi) + sy, dz = (pj.z - zi) + sz;
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double s = pj.w * inv_r3(r2, k.c15);
+    fx = fma(s, dx, fx);
+    fy = fma(s, dy, fy);
+    fz = fma(s, dz, fz);
 }
 
 // Squared distance from a point to an axis-aligned box (centre c, half
@@ -546,20 +572,44 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
         if (V == PV_R3 || V == PV_T3_NEAREST || V == PV_S1R2_NEAREST) {
             // Spatially ordered sources: when the tile's bounding box lies
             // outside every lane's softening bound (B widened by 1e-9 for
-            // rounding) and, for the periodic variants, within half a box of
-            // every lane on each wrapped axis (so no displacement wraps), its
-            // pairs are the plain R3 pair: same arithmetic, no per-pair tests
+            // rounding) and, for the periodic variants, every lane sees the
+            // whole tile on one side of each wrap (tile_shift), its pairs are
+            // the R3 pair, with fixed per-lane shifts where the tile wraps:
+            // same arithmetic, no per-pair tests
             const double4 te = ext[t0 / kTile];
-            bool far = true;
+            bool far = true, shifted = false;
+            double sx[NI], sy[NI], sz[NI];
 #pragma unroll
             for (int q = 0; q < NI; ++q) {
-                far = far && box_dist2(xi[q], yi[q], zi[q], tc, te) > B[q] * (1.0 + 1e-9);
-                if (V == PV_T3_NEAREST)
-                    far = far && no_wrap(xi[q], tc.x, te.x, k) && no_wrap(yi[q], tc.y, te.y, k) &&
-                          no_wrap(zi[q], tc.z, te.z, k);
-                if (V == PV_S1R2_NEAREST) far = far && no_wrap(zi[q], tc.z, te.z, k);
+                sx[q] = sy[q] = sz[q] = 0.0;
+                if (V != PV_R3) {
+                    far = far && tile_shift(zi[q], tc.z, te.z, k, sz[q]);
+                    if (V == PV_T3_NEAREST)
+                        far = far && tile_shift(xi[q], tc.x, te.x, k, sx[q]) && tile_shift(yi[q], tc.y, te.y, k, sy[q]);
+                    shifted = shifted || sx[q] != 0.0 || sy[q] != 0.0 || sz[q] != 0.0;
+                }
+                far = far && box_dist2(xi[q] - sx[q], yi[q] - sy[q], zi[q] - sz[q], tc, te) > B[q] * (1.0 + 1e-9);
             }
             if (__all(far)) {
+                if (V != PV_R3 && __any(shifted)) {
+                    for (; u0 + UNR <= cnt; u0 += UNR) {
+                        double4 pj[UNR];
+#pragma unroll
+                        for (int u = 0; u < UNR; ++u) pj[u] = tj[u0 + u];
+#pragma unroll
+                        for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                            for (int q = 0; q < NI; ++q)
+                                pair_far_shift(k, pj[u], xi[q], yi[q], zi[q], sx[q], sy[q], sz[q], fx[q], fy[q], fz[q]);
+                    }
+                    for (; u0 < cnt; ++u0) {
+                        const double4 pj = tj[u0];
+#pragma unroll
+                        for (int q = 0; q < NI; ++q)
+                            pair_far_shift(k, pj, xi[q], yi[q], zi[q], sx[q], sy[q], sz[q], fx[q], fy[q], fz[q]);
+                    }
+                    continue;
+                }
                 for (; u0 + UNR <= cnt; u0 += UNR) {
                     double4 pj[UNR];
 #pragma unroll
