@@ -485,11 +485,12 @@ __device__ __forceinline__ void pair_r3_far(const KScalars &k, const double4 pj,
 // :833-836) for every source of a tile at once: the displacements from x to
 // the tile (centre c, half extent e on this axis) lie in [c - e - x, c + e - x].
 // All inside (-L/2, L/2): shift 0; all above L/2: every one wraps to d - L,
-// shift -L; all below -L/2: shift +L (margins of 1e-12 L/2 cover rounding).
-// False when the tile straddles a wrap.  d + shift is the reference's d -/+ L
-// to the bit.
-__device__ __forceinline__ bool tile_shift(double x, double c, double e, const KScalars &k, double &s) {
-    const double m = k.halfL * 1e-12;
+// shift -L; all below -L/2: shift +L (margins of rel * L/2 cover the rounding
+// of the displacements: 1e-12 for FP64 ones, 1e-5 for FP32 ones).  False when
+// the tile straddles a wrap.  d + shift is the reference's d -/+ L to the bit.
+__device__ __forceinline__ bool tile_shift(double x, double c, double e, const KScalars &k, double &s,
+                                           double rel = 1e-12) {
+    const double m = k.halfL * rel;
     s = 0.0;
     if (fabs(x - c) + e < k.halfL - m) return true;
     if (c - e - x > k.halfL + m) {
@@ -892,6 +893,40 @@ __device__ __forceinline__ void mixed_pair_z(const KScalars &k, const float2 *__
     }
 }
 
+// FP32 pairs of a far tile (no pair within the softening bound, no straddled
+// wrap) with the nearest z-image: mixed_pair_z's arithmetic without the
+// per-pair wrap and bound tests, the wrap applied as a per-lane shift.
+template <bool SHIFT, int NI, int UNR>
+__device__ __forceinline__ void mixed_far_tile_z(const float4 *__restrict__ tr, const int cnt, const float *xr,
+                                                 const float *yr, const float *zr, const float *sz, float *ax,
+                                                 float *ay, float *az) {
+    auto pair = [&](const float4 pj, const int q) {
+        const float dx = pj.x - xr[q], dy = pj.y - yr[q];
+        const float dz = SHIFT ? (pj.z - zr[q]) + sz[q] : pj.z - zr[q];
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float y = __builtin_amdgcn_rsqf(r2);
+        const float s = ((y * y) * y) * pj.w;
+        ax[q] = fmaf(s, dx, ax[q]);
+        ay[q] = fmaf(s, dy, ay[q]);
+        az[q] = fmaf(s, dz, az[q]);
+    };
+    int j = 0;
+    for (; j + UNR <= cnt; j += UNR) {
+        float4 pj[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) pj[u] = tr[j + u];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+#pragma unroll
+            for (int q = 0; q < NI; ++q) pair(pj[u], q);
+    }
+    for (; j < cnt; ++j) {
+        const float4 pj = tr[j];
+#pragma unroll
+        for (int q = 0; q < NI; ++q) pair(pj, q);
+    }
+}
+
 template <int V, int NI, int UNR>
 __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, const int split) {
     const double4 *__restrict__ irec = a.irec;
@@ -902,6 +937,7 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
     const double4 *__restrict__ tiles = a.tiles;
     const double *__restrict__ table = a.table;
     const float2 *__restrict__ tab32 = a.tab32;
+    const double4 *__restrict__ ext = a.tile_ext;
     double *__restrict__ part = a.part;
     KScalars k = a.k;
     k.c15 = in_vgpr(a.k.c15);
@@ -939,6 +975,33 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
         const int64_t tend = min(j1, t0 + kTile);
         const int cnt = (int)(tend - t0);
         const float4 *__restrict__ tr = jrel + t0;
+        if (V == PV_S1R2_NEAREST) {
+            // Far tile (decided in FP64, with margins far above the FP32
+            // rounding of the displacements): every lane sees the whole tile
+            // on one side of the z wrap and outside 1.001 x its bound
+            const double4 te = ext[t0 / kTile];
+            bool far = true, shifted = false;
+            float sz[NI];
+#pragma unroll
+            for (int q = 0; q < NI; ++q) {
+                double sh = 0.0;
+                far = far && tile_shift(zi[q], c.z, te.z, k, sh, 1e-5);
+                far = far && box_dist2(xi[q], yi[q], zi[q] - sh, c, te) > (double)B[q] * (1.0 + 1e-3);
+                sz[q] = (float)sh;  // 0, -L or +L: (float)(-L) is -Lf
+                shifted = shifted || sh != 0.0;
+            }
+            if (__all(far)) {
+                if (__any(shifted)) mixed_far_tile_z<true, NI, UNR>(tr, cnt, xr, yr, zr, sz, ax, ay, az);
+                else mixed_far_tile_z<false, NI, UNR>(tr, cnt, xr, yr, zr, sz, ax, ay, az);
+#pragma unroll
+                for (int q = 0; q < NI; ++q) {
+                    fx[q] += (double)ax[q];
+                    fy[q] += (double)ay[q];
+                    fz[q] += (double)az[q];
+                }
+                continue;
+            }
+        }
         bool near = false;
         int j = 0;
         for (; j + UNR <= cnt; j += UNR) {
@@ -1206,9 +1269,13 @@ __device__ __forceinline__ void run_body(const PairArgs &a, int bx, int by) {
 
 // The mixed R3 kernel is held to 80 VGPRs (6 waves per SIMD; no spills),
 // 1% faster than the 84 the compiler picks on its own (measured at N = 2^21).
+// The mixed nearest z-image kernel is held to 128 (4 waves; 12 bytes of
+// spills): 6% faster than its own 132 at 3 waves (N = 524288).
 template <int BK, int V, int NI, int UNR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
-    (BK == BK_MIXED_R3 || (BK == BK_F64 && (V == PV_T3_NEAREST || V == PV_S1R2_NEAREST))) ? 6 : 1)))
+    (BK == BK_MIXED_R3 || (BK == BK_F64 && (V == PV_T3_NEAREST || V == PV_S1R2_NEAREST)))
+        ? 6
+        : ((BK == BK_MIXED_Z && V == PV_S1R2_NEAREST) ? 4 : 1))))
 void pair_kernel(const PairArgs a) {
     run_body<BK, V, NI, UNR>(a, blockIdx.x, blockIdx.y);
 }
@@ -1419,7 +1486,7 @@ bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
 // no-wrap test; in input order a wave of a lattice spans the box); the
 // reduction scatters the sums back to input order.
 bool sorted_rows(const EvalPlan &p) {
-    const bool nearest = !is_mixed(p) && (p.variant == PV_T3_NEAREST || p.variant == PV_S1R2_NEAREST);
+    const bool nearest = (!is_mixed(p) && p.variant == PV_T3_NEAREST) || p.variant == PV_S1R2_NEAREST;
     return (pencil_order(p) || nearest) && sorted_sources(p) && p.id_min == 0 && p.nrows == p.N;
 }
 // The LDS-window T3 kernel needs rows in pencils and room for a 5 x 5 window

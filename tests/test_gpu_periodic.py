@@ -209,12 +209,13 @@ def test_t3_clustered_and_outside_box_input(steps, oracle, t3_tab):
     assert np.median(e) <= 1e-12
 
 
-@pytest.mark.parametrize("mode", ["t3", "s1r2"])
+@pytest.mark.parametrize("mode", ["t3", "s1r2", "s1r2_mixed"])
 def test_nearest_image_far_tiles(steps, oracle, mode):
-    """Nearest-image modes (IS_PERIODIC = 1) at N >= 65536, where the sources
-    are Morton-ordered and tiles that no lane can wrap or soften take the
-    plain R3 loop: sampled rows against the oracle, and a row subset (other
-    waves, so other far-tile decisions) bitwise equal to the full evaluation."""
+    """Nearest-image modes (IS_PERIODIC = 1) at N >= 65536, where rows and
+    sources are Morton-ordered and tiles that no lane can soften and that no
+    lane sees across a wrap take the R3 loop with per-lane shifts: sampled rows
+    against the oracle, and a row subset (input order, so other waves and
+    other far-tile decisions) bitwise equal to the full evaluation."""
     print(f"case nearest image far tiles {mode}", flush=True)
     if mode == "t3":
         w = wl.t3_lattice(n_side=48, seed=1001, is_periodic=1)
@@ -226,14 +227,23 @@ def test_nearest_image_far_tiles(steps, oracle, mode):
         radial = oracle.radial_table(w.Rsim, 0.5 * w.L, 1000, 1000)    # main.cc:1302
         kw = dict(is_periodic=1, L=w.L, Rsim=w.Rsim, cosmology=1, comoving=1,
                   mass_in_unit_sphere=w.mass_in_unit_sphere, radial_table=radial)
-        d, keep = steps.make_desc(steps.TOPO_S1R2, **kw)
+        prec = steps.PREC_MIXED if mode == "s1r2_mixed" else steps.PREC_FP64
+        d, keep = steps.make_desc(steps.TOPO_S1R2, precision=prec, **kw)
         p = oracle.params(2, cut_inclusive=0, **kw)
     assert w.N >= 65536
     F = steps.force_eval(d, w.x, w.M, w.soft)
     rows = np.linspace(0, w.N - 1, 64).astype(np.int32)
     e = rel_err(F[rows], oracle.forces_rows(p, w.x, w.M, w.soft, rows))
     report(f"{mode} nearest N={w.N} sampled", e)
-    assert np.median(e) <= 1e-12 and e.max() <= 1e-10
+    if mode == "s1r2_mixed":
+        # FP32 pairs against the FP64 evaluation of every row (bar of the
+        # mixed path with the comoving term: p99 3e-5)
+        d64, keep64 = steps.make_desc(steps.TOPO_S1R2, **kw)
+        ea = rel_err(F, steps.force_eval(d64, w.x, w.M, w.soft))
+        report(f"{mode} nearest N={w.N} all rows vs fp64", ea)
+        assert np.median(e) <= 2e-6 and np.percentile(ea, 99) <= 3e-5 and np.median(ea) <= 2e-6
+    else:
+        assert np.median(e) <= 1e-12 and e.max() <= 1e-10
     Fs = steps.force_eval(d, w.x, w.M, w.soft, id_min=37, id_max=w.N - 1)
     assert np.array_equal(Fs, F[37:])
 
