@@ -228,7 +228,7 @@ def variants_leg(reps: int) -> dict:
         if name == "t3_ewald":
             r["tap_bytes_per_s"] = T3_TAP_BYTES_PER_PAIR * pairs / (pair_ms * 1e-3)
             r["tap_frac_of_lds_peak"] = r["tap_bytes_per_s"] / LDS_PEAK_BYTES_PER_S
-        if name in ("s1r2_cic", "s1r2_tsc", "s1r2_nearest"):
+        if name in ("t3_ewald", "t3_nearest", "s1r2_cic", "s1r2_tsc", "s1r2_nearest"):
             # FP32-compensated path on the same state: speed and error against FP64
             sim.set_precision(steps_amd.PREC_MIXED)
             _, pair_m, _ = time_path(sim, reps, 1)

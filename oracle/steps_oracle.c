@@ -777,6 +777,68 @@ int oracle_forces_rows(const oracle_params *p, int N, const double *x, const dou
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
+/* The reference's single-precision GPU build (USE_SINGLE_PRECISION,
+ * global_variables.h:26-32, set by Template-LinuxICC-Makefile:24) for R3:
+ * ForceKernel (forces_cuda.cu:522-563) with REAL = float.  Positions, masses
+ * and softening lengths are stored as float, and every operation, the
+ * softening law (forces_cuda.cu:466-519) and the accumulation over
+ * j = 0..N-1 run in float with no compensation.  A statistical stand-in for
+ * that build, not a bitwise one: the fused multiply-adds nvcc contracts by
+ * default are modelled as fmaf in r^2 and in the accumulation; sqrt and the
+ * divisions are IEEE float. */
+static float softening_f32(float r, float beta)
+{
+    const float r3 = (r * r) * r;
+    const float inv_r3 = 1.0f / r3;
+    if (r >= beta) return inv_r3;
+    const float b2 = beta * beta, b3 = b2 * beta, b4 = b2 * b2, b5 = b4 * beta, b6 = b3 * b3;
+    if (r > 0.5f * beta)
+        return (-32.0f / (3.0f * b6)) * r3 + (38.4f / b5) * (r * r) + (-48.0f / b4) * r +
+               64.0f / (3.0f * b3) + (float)(-1.0 / 15.0) * inv_r3;
+    return (32.0f / b6) * r3 + (-38.4f / b5) * (r * r) + (float)(32.0 / 3.0) / b3;
+}
+
+int oracle_forces_r3_fp32_rows(const oracle_params *p, int N, const double *x, const double *M,
+                               const double *h, double *F, const int *rows, int nrows)
+{
+    int rc = check_args(p, N, x, M, h);
+    if (rc) return rc;
+    if (p->topology != ORC_R3) return -4;
+    for (int k = 0; k < nrows; k++)
+        if (rows[k] < 0 || rows[k] >= N) return -3;
+    const float mius = (float)p->mass_in_unit_sphere;
+    const float DE = (float)(p->H0 * p->H0 * p->Omega_lambda);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int k = 0; k < nrows; k++) {
+        const int i = rows[k];
+        const float xi = (float)x[3 * i], yi = (float)x[3 * i + 1], zi = (float)x[3 * i + 2];
+        const float hi = (float)h[i];
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        for (int j = 0; j < N; j++) {
+            const float dx = (float)x[3 * j] - xi, dy = (float)x[3 * j + 1] - yi,
+                        dz = (float)x[3 * j + 2] - zi;
+            const float r = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+            const float w = (float)M[j] * softening_f32(r, hi + (float)h[j]);
+            fx = fmaf(w, dx, fx);
+            fy = fmaf(w, dy, fy);
+            fz = fmaf(w, dz, fz);
+        }
+        if (p->cosmology == 1 && p->comoving == 1) {
+            fx = fmaf(mius, xi, fx);
+            fy = fmaf(mius, yi, fy);
+            fz = fmaf(mius, zi, fz);
+        } else if (p->cosmology == 1 && p->comoving == 0) {
+            fx = fmaf(DE, xi, fx);
+            fy = fmaf(DE, yi, fy);
+            fz = fmaf(DE, zi, fz);
+        }
+        F[3 * (size_t)k] = fx;
+        F[3 * (size_t)k + 1] = fy;
+        F[3 * (size_t)k + 2] = fz;
+    }
+    return 0;
+}
+
 /* long-double brute force of the R3 sum (external term included). */
 void oracle_forces_r3_longdouble(const oracle_params *p, int N, const double *x,
                                  const double *M, const double *h, long double *F)

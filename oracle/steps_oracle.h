@@ -23,6 +23,10 @@
  *   radial table           independent adaptive quadrature of the integrand,
  *                          long-cylinder limit
  *   softening setup, kdk   closed forms, two-body orbit
+ *   r3 fp32 stand-in      two-body case exact to float rounding, error
+ *                          against the FP64 oracle within float bounds and
+ *                          growing with N (tests/test_oracle_r3.py); it is a
+ *                          comparison point, not a parity oracle
  *   fast_erfcx branch of the S1R2 table (ewald_space.cc:539-561): parity
  *                          unpinned beyond the direct-image-sum bound, because
  *                          the reference's own A&S 7.1.26 fit is only
@@ -73,6 +77,13 @@ int oracle_forces(const oracle_params *p, int N, const double *x, const double *
 /* same over an explicit row list (F row k belongs to rows[k]) */
 int oracle_forces_rows(const oracle_params *p, int N, const double *x, const double *M,
                        const double *h, double *F, const int *rows, int nrows);
+
+/* the reference's single-precision (REAL = float) R3 kernel over an explicit
+ * row list: all-float arithmetic and accumulation, a statistical stand-in
+ * for that build (the comparison point of the mixed-precision path).
+ * Returns -4 for a topology other than R3. */
+int oracle_forces_r3_fp32_rows(const oracle_params *p, int N, const double *x, const double *M,
+                               const double *h, double *F, const int *rows, int nrows);
 
 /* reference-faithful OpenMP baseline (schedule(dynamic,chunk), pow, three
  * omp atomic adds per pair, -O3 -ffast-math); used only for timing. */

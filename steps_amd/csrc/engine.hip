@@ -213,7 +213,7 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
         return fail(STEPS_ERR_ARG, "unknown topology");
     }
     if (p.precision == STEPS_PREC_MIXED && !mixed_supported(p.variant))
-        return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3 and S1R2 (nearest z-image, CIC or TSC correction)");
+        return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3, T3 and S1R2 (nearest z-image, CIC or TSC correction)");
 
     // split the source range so that (row blocks) x (splits) fills the chip
     const int ni = rows_per_lane(p.variant, p.precision);
@@ -545,7 +545,7 @@ int steps_sim_set_precision(steps_sim *s, int32_t precision) {
     if (!s) return fail(STEPS_ERR_ARG, "null sim");
     if (precision != STEPS_PREC_FP64 && precision != STEPS_PREC_MIXED) return fail(STEPS_ERR_ARG, "unknown precision");
     if (precision == STEPS_PREC_MIXED && !desc_mixed_capable(s->desc))
-        return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3 and S1R2 (nearest z-image, CIC or TSC correction)");
+        return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3, T3 and S1R2 (nearest z-image, CIC or TSC correction)");
     s->desc.precision = precision;
     return STEPS_OK;
 }

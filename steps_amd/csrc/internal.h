@@ -31,12 +31,14 @@ enum PairVariant : int {
 };
 
 constexpr bool is_s1r2_lookup(int v) { return v == PV_S1R2_NGP || v == PV_S1R2_CIC || v == PV_S1R2_TSC; }
-// Variants with a mixed-precision kernel: R3 and S1R2 with the nearest z-image
-// (plain, or with the CIC / TSC interpolated Ewald correction).  NGP is left
-// out: its correction is piecewise constant, so FP32 rounding of (rho, dz)
-// moves pairs near cell boundaries to the neighbouring node.
+// Variants with a mixed-precision kernel: R3, T3 (nearest image, tricubic
+// Ewald correction) and S1R2 with the nearest z-image (plain, or with the CIC
+// / TSC interpolated Ewald correction).  NGP is left out: its correction is
+// piecewise constant, so FP32 rounding of (rho, dz) moves pairs near cell
+// boundaries to the neighbouring node.  So is the S1R2 image sum.
 constexpr bool mixed_supported(int v) {
-    return v == PV_R3 || v == PV_S1R2_NEAREST || v == PV_S1R2_CIC || v == PV_S1R2_TSC;
+    return v == PV_R3 || v == PV_T3_NEAREST || v == PV_T3_EWALD || v == PV_S1R2_NEAREST || v == PV_S1R2_CIC ||
+           v == PV_S1R2_TSC;
 }
 // i-particles per lane of each kernel instantiation: table variants gather
 // 8..192 table words per pair and keep one row per lane.

@@ -49,6 +49,7 @@ struct KScalars {
     double last_r;  // S1R2: index of the last rho node, Nrho - 1 (Nrho >= 2)
     // FP32 copies for the mixed-precision S1R2 pair
     float Lf, halfLf, rho_maxf, inv_drhof, inv_dzgf, last_rf;
+    float inv_gsf;  // FP32 copy of inv_gs for the mixed-precision T3 pair
 };
 
 // Everything one evaluation's pair kernel reads: rows (caller order),
@@ -63,7 +64,8 @@ struct PairArgs {
     const double4 *tiles;
     const double4 *tile_ext;  // half extents of the tiles' bounding boxes (sorted sources)
     const double *table;
-    const float2 *tab32;
+    const float2 *tab32;      // FP32 S1R2 table (mixed)
+    const float4 *t3tab32;    // FP32 T3 table, (Dx, Dy, Dz, 0) per node (mixed)
     const uint32_t *keys;     // pencil keys of the sorted particles (T3 Ewald), else null
     double *part;
     KScalars k;
@@ -504,6 +506,13 @@ __device__ __forceinline__ bool tile_shift(double x, double c, double e, const K
     return false;
 }
 
+// x moved by a period towards c when it lies more than half a period away
+// (positions inside the box need at most one step).
+__device__ __forceinline__ double near_image(double x, double c, const KScalars &k) {
+    const double d = x - c;
+    return (d > k.halfL) ? x - k.L : ((d < -k.halfL) ? x + k.L : x);
+}
+
 // The R3 pair of a far tile with per-axis nearest-image shifts.
 __device__ __forceinline__ void pair_far_shift(const KScalars &k, const double4 pj, const double xi, const double yi,
                                                const double zi, const double sx, const double sy, const double sz,
@@ -856,12 +865,20 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
     Dz = Dzv;
 }
 
-// FP32 displacement with the nearest z-image (forces_cuda.cu:833-836) and r^2.
+// FP32 displacement with the nearest z-image (forces_cuda.cu:833-836) and r^2;
+// T3 nearest image: the nearest image on every axis (forces_cuda.cu:622-627).
+template <int V>
 __device__ __forceinline__ float mixed_r2_z(const KScalars &k, const float4 pj, float xr, float yr, float zr,
                                             float &dx, float &dy, float &dz) {
     dx = pj.x - xr;
     dy = pj.y - yr;
     dz = pj.z - zr;
+    if (V == PV_T3_NEAREST) {
+        if (dx > k.halfLf) dx -= k.Lf;
+        else if (dx < -k.halfLf) dx += k.Lf;
+        if (dy > k.halfLf) dy -= k.Lf;
+        else if (dy < -k.halfLf) dy += k.Lf;
+    }
     if (dz > k.halfLf) dz -= k.Lf;
     else if (dz < -k.halfLf) dz += k.Lf;
     return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -872,13 +889,13 @@ __device__ __forceinline__ void mixed_pair_z(const KScalars &k, const float2 *__
                                              float xr, float yr, float zr, float B, bool &near, float &ax,
                                              float &ay, float &az) {
     float dx, dy, dz;
-    const float r2 = mixed_r2_z(k, pj, xr, yr, zr, dx, dy, dz);
+    const float r2 = mixed_r2_z<V>(k, pj, xr, yr, zr, dx, dy, dz);
     const bool nr = r2 < B;  // left to the FP64 redo (r2 may be 0 here)
     near = near || nr;
     const float y = __builtin_amdgcn_rsqf(r2);
     const float y3 = nr ? 0.0f : (y * y) * y;
     const float m = nr ? 0.0f : pj.w;
-    if (V == PV_S1R2_NEAREST) {
+    if (V == PV_S1R2_NEAREST || V == PV_T3_NEAREST) {
         const float s = y3 * m;
         ax = fmaf(s, dx, ax);
         ay = fmaf(s, dy, ay);
@@ -894,14 +911,17 @@ __device__ __forceinline__ void mixed_pair_z(const KScalars &k, const float2 *__
 }
 
 // FP32 pairs of a far tile (no pair within the softening bound, no straddled
-// wrap) with the nearest z-image: mixed_pair_z's arithmetic without the
-// per-pair wrap and bound tests, the wrap applied as a per-lane shift.
-template <bool SHIFT, int NI, int UNR>
+// wrap) with the nearest image (z only for S1R2, every axis for T3):
+// mixed_pair_z's arithmetic without the per-pair wrap and bound tests, the
+// wrap applied as a per-lane shift.
+template <bool SHIFT, bool XY, int NI, int UNR>
 __device__ __forceinline__ void mixed_far_tile_z(const float4 *__restrict__ tr, const int cnt, const float *xr,
-                                                 const float *yr, const float *zr, const float *sz, float *ax,
-                                                 float *ay, float *az) {
+                                                 const float *yr, const float *zr, const float *sx,
+                                                 const float *sy, const float *sz, float *ax, float *ay,
+                                                 float *az) {
     auto pair = [&](const float4 pj, const int q) {
-        const float dx = pj.x - xr[q], dy = pj.y - yr[q];
+        const float dx = (SHIFT && XY) ? (pj.x - xr[q]) + sx[q] : pj.x - xr[q];
+        const float dy = (SHIFT && XY) ? (pj.y - yr[q]) + sy[q] : pj.y - yr[q];
         const float dz = SHIFT ? (pj.z - zr[q]) + sz[q] : pj.z - zr[q];
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float y = __builtin_amdgcn_rsqf(r2);
@@ -962,12 +982,24 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
     for (int ti = 0; ti < ntiles; ++ti) {
         const int64_t t0 = j0 + (int64_t)ti * kTile;
         const double4 c = tiles[t0 / kTile];
+        // each row's image nearest the tile centre on the periodic axes, so
+        // that a close pair across the box edge rounds like any close pair
+        // (the per-pair wrap below then fires only for distant pairs)
+        double xw[NI], yw[NI], zw[NI];
         float xr[NI], yr[NI], zr[NI], B[NI], ax[NI], ay[NI], az[NI];
 #pragma unroll
         for (int q = 0; q < NI; ++q) {
-            xr[q] = (float)(xi[q] - c.x);
-            yr[q] = (float)(yi[q] - c.y);
-            zr[q] = (float)(zi[q] - c.z);
+            xw[q] = xi[q];
+            yw[q] = yi[q];
+            zw[q] = zi[q];
+            if (V == PV_T3_NEAREST) {
+                xw[q] = near_image(xw[q], c.x, k);
+                yw[q] = near_image(yw[q], c.y, k);
+            }
+            zw[q] = near_image(zw[q], c.z, k);
+            xr[q] = (float)(xw[q] - c.x);
+            yr[q] = (float)(yw[q] - c.y);
+            zr[q] = (float)(zw[q] - c.z);
             const float b = (float)((hi[q] + c.w) * (1.0 + 1e-5));
             B[q] = b * b;
             ax[q] = ay[q] = az[q] = 0.0f;
@@ -975,24 +1007,29 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
         const int64_t tend = min(j1, t0 + kTile);
         const int cnt = (int)(tend - t0);
         const float4 *__restrict__ tr = jrel + t0;
-        if (V == PV_S1R2_NEAREST) {
+        if (V == PV_S1R2_NEAREST || V == PV_T3_NEAREST) {
             // Far tile (decided in FP64, with margins far above the FP32
             // rounding of the displacements): every lane sees the whole tile
-            // on one side of the z wrap and outside 1.001 x its bound
+            // on one side of each wrap and outside 1.001 x its bound
+            constexpr bool XY = V == PV_T3_NEAREST;
             const double4 te = ext[t0 / kTile];
             bool far = true, shifted = false;
-            float sz[NI];
+            float sx[NI], sy[NI], sz[NI];
 #pragma unroll
             for (int q = 0; q < NI; ++q) {
-                double sh = 0.0;
-                far = far && tile_shift(zi[q], c.z, te.z, k, sh, 1e-5);
-                far = far && box_dist2(xi[q], yi[q], zi[q] - sh, c, te) > (double)B[q] * (1.0 + 1e-3);
-                sz[q] = (float)sh;  // 0, -L or +L: (float)(-L) is -Lf
-                shifted = shifted || sh != 0.0;
+                double hx = 0.0, hy = 0.0, hz = 0.0;
+                far = far && tile_shift(zw[q], c.z, te.z, k, hz, 1e-5);
+                if (XY)
+                    far = far && tile_shift(xw[q], c.x, te.x, k, hx, 1e-5) && tile_shift(yw[q], c.y, te.y, k, hy, 1e-5);
+                far = far && box_dist2(xw[q] - hx, yw[q] - hy, zw[q] - hz, c, te) > (double)B[q] * (1.0 + 1e-3);
+                sx[q] = (float)hx;  // 0, -L or +L: (float)(-L) is -Lf
+                sy[q] = (float)hy;
+                sz[q] = (float)hz;
+                shifted = shifted || hx != 0.0 || hy != 0.0 || hz != 0.0;
             }
             if (__all(far)) {
-                if (__any(shifted)) mixed_far_tile_z<true, NI, UNR>(tr, cnt, xr, yr, zr, sz, ax, ay, az);
-                else mixed_far_tile_z<false, NI, UNR>(tr, cnt, xr, yr, zr, sz, ax, ay, az);
+                if (__any(shifted)) mixed_far_tile_z<true, XY, NI, UNR>(tr, cnt, xr, yr, zr, sx, sy, sz, ax, ay, az);
+                else mixed_far_tile_z<false, XY, NI, UNR>(tr, cnt, xr, yr, zr, sx, sy, sz, ax, ay, az);
 #pragma unroll
                 for (int q = 0; q < NI; ++q) {
                     fx[q] += (double)ax[q];
@@ -1026,7 +1063,7 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
 #pragma unroll
                 for (int q = 0; q < NI; ++q) {
                     float dx, dy, dz;
-                    if (mixed_r2_z(k, pr, xr[q], yr[q], zr[q], dx, dy, dz) < B[q]) {
+                    if (mixed_r2_z<V>(k, pr, xr[q], yr[q], zr[q], dx, dy, dz) < B[q]) {
                         const double b = hi[q] + c.w;
                         pair_f64<V>(k, table, soft, jj, jrec[jj], xi[q], yi[q], zi[q], hi[q], b * b, fx[q],
                                     fy[q], fz[q]);
@@ -1257,25 +1294,279 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
     }
 }
 
-enum BodyKind : int { BK_F64 = 0, BK_MIXED_R3 = 1, BK_MIXED_Z = 2, BK_T3_WIN = 3 };
+// ---------------------------------------------------------------------------
+// Mixed-precision T3 Ewald pair kernel.  Each displacement is formed and
+// wrapped in FP64 (forces_cuda.cu:620-627) and rounded to FP32 once: the
+// sources are in z pencils, whose 64-source tiles span the box in z, so
+// tile-recentred FP32 positions would carry errors of ulp(L/2) on close
+// pairs.  r^2, r^-3 and the tricubic correction are FP32, the correction read
+// from an FP32 copy of the device table with one 16-byte (Dx, Dy, Dz, 0) word
+// per node, so a stencil row is four aligned 16-byte reads.  Pairs within
+// 1.00001 x the softening bound are zeroed in FP32 and redone in FP64 (FP64
+// table); FP32 sums run over one 64-source tile and are added to FP64
+// accumulators.  WIN: rows and sources in pencils and the table window of
+// body_t3_win staged in LDS (16 bytes a node: at most 3264 nodes, 51 KB,
+// three workgroups per CU); a lane whose stencil leaves the window reads the
+// global FP32 table, so both forms give the same sums bitwise.  Like the FP64
+// kernel it is bound by LDS bank conflicts, not by LDS bytes, and runs at
+// about the FP64 kernel's rate (35 against 34 G pairs/s at 64^3).
+// ---------------------------------------------------------------------------
+constexpr int kT3WinNodes32 = 3264;
+// Window rows start on 256-byte boundaries (16 nodes), so every row has the
+// same bank alignment: 35.3 against 33.1 G pairs/s with unpadded rows at 64^3
+// (where a 6 x 8 window then no longer fits and its workgroup reads the global
+// table), and 30.3 with 78 KB windows at two workgroups per CU.
+constexpr unsigned kT3WinPad32 = 16u;
+__host__ __device__ constexpr unsigned t3_win_stride32(unsigned nz) { return (nz + kT3WinPad32 - 1u) / kT3WinPad32 * kT3WinPad32; }
+
+struct T3Stencil32 {
+    float wx[4], wy[4], wz[4], wxs[4], wys[4];
+    unsigned mx[4], my[4];
+    int iz0;
+};
+
+// t3_axis in FP32 (forces_cuda.cu:93-111).
+__device__ __forceinline__ int t3_axis_f32(float d, const KScalars &k, float w[4]) {
+    const float u = (d + k.halfLf) * k.inv_gsf - 0.5f;
+    const float uf = floorf(u);
+    const float t = u - uf;
+    const float t2 = t * t, t3 = t * t2;
+    w[0] = -0.5f * t3 + t2 - 0.5f * t;
+    w[1] = 1.5f * t3 - 2.5f * t2 + 1.0f;
+    w[2] = -1.5f * t3 + 2.0f * t2 + 0.5f * t;
+    w[3] = 0.5f * t3 - 0.5f * t2;
+    return wrap_index((int)uf, k.ng);
+}
+
+__device__ __forceinline__ void t3_stencil_f32(const KScalars &k, float dx, float dy, float dz, T3Stencil32 &s) {
+    const int Ng = k.ng, lo = k.t3_lo;
+    const int ix0 = t3_axis_f32(dx, k, s.wx);
+    const int iy0 = t3_axis_f32(dy, k, s.wy);
+    s.iz0 = t3_axis_f32(dz, k, s.wz);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int jx = wrap_once(ix0 - 1 + q, Ng);
+        const int jy = wrap_once(iy0 - 1 + q, Ng);
+        s.mx[q] = (unsigned)((jx >= lo) ? jx - lo : Ng - 1 - lo - jx);
+        s.my[q] = (unsigned)((jy >= lo) ? jy - lo : Ng - 1 - lo - jy);
+        s.wxs[q] = (jx >= lo) ? s.wx[q] : -s.wx[q];
+        s.wys[q] = (jy >= lo) ? s.wy[q] : -s.wy[q];
+    }
+}
+
+// t3_contract in FP32 over 16-byte nodes: load(i, j, n) fills n[q] with z
+// node iz0 - 1 + q of table row (mx[i], my[j]).
+template <class Load>
+__device__ __forceinline__ void t3_contract_f32(const T3Stencil32 &s, Load load, float &Dx, float &Dy, float &Dz) {
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float u0 = 0.0f, u1 = 0.0f, u2 = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float4 n[4];
+            load(i, j, n);
+            float t0 = s.wz[0] * n[0].x, t1 = s.wz[0] * n[0].y, t2 = s.wz[0] * n[0].z;
+#pragma unroll
+            for (int q = 1; q < 4; ++q) {
+                t0 = fmaf(s.wz[q], n[q].x, t0);
+                t1 = fmaf(s.wz[q], n[q].y, t1);
+                t2 = fmaf(s.wz[q], n[q].z, t2);
+            }
+            u0 = fmaf(s.wy[j], t0, u0);
+            u1 = fmaf(s.wys[j], t1, u1);
+            u2 = fmaf(s.wy[j], t2, u2);
+        }
+        s0 = fmaf(s.wxs[i], u0, s0);
+        s1 = fmaf(s.wx[i], u1, s1);
+        s2 = fmaf(s.wx[i], u2, s2);
+    }
+    Dx = s0;
+    Dy = s1;
+    Dz = s2;
+}
+
+// Nearest-image displacement in FP64, rounded to FP32; returns the FP32 r^2.
+// The main loop and the FP64 redo both call it, so they take the same
+// near-pair decisions.
+__device__ __forceinline__ float t3_mixed_disp(const KScalars &k, const double4 &pj, double xi, double yi, double zi,
+                                               float &dx, float &dy, float &dz) {
+    double ex = pj.x - xi, ey = pj.y - yi, ez = pj.z - zi;
+    if (fabs(ex) > k.halfL) ex -= copysign(k.L, ex);
+    if (fabs(ey) > k.halfL) ey -= copysign(k.L, ey);
+    if (fabs(ez) > k.halfL) ez -= copysign(k.L, ez);
+    dx = (float)ex;
+    dy = (float)ey;
+    dz = (float)ez;
+    return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+}
+
+template <bool WIN>
+__device__ __forceinline__ void body_t3_mixed(const PairArgs &a, const int bx, const int split) {
+    __shared__ __align__(16) float4 win[WIN ? kT3WinNodes32 : 1];
+    __shared__ int brange[4];
+    const double4 *__restrict__ irec = a.irec;
+    const double *__restrict__ isoft = a.isoft;
+    const double4 *__restrict__ jrec = a.jrec;
+    const double *__restrict__ soft = a.soft;
+    const double4 *__restrict__ tiles = a.tiles;
+    const double *__restrict__ table = a.table;
+    const float4 *__restrict__ T32 = a.t3tab32;
+    const uint32_t *__restrict__ keys = a.keys;
+    KScalars k = a.k;
+    k.c15 = in_vgpr(a.k.c15);
+    const int tid = threadIdx.x;
+    const int64_t row_base = (int64_t)bx * kBlock;
+    const int64_t j0 = (int64_t)split * k.jchunk;  // multiple of kTile
+    const int64_t j1 = min(k.N, j0 + k.jchunk);
+    const int64_t r = min(row_base + tid, k.nrows - 1);
+    const double4 p = irec[k.id_min + r];
+    const double xi = p.x, yi = p.y, zi = p.z, hi = isoft[k.id_min + r];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int Ng = k.ng, lo = k.t3_lo;
+    const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3), WS = t3_win_stride32(NZ);
+    int bxmin = 0, bxmax = 0, bymin = 0, bymax = 0;
+    if (WIN) {
+        // rows are the sorted particles (id_min = 0, nrows = N)
+        if (tid == 0) {
+            brange[0] = INT32_MAX;
+            brange[1] = -1;
+            brange[2] = INT32_MAX;
+            brange[3] = -1;
+        }
+        __syncthreads();
+        const uint32_t kr = keys[r];
+        const int rbx = (int)(kr >> 20), rby = pencil_y(kr, Ng);
+        atomicMin(&brange[0], rbx);
+        atomicMax(&brange[1], rbx);
+        atomicMin(&brange[2], rby);
+        atomicMax(&brange[3], rby);
+        __syncthreads();
+        bxmin = brange[0];
+        bxmax = brange[1];
+        bymin = brange[2];
+        bymax = brange[3];
+    }
+
+    int cur = -1;  // cell column of the sources the window was built for
+    bool wok = false;
+    int fx0 = 0, nxw = 0, fy0 = 0, nyw = 0;
+    const int ntiles = (int)((j1 - j0 + kTile - 1) / kTile);  // 32-bit loop bounds: SALU tests
+    for (int ti = 0; ti < ntiles; ++ti) {
+        const int64_t t0 = j0 + (int64_t)ti * kTile;
+        const double hb = hi + tiles[t0 / kTile].w;
+        const float bf = (float)(hb * (1.0 + 1e-5));
+        const float B = bf * bf;
+        const int cnt = (int)min((int64_t)kTile, j1 - t0);
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        bool near = false;
+        for (int u = 0; u < cnt; ++u) {
+            const int64_t j = t0 + u;
+            if (WIN) {
+                const uint32_t kj = keys[j];
+                const int pk = (int)(kj >> 10);
+                if (pk != cur) {  // the same for every wave: both barriers are reached by all
+                    cur = pk;
+                    wok = t3_win_axis(pk >> 10, bxmin, bxmax, Ng, lo, fx0, nxw) &&
+                          t3_win_axis(pencil_y(kj, Ng), bymin, bymax, Ng, lo, fy0, nyw) &&
+                          (int64_t)nxw * nyw * WS <= kT3WinNodes32;
+                    __syncthreads();  // every wave is done with the previous window
+                    if (wok) {
+                        const unsigned nn = (unsigned)(nxw * nyw) * NZ;
+                        for (unsigned o = tid; o < nn; o += kBlock) {
+                            const unsigned row = o / NZ, z = o - row * NZ;
+                            const unsigned wa = row / (unsigned)nyw, wc = row - wa * (unsigned)nyw;
+                            win[row * WS + z] = T32[(((unsigned)fx0 + wa) * NY + (unsigned)fy0 + wc) * NZ + z];
+                        }
+                    }
+                    __syncthreads();  // the window is complete
+                }
+            }
+            const double4 pj = jrec[j];
+            float dx, dy, dz;
+            const float r2 = t3_mixed_disp(k, pj, xi, yi, zi, dx, dy, dz);
+            const bool nr = r2 < B;  // left to the FP64 redo (r2 may be 0 here)
+            near = near || nr;
+            const float y = __builtin_amdgcn_rsqf(r2);
+            const float y3 = nr ? 0.0f : (y * y) * y;
+            const float m = nr ? 0.0f : (float)pj.w;
+            T3Stencil32 st;
+            t3_stencil_f32(k, dx, dy, dz, st);
+            bool in = WIN && wok;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                in = in && (st.mx[q] - (unsigned)fx0 < (unsigned)nxw) && (st.my[q] - (unsigned)fy0 < (unsigned)nyw);
+            float Dx, Dy, Dz;
+            if (in)
+                t3_contract_f32(
+                    st,
+                    [&](int i, int jj, float4 *n) {
+                        const float4 *w0 =
+                            win + ((st.mx[i] - (unsigned)fx0) * (unsigned)nyw + (st.my[jj] - (unsigned)fy0)) * WS +
+                            (unsigned)st.iz0;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) n[q] = w0[q];
+                    },
+                    Dx, Dy, Dz);
+            else
+                t3_contract_f32(
+                    st,
+                    [&](int i, int jj, float4 *n) {
+                        const float4 *g = T32 + (st.mx[i] * NY + st.my[jj]) * NZ + (unsigned)st.iz0;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) n[q] = g[q];
+                    },
+                    Dx, Dy, Dz);
+            ax = fmaf(m, fmaf(y3, dx, -Dx), ax);
+            ay = fmaf(m, fmaf(y3, dy, -Dy), ay);
+            az = fmaf(m, fmaf(y3, dz, -Dz), az);
+        }
+        if (__builtin_expect(near, 0)) {
+            for (int u = 0; u < cnt; ++u) {
+                const int64_t jj = t0 + u;
+                const double4 pj = jrec[jj];
+                float dx, dy, dz;
+                if (t3_mixed_disp(k, pj, xi, yi, zi, dx, dy, dz) < B)
+                    pair_f64<PV_T3_EWALD>(k, table, soft, jj, pj, xi, yi, zi, hi, hb * hb, fx, fy, fz);
+            }
+        }
+        fx += (double)ax;
+        fy += (double)ay;
+        fz += (double)az;
+    }
+    const int64_t rr = row_base + tid;
+    if (rr < k.nrows) {
+        double *out = a.part + (size_t)split * 3u * (size_t)k.nrows;
+        out[rr] = fx;
+        out[k.nrows + rr] = fy;
+        out[2 * k.nrows + rr] = fz;
+    }
+}
+
+enum BodyKind : int { BK_F64 = 0, BK_MIXED_R3 = 1, BK_MIXED_Z = 2, BK_T3_WIN = 3, BK_T3_MIXED = 4, BK_T3_WIN_MIXED = 5 };
 
 template <int BK, int V, int NI, int UNR>
 __device__ __forceinline__ void run_body(const PairArgs &a, int bx, int by) {
     if (BK == BK_F64) body_f64<V, NI, UNR>(a, bx, by);
     else if (BK == BK_T3_WIN) body_t3_win(a, bx, by);
+    else if (BK == BK_T3_MIXED) body_t3_mixed<false>(a, bx, by);
+    else if (BK == BK_T3_WIN_MIXED) body_t3_mixed<true>(a, bx, by);
     else if (BK == BK_MIXED_R3) body_mixed_r3<NI, UNR>(a, bx, by);
     else body_mixed_z<V, NI, UNR>(a, bx, by);
 }
 
 // The mixed R3 kernel is held to 80 VGPRs (6 waves per SIMD; no spills),
 // 1% faster than the 84 the compiler picks on its own (measured at N = 2^21).
-// The mixed nearest z-image kernel is held to 128 (4 waves; 12 bytes of
-// spills): 6% faster than its own 132 at 3 waves (N = 524288).
+// The mixed nearest-image kernels are held to 128 (4 waves): the S1R2 one
+// ran 6% faster than at its own 132 VGPRs and 3 waves (N = 524288).  The
+// mixed T3 window kernel is held to 168 (3 waves, which its 51 KB of LDS
+// allows).
 template <int BK, int V, int NI, int UNR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     (BK == BK_MIXED_R3 || (BK == BK_F64 && (V == PV_T3_NEAREST || V == PV_S1R2_NEAREST)))
         ? 6
-        : ((BK == BK_MIXED_Z && V == PV_S1R2_NEAREST) ? 4 : 1))))
+        : ((BK == BK_MIXED_Z && (V == PV_S1R2_NEAREST || V == PV_T3_NEAREST)) ? 4
+                                                                                : (BK == BK_T3_WIN_MIXED ? 3 : 1)))))
 void pair_kernel(const PairArgs a) {
     run_body<BK, V, NI, UNR>(a, blockIdx.x, blockIdx.y);
 }
@@ -1294,6 +1585,12 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_packed(const PackSeg *__re
 __global__ void table_f32_kernel(const double *__restrict__ T, float2 *__restrict__ T32, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) T32[i] = make_float2((float)T[2 * i], (float)T[2 * i + 1]);
+}
+
+// T3 device layout (three doubles a node) to 16-byte FP32 nodes (Dx, Dy, Dz, 0).
+__global__ void t3_f32_kernel(const double *__restrict__ T, float4 *__restrict__ T32, int64_t nodes) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nodes) T32[i] = make_float4((float)T[3 * i], (float)T[3 * i + 1], (float)T[3 * i + 2], 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1422,7 +1719,7 @@ __global__ __launch_bounds__(kTile) void tiles_kernel(const double4 *__restrict_
 // on the N = 2^21 R3 workload together with the rows per lane: FP64 1 row x 4
 // sources (80 VGPRs, 6 waves per SIMD), mixed 2 rows x 8 sources.
 constexpr int unroll_of(int v, int precision) {
-    return precision == STEPS_PREC_MIXED ? (v == PV_R3 ? 8 : (v == PV_S1R2_NEAREST ? 8 : 2))
+    return precision == STEPS_PREC_MIXED ? ((v == PV_R3 || v == PV_S1R2_NEAREST || v == PV_T3_NEAREST) ? 8 : 2)
                                          : ((v == PV_R3 || v == PV_T3_NEAREST || v == PV_S1R2_NEAREST) ? 4 : 1);
 }
 
@@ -1446,6 +1743,8 @@ hipError_t launch_pair(int variant, int precision, const PairArgs *a, const Pack
     if (precision == STEPS_PREC_MIXED) {
         switch (variant) {
         case PV_R3: return launch_pair_t<BK_MIXED_R3, PV_R3>(a, segs, nseg, gx, gy, st);
+        case PV_T3_NEAREST: return launch_pair_t<BK_MIXED_Z, PV_T3_NEAREST>(a, segs, nseg, gx, gy, st);
+        case PV_T3_EWALD: return launch_pair_t<BK_T3_MIXED, PV_T3_EWALD>(a, segs, nseg, gx, gy, st);
         case PV_S1R2_NEAREST: return launch_pair_t<BK_MIXED_Z, PV_S1R2_NEAREST>(a, segs, nseg, gx, gy, st);
         case PV_S1R2_CIC: return launch_pair_t<BK_MIXED_Z, PV_S1R2_CIC>(a, segs, nseg, gx, gy, st);
         case PV_S1R2_TSC: return launch_pair_t<BK_MIXED_Z, PV_S1R2_TSC>(a, segs, nseg, gx, gy, st);
@@ -1486,7 +1785,7 @@ bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
 // no-wrap test; in input order a wave of a lattice spans the box); the
 // reduction scatters the sums back to input order.
 bool sorted_rows(const EvalPlan &p) {
-    const bool nearest = (!is_mixed(p) && p.variant == PV_T3_NEAREST) || p.variant == PV_S1R2_NEAREST;
+    const bool nearest = p.variant == PV_T3_NEAREST || p.variant == PV_S1R2_NEAREST;
     return (pencil_order(p) || nearest) && sorted_sources(p) && p.id_min == 0 && p.nrows == p.N;
 }
 // The LDS-window T3 kernel needs rows in pencils and room for a 5 x 5 window
@@ -1495,9 +1794,23 @@ bool t3_window(const EvalPlan &p) {
     return p.variant == PV_T3_EWALD && sorted_rows(p) && !is_mixed(p) &&
            (int64_t)t3_win_stride(3u * (unsigned)(p.t3_ng + 3) + 2u) * 25 <= kT3WinDoubles;
 }
+// The same for the mixed T3 kernel (16-byte nodes; Ng <= 127).
+bool t3_window_mixed(const EvalPlan &p) {
+    return p.variant == PV_T3_EWALD && sorted_rows(p) && is_mixed(p) &&
+           (int64_t)t3_win_stride32((unsigned)(p.t3_ng + 3)) * 25 <= kT3WinNodes32;
+}
 
 size_t s1r2_f32_entries(const EvalPlan &p) {
     return (is_mixed(p) && is_s1r2_lookup(p.variant)) ? s1r2_device_nodes(p.s1r2_nrho, p.s1r2_nz) : 0;
+}
+// Nodes of the FP32 T3 table of the mixed kernel (one float4 each), sized for
+// the unfolded layout so that the scratch size does not depend on the table.
+size_t t3_f32_nodes(const EvalPlan &p) {
+    return (is_mixed(p) && p.variant == PV_T3_EWALD) ? t3_device_entries(p.t3_ng, 0) / 3 : 0;
+}
+// Bytes of the FP32 table copy of a mixed table kernel (S1R2 or T3).
+size_t table32_bytes(const EvalPlan &p) {
+    return std::max(s1r2_f32_entries(p) * sizeof(float2), t3_f32_nodes(p) * sizeof(float4));
 }
 
 // Pieces of an evaluation's scratch buffer, in the order tile_extra_bytes
@@ -1523,7 +1836,7 @@ ScratchView scratch_view(const EvalPlan &p, double *partials) {
     v.tile_ext = (double4 *)((char *)v.tiles + tb);
     v.jrel = (float4 *)((char *)v.tile_ext + tb);
     v.tab32 = (float2 *)((char *)v.jrel + align256(n * sizeof(float4)));
-    v.jrec_s = (double4 *)((char *)v.tab32 + align256(s1r2_f32_entries(p) * sizeof(float2)));
+    v.jrec_s = (double4 *)((char *)v.tab32 + align256(table32_bytes(p)));
     v.soft_s = (double *)((char *)v.jrec_s + align256(n * sizeof(double4)));
     v.order = (char *)v.soft_s + align256(n * sizeof(double));
     return v;
@@ -1556,6 +1869,7 @@ KScalars make_scalars(const EvalPlan &p) {
     k.inv_drhof = (float)k.inv_drho;
     k.inv_dzgf = (float)k.inv_dzg;
     k.last_rf = (float)k.last_r;
+    k.inv_gsf = (float)k.inv_gs;
     return k;
 }
 
@@ -1571,6 +1885,7 @@ PairArgs make_args(const EvalPlan &p, const DeviceParticles &dp, const ScratchVi
     a.tile_ext = v.tile_ext;
     a.table = (p.variant == PV_T3_EWALD) ? p.t3 : (is_s1r2_lookup(p.variant) ? p.s1r2 : nullptr);
     a.tab32 = v.tab32;
+    a.t3tab32 = (const float4 *)v.tab32;  // the same scratch region: one mixed table per evaluation
     a.keys = (sorted && pencil_order(p)) ? spatial_order_keys(v.order, p.N) : nullptr;
     a.part = v.part;
     a.k = make_scalars(p);
@@ -1663,7 +1978,7 @@ size_t tile_extra_bytes(const EvalPlan &p) {
     size_t b = 2 * align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4));
     if (sorted_sources(p)) {
         b += align256(n * sizeof(float4));
-        b += align256(s1r2_f32_entries(p) * sizeof(float2));
+        b += align256(table32_bytes(p));
         b += align256(n * sizeof(double4)) + align256(n * sizeof(double));
         b += align256(order_scratch_bytes(p.N));
     }
@@ -1680,6 +1995,12 @@ hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, d
     if (n32 > 0) {
         hipLaunchKernelGGL(table_f32_kernel, dim3((unsigned)((n32 + 255) / 256)), dim3(256), 0, st, p.s1r2,
                            v.tab32, n32);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (t3_f32_nodes(p) > 0) {
+        const int64_t nodes = (int64_t)(t3_device_entries(p.t3_ng, p.t3_lo) / 3);  // the layout in use
+        hipLaunchKernelGGL(t3_f32_kernel, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, st, p.t3,
+                           (float4 *)v.tab32, nodes);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (sorted_sources(p)) {
@@ -1705,6 +2026,8 @@ hipError_t launch_forces_pairs(const EvalPlan &p, const DeviceParticles &dp, dou
     if (!has_pairs(p)) return hipSuccess;
     const PairArgs a = make_args(p, dp, scratch_view(p, partials));
     if (t3_window(p)) return launch_pair_t<BK_T3_WIN, PV_T3_EWALD>(&a, nullptr, 0, row_blocks(p), (unsigned)p.splits, st);
+    if (t3_window_mixed(p))
+        return launch_pair_t<BK_T3_WIN_MIXED, PV_T3_EWALD>(&a, nullptr, 0, row_blocks(p), (unsigned)p.splits, st);
     return launch_pair(p.variant, is_mixed(p) ? STEPS_PREC_MIXED : STEPS_PREC_FP64, &a, nullptr, 0, row_blocks(p),
                        (unsigned)p.splits, st);
 }

@@ -73,7 +73,8 @@ def lib() -> C.CDLL:
     for name in ("oracle_forces",):
         getattr(L, name).restype = C.c_int
         getattr(L, name).argtypes = [C.POINTER(OracleParams), C.c_int, _DP, _DP, _DP, _DP, C.c_int, C.c_int]
-    for name in ("oracle_forces_rows", "oracle_forces_faithful_rows", "oracle_forces_fast_rows"):
+    for name in ("oracle_forces_rows", "oracle_forces_faithful_rows", "oracle_forces_fast_rows",
+                 "oracle_forces_r3_fp32_rows"):
         getattr(L, name).restype = C.c_int
         getattr(L, name).argtypes = [C.POINTER(OracleParams), C.c_int, _DP, _DP, _DP, _DP, _IP, C.c_int]
     L.oracle_max_threads.restype = C.c_int
@@ -194,7 +195,7 @@ def forces_rows(p: OracleParams, x, M, h, rows, kind: str = "oracle") -> np.ndar
     rows = np.ascontiguousarray(rows, dtype=np.int32)
     F = np.zeros(3 * max(rows.size, 1))
     fn = {"oracle": lib().oracle_forces_rows, "faithful": lib().oracle_forces_faithful_rows,
-          "fast": lib().oracle_forces_fast_rows}[kind]
+          "fast": lib().oracle_forces_fast_rows, "fp32": lib().oracle_forces_r3_fp32_rows}[kind]
     rc = fn(C.byref(p), M.size, dp(x), dp(M), dp(h), dp(F), ip(rows), rows.size)
     assert rc == 0, rc
     return F[: 3 * rows.size].reshape(-1, 3)

@@ -61,8 +61,8 @@ def test_argument_errors_before_device():
     assert _eval_rc(ds, 10, 0, 9) == _abi.STEPS_ERR_TABLE
     dr, _ = steps_amd.make_desc(2, is_periodic=1, L=10.0, Rsim=5.0, cosmology=1, comoving=1)
     assert _eval_rc(dr, 10, 0, 9) == _abi.STEPS_ERR_TABLE     # radial table needed
-    dm, _ = steps_amd.make_desc(1, is_periodic=1, L=10.0, precision=1)
-    assert _eval_rc(dm, 10, 0, 9) == _abi.STEPS_ERR_ARG       # mixed is R3-only
+    dm, _ = steps_amd.make_desc(2, is_periodic=2, L=10.0, Rsim=5.0, s1r2_lookup=0, precision=1)
+    assert _eval_rc(dm, 10, 0, 9) == _abi.STEPS_ERR_ARG       # no mixed kernel for the S1R2 image sum
     bad = np.zeros(30)
     bad[4] = np.nan
     assert _eval_rc(d, 10, 0, 9, x=bad) == _abi.STEPS_ERR_NONFINITE

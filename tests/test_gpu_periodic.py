@@ -88,6 +88,36 @@ def test_t3_row_subset_and_shuffled_input(steps, oracle, t3_tab):
     assert np.median(e) <= 1e-12 and np.percentile(e, 99) <= 1e-10
 
 
+@pytest.mark.parametrize("is_periodic", [2, 1])
+def test_t3_mixed_precision(steps, oracle, t3_tab, is_periodic):
+    """Mixed-precision T3 (tricubic Ewald correction from an FP32 table, or
+    the nearest image only) at N = 48^3: every row against the FP64 path,
+    sampled rows against the oracle (pair-sum bar p99 <= 1e-5; T3 has no
+    external term); deterministic; Ewald: a row subset, which takes the kernel
+    without the LDS window, bitwise equal to the full evaluation (window)."""
+    print(f"case t3 mixed is_periodic={is_periodic} n_side=48", flush=True)
+    w = wl.t3_lattice(n_side=48, seed=1001, is_periodic=is_periodic)
+    tab = t3_tab if is_periodic >= 2 else None
+    kw = dict(is_periodic=is_periodic, L=w.L, t3_table=tab)
+    d64, k64 = steps.make_desc(steps.TOPO_T3, **kw)
+    dmx, kmx = steps.make_desc(steps.TOPO_T3, precision=steps.PREC_MIXED, **kw)
+    F64 = steps.force_eval(d64, w.x, w.M, w.soft)
+    Fmx = steps.force_eval(dmx, w.x, w.M, w.soft)
+    assert not np.array_equal(Fmx, F64)                 # the FP32 path really ran
+    e = rel_err(Fmx, F64)
+    report(f"T3 mixed is_periodic={is_periodic} N={w.N} vs fp64", e)
+    assert np.median(e) <= 2e-6 and np.percentile(e, 99) <= 1e-5
+    rows = np.linspace(0, w.N - 1, 32).astype(np.int32)
+    p = oracle.params(1, is_periodic=is_periodic, L=w.L, t3_table=tab, interp_order=4)
+    eo = rel_err(Fmx[rows], oracle.forces_rows(p, w.x, w.M, w.soft, rows))
+    report(f"T3 mixed is_periodic={is_periodic} sampled rows vs oracle", eo)
+    assert np.median(eo) <= 2e-6
+    assert np.array_equal(steps.force_eval(dmx, w.x, w.M, w.soft), Fmx)
+    if is_periodic >= 2:
+        Fs = steps.force_eval(dmx, w.x, w.M, w.soft, id_min=37, id_max=w.N - 1)
+        assert np.array_equal(Fs, Fmx[37:])
+
+
 def _foldable(T, ng):
     """The library's fold rule: exact antisymmetry in x and in y, except that
     components vanishing by symmetry may hold noise below 2^-52 max|T|, and

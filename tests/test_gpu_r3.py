@@ -92,6 +92,14 @@ def test_r3_mixed_precision_error(steps, oracle):
         e = rel_err(Fmx, F64)
         report(f"R3 mixed vs fp64 N=16384 cosmology={cosmology}", e)
         assert np.percentile(e, 99) <= 1e-5
+    # the comparison point: the reference's all-float build (stand-in), same
+    # rows, pair sum only; the mixed path must be the more accurate
+    rows = np.arange(0, w.N, 8, dtype=np.int32)
+    e32 = rel_err(oracle.forces_rows(oracle.params(0, cosmology=0), w.x, w.M, w.soft, rows, kind="fp32"),
+                  F64[rows])
+    emx = rel_err(Fmx[rows], F64[rows])
+    report("R3 reference all-float stand-in vs fp64 N=16384", e32)
+    assert np.median(emx) < np.median(e32) and np.percentile(emx, 99) < np.percentile(e32, 99)
 
 
 def test_r3_full_size_2m(steps, oracle):
@@ -121,4 +129,7 @@ def test_r3_full_size_2m(steps, oracle):
     em = rel_err(sim.get_forces(), F)
     report("R3 N=2^21 mixed pair sum vs fp64", em)
     assert np.percentile(em, 99) <= 1e-5
+    e32 = rel_err(oracle.forces_rows(p, w.x, w.M, w.soft, rows, kind="fp32"), F[rows])
+    report("R3 N=2^21 reference all-float stand-in vs fp64 (sampled rows)", e32)
+    assert np.median(em[rows]) * 10 < np.median(e32)
     sim.close()

@@ -138,7 +138,7 @@ def test_admission_and_eviction_keep_results(ensemble):
 
 def test_mixed_precision_requests_match_alone(ensemble):
     sims, ref = ensemble
-    capable = [s for k, s in enumerate(sims) if k != 4]     # all but the T3 simulation
+    capable = sims                                          # every variant here has a mixed kernel
     ref_mixed = [alone(w, d, precision=1) for (w, d, _) in capable]
     srv = steps_amd.Server(streams=3)
     ids, results, states = run_mix(capable, srv, precisions=[1] * len(capable))
@@ -151,7 +151,8 @@ def test_mixed_precision_requests_match_alone(ensemble):
 def test_failing_request_is_isolated(ensemble):
     sims, ref = ensemble
     srv = steps_amd.Server(streams=4)
-    c, d, _ = sims[4]             # T3: it has no mixed-precision path
+    c = sims[5][0]                # S1R2 image sum: it has no mixed-precision path
+    d, keep = steps_amd.make_desc(steps_amd.TOPO_S1R2, is_periodic=2, L=c.L, Rsim=c.Rsim, s1r2_lookup=0)
     w0, d0, _ = sims[0]
     bad = srv.add_sim(d, c.x, c.M, c.soft)
     good = srv.add_sim(d0, w0.x, w0.M, w0.soft, w0.v)

@@ -118,3 +118,36 @@ def test_softening_setup_closed_form(oracle):
     assert mmin == 2.0
     assert rp == pytest.approx(2.0 / (4.0 * math.pi * 0.3 ** 3 / 3.0), rel=1e-15)
     assert np.allclose(h, 0.3 * np.cbrt(M / 2.0), rtol=1e-15)
+
+
+def test_fp32_stand_in_two_body_and_branches(oracle):
+    """The all-float stand-in of the reference's single-precision build: a
+    two-body force is the FP64 one to float rounding, in every softening
+    branch, and other topologies are refused."""
+    rows = np.array([0, 1], dtype=np.int32)
+    for sep in (5.0, 0.15, 0.05):  # beta = 0.2: r >= beta, beta/2 < r < beta, r <= beta/2
+        x = np.array([[0.25, -0.5, 1.0], [0.25 + 0.6 * sep, -0.5 + 0.8 * sep, 1.0]])
+        M = np.array([1.5, 0.75])
+        h = np.array([0.1, 0.1])
+        F64 = oracle.forces(oracle.params(0), x, M, h)
+        F32 = oracle.forces_rows(oracle.params(0), x, M, h, rows, kind="fp32")
+        assert np.all(rel_err(F32, F64) <= 2e-6)
+        assert not np.array_equal(F32, F64)  # the float path really ran
+    with pytest.raises(AssertionError):
+        oracle.forces_rows(oracle.params(1, is_periodic=1, L=10.0), x, M, h, rows, kind="fp32")
+
+
+def test_fp32_stand_in_error_grows_with_n(oracle):
+    """Uncompensated float accumulation: against the FP64 oracle the error is
+    bounded by float rounding times the number of terms and grows with N."""
+    med = []
+    for N in (512, 16384):
+        w = wl.r3_compactified(N=N, seed=1000)
+        p = oracle.params(0, cosmology=0)
+        rows = np.linspace(0, N - 1, 48).astype(np.int32)
+        e = rel_err(oracle.forces_rows(p, w.x, w.M, w.soft, rows, kind="fp32"),
+                    oracle.forces_rows(p, w.x, w.M, w.soft, rows))
+        report(f"fp32 stand-in vs fp64 N={N}", e)
+        assert np.median(e) <= 6e-8 * N
+        med.append(np.median(e))
+    assert med[1] > med[0]
