@@ -1557,15 +1557,17 @@ __device__ __forceinline__ void run_body(const PairArgs &a, int bx, int by) {
 
 // The mixed R3 kernel is held to 80 VGPRs (6 waves per SIMD; no spills),
 // 1% faster than the 84 the compiler picks on its own (measured at N = 2^21).
-// The mixed nearest-image kernels are held to 128 (4 waves): the S1R2 one
-// ran 6% faster than at its own 132 VGPRs and 3 waves (N = 524288).  The
+// The mixed S1R2 nearest-image kernel is held to 128 (4 waves): 6% faster
+// than at its own 132 VGPRs and 3 waves (N = 524288); the mixed T3 one (one
+// row per lane) to 80 (6 waves; 52 bytes of spills, still 5% faster than
+// at 4 waves).  The
 // mixed T3 window kernel is held to 168 (3 waves, which its 51 KB of LDS
 // allows).
 template <int BK, int V, int NI, int UNR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     (BK == BK_MIXED_R3 || (BK == BK_F64 && (V == PV_T3_NEAREST || V == PV_S1R2_NEAREST)))
         ? 6
-        : ((BK == BK_MIXED_Z && (V == PV_S1R2_NEAREST || V == PV_T3_NEAREST)) ? 4
+        : ((BK == BK_MIXED_Z && (V == PV_S1R2_NEAREST || V == PV_T3_NEAREST)) ? (V == PV_T3_NEAREST ? 6 : 4)
                                                                                 : (BK == BK_T3_WIN_MIXED ? 3 : 1)))))
 void pair_kernel(const PairArgs a) {
     run_body<BK, V, NI, UNR>(a, blockIdx.x, blockIdx.y);
