@@ -1307,16 +1307,20 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
 // accumulators.  WIN: rows and sources in pencils and the table window of
 // body_t3_win staged in LDS (16 bytes a node: at most 3264 nodes, 51 KB,
 // three workgroups per CU); a lane whose stencil leaves the window reads the
-// global FP32 table, so both forms give the same sums bitwise.  Like the FP64
-// kernel it is bound by LDS bank conflicts, not by LDS bytes, and runs at
-// about the FP64 kernel's rate (35 against 34 G pairs/s at 64^3).
+// global FP32 table, so both forms give the same sums bitwise.  With every
+// workgroup on the window it is bound by LDS bank conflicts like the FP64
+// kernel (33 against 34 G pairs/s at 64^3); a third of the workgroups on the
+// global table takes it to 38-39.
 // ---------------------------------------------------------------------------
 constexpr int kT3WinNodes32 = 3264;
-// Window rows start on 256-byte boundaries (16 nodes), so every row has the
-// same bank alignment: 35.3 against 33.1 G pairs/s with unpadded rows at 64^3
-// (where a 6 x 8 window then no longer fits and its workgroup reads the global
-// table), and 30.3 with 78 KB windows at two workgroups per CU.
-constexpr unsigned kT3WinPad32 = 16u;
+// Row blocks with bx % 3 < 2 use the window and the others read the global
+// FP32 table, so that the LDS and the vector-L1 path both carry taps: at 64^3
+// 38-39 G pairs/s, against 33.1 with every block on the window (LDS-bound) and
+// 26.0 with none (vector-L1-bound).  Shares of 4/7, 5/7 and 7/9 measured 38.0,
+// 37.7 and 37.3; shares over an even count (1/2, 3/4) left the global blocks
+// on a few XCDs and ran at 26.  Rows padded to 256 bytes ran at 31.5.
+constexpr unsigned kT3WinNum32 = 2u, kT3WinDen32 = 3u;
+constexpr unsigned kT3WinPad32 = 1u;  // window rows start at multiples of this many nodes
 __host__ __device__ constexpr unsigned t3_win_stride32(unsigned nz) { return (nz + kT3WinPad32 - 1u) / kT3WinPad32 * kT3WinPad32; }
 
 struct T3Stencil32 {
@@ -1426,6 +1430,7 @@ __device__ __forceinline__ void body_t3_mixed(const PairArgs &a, const int bx, c
     const int Ng = k.ng, lo = k.t3_lo;
     const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3), WS = t3_win_stride32(NZ);
     int bxmin = 0, bxmax = 0, bymin = 0, bymax = 0;
+    const bool win_share = (unsigned)bx % kT3WinDen32 < kT3WinNum32;
     if (WIN) {
         // rows are the sorted particles (id_min = 0, nrows = N)
         if (tid == 0) {
@@ -1467,7 +1472,7 @@ __device__ __forceinline__ void body_t3_mixed(const PairArgs &a, const int bx, c
                 const int pk = (int)(kj >> 10);
                 if (pk != cur) {  // the same for every wave: both barriers are reached by all
                     cur = pk;
-                    wok = t3_win_axis(pk >> 10, bxmin, bxmax, Ng, lo, fx0, nxw) &&
+                    wok = win_share && t3_win_axis(pk >> 10, bxmin, bxmax, Ng, lo, fx0, nxw) &&
                           t3_win_axis(pencil_y(kj, Ng), bymin, bymax, Ng, lo, fy0, nyw) &&
                           (int64_t)nxw * nyw * WS <= kT3WinNodes32;
                     __syncthreads();  // every wave is done with the previous window
