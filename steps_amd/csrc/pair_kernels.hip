@@ -316,19 +316,32 @@ __device__ __forceinline__ int s1r2_zfix(int iz, int lo_ok, int hi_ok, int Nz) {
 // Divisions by the grid spacings become multiplications by their reciprocals
 // and rho, 1/rho come from one reciprocal square root (each within an ulp or
 // two of the reference's sqrt and divides; the interpolants are continuous).
+// The interpolation split into its index math and a gather over rho rows:
+// row(ir) returns the z run of rho node ir (device layout: z nodes -1 .. Nz+1),
+// from the global table or from an LDS window of rows; s1r2_rows gives the
+// rho nodes [lo, hi] a pair's stencil reads.
 template <int ORDER>
-__device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, const KScalars &k, double dx, double dy,
-                                            double dz, double &q, double &Dz) {
+__device__ __forceinline__ void s1r2_rows(const KScalars &k, double rho, int &lo, int &hi) {
+    if (ORDER == 0) {
+        lo = hi = (int)floor(fmin(rho, k.rho_max) * k.inv_drho + 0.5);
+    } else if (ORDER == 2) {
+        lo = (int)floor(fmin(rho * k.inv_drho, k.last_r));
+        hi = lo + 1;
+    } else {
+        const int jr = (int)floor(fmin(rho * k.inv_drho, k.last_r) + 0.5);
+        lo = jr - 1;
+        hi = jr + 1;
+    }
+}
+
+template <int ORDER, class Row>
+__device__ __forceinline__ void s1r2_gather(Row row, const KScalars &k, double rho, double uz, double &Dr,
+                                            double &Dzv) {
     const int Nz = k.nz;
-    const double rxy2 = fma(dx, dx, dy * dy);
-    const double inv_rho = (rxy2 > 0.0) ? rsqrt_f64(rxy2) : 0.0;
-    const double rho = rxy2 * inv_rho;
-    const double uz = fma(dz + k.halfL, k.inv_dzg, -0.5);
-    double Dr, Dzv;
     if (ORDER == 0) {
         const int ir = (int)floor(fmin(rho, k.rho_max) * k.inv_drho + 0.5);  // in [0, Nrho-1]
         const int iz = s1r2_zfix((int)floor(uz + 0.5), 0, Nz, Nz);
-        const double2 t = s1r2_row(T, k, ir)[iz + 1];
+        const double2 t = row(ir)[iz + 1];
         Dr = t.x;
         Dzv = t.y;
     } else if (ORDER == 2) {
@@ -341,8 +354,8 @@ __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, const 
         const double uzf = floor(uz);
         const double fz = uz - uzf;
         const int iz0 = s1r2_zfix((int)uzf, -1, Nz - 1, Nz);
-        const double2 *r0 = s1r2_row(T, k, ir0) + (iz0 + 1);      // z nodes iz0, iz0 + 1
-        const double2 *r1 = s1r2_row(T, k, ir0 + 1) + (iz0 + 1);
+        const double2 *r0 = row(ir0) + (iz0 + 1);  // z nodes iz0, iz0 + 1
+        const double2 *r1 = row(ir0 + 1) + (iz0 + 1);
         const double2 t00 = r0[0], t01 = r0[1], t10 = r1[0], t11 = r1[1];
         // bilinear as two lerps in rho and one in z (same interpolant as the
         // reference's four weights, fewer operations)
@@ -364,13 +377,30 @@ __device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, const 
         Dzv = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double2 *rr = s1r2_row(T, k, jr - 1 + c) + jz;  // rho node jr-1+c, z nodes jz-1 .. jz+1
+            const double2 *rr = row(jr - 1 + c) + jz;  // rho node jr-1+c, z nodes jz-1 .. jz+1
             const double w0 = wr[c] * wz[0], w1 = wr[c] * wz[1], w2 = wr[c] * wz[2];
             const double2 a = rr[0], b = rr[1], d = rr[2];
             Dr += w0 * a.x + w1 * b.x + w2 * d.x;
             Dzv += w0 * a.y + w1 * b.y + w2 * d.y;
         }
     }
+}
+
+// rho = |(dx, dy)|, its reciprocal, and the z grid coordinate of dz.
+__device__ __forceinline__ void s1r2_coords(const KScalars &k, double dx, double dy, double dz, double &rho,
+                                            double &inv_rho, double &uz) {
+    const double rxy2 = fma(dx, dx, dy * dy);
+    inv_rho = (rxy2 > 0.0) ? rsqrt_f64(rxy2) : 0.0;
+    rho = rxy2 * inv_rho;
+    uz = fma(dz + k.halfL, k.inv_dzg, -0.5);
+}
+
+template <int ORDER>
+__device__ __forceinline__ void s1r2_interp(const double *__restrict__ T, const KScalars &k, double dx, double dy,
+                                            double dz, double &q, double &Dz) {
+    double rho, inv_rho, uz, Dr, Dzv;
+    s1r2_coords(k, dx, dy, dz, rho, inv_rho, uz);
+    s1r2_gather<ORDER>([&](int ir) { return s1r2_row(T, k, ir); }, k, rho, uz, Dr, Dzv);
     q = Dr * inv_rho;
     Dz = Dzv;
 }
@@ -389,13 +419,23 @@ __device__ __forceinline__ double pair_w_tile(double r2, double B, double hi,
     return w;
 }
 
-// One pair (i = lane row, j = uniform source) for variant V, FP64.
-template <int V>
+// An LDS copy of the S1R2 table rows for rho nodes w0 .. w0 + nw - 1 (each
+// the full z run of the device layout), valid when ok.
+struct S1R2Win {
+    const double2 *win;
+    int w0, nw;
+    bool ok;
+};
+
+// One pair (i = lane row, j = uniform source) for variant V, FP64.  WIN: an
+// S1R2 lookup pair reads its rho rows from the window when they all lie in
+// it, else from the global table (the same values either way).
+template <int V, bool WIN = false>
 __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__restrict__ table,
                                          const double *__restrict__ soft, const int64_t j,
                                          const double4 pj, const double xi, const double yi,
                                          const double zi, const double hi, const double B,
-                                         double &fx, double &fy, double &fz) {
+                                         double &fx, double &fy, double &fz, const S1R2Win sw = S1R2Win{}) {
     double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
     if (V == PV_R3) {
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -451,13 +491,22 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
             fz = fma(s, dz, fz);
         } else {
             // m (w d - D) with D_x = q dx, D_y = q dy: x and y share m (w - q)
-            double q, Dz;
-            if (V == PV_S1R2_NGP)
-                s1r2_interp<0>(table, k, dx, dy, dz, q, Dz);
-            else if (V == PV_S1R2_CIC)
-                s1r2_interp<2>(table, k, dx, dy, dz, q, Dz);
-            else
-                s1r2_interp<4>(table, k, dx, dy, dz, q, Dz);
+            constexpr int ORD = (V == PV_S1R2_NGP) ? 0 : ((V == PV_S1R2_CIC) ? 2 : 4);
+            double rho, inv_rho, uz, Dr, Dzv;
+            s1r2_coords(k, dx, dy, dz, rho, inv_rho, uz);
+            bool in = false;
+            if (WIN && sw.ok) {
+                int lo, hi;
+                s1r2_rows<ORD>(k, rho, lo, hi);
+                in = lo >= sw.w0 && hi < sw.w0 + sw.nw;
+            }
+            if (in) {
+                const unsigned nz3 = (unsigned)(k.nz + 3);
+                s1r2_gather<ORD>([&](int ir) { return sw.win + (unsigned)(ir - sw.w0) * nz3; }, k, rho, uz, Dr, Dzv);
+            } else {
+                s1r2_gather<ORD>([&](int ir) { return s1r2_row(table, k, ir); }, k, rho, uz, Dr, Dzv);
+            }
+            const double q = Dr * inv_rho, Dz = Dzv;
             const double sxy = pj.w * (w - q);
             fx = fma(sxy, dx, fx);
             fy = fma(sxy, dy, fy);
@@ -534,6 +583,16 @@ __device__ __forceinline__ double box_dist2(double x, double y, double z, const 
     return fma(gx, gx, fma(gy, gy, gz * gz));
 }
 
+// S1R2 CIC / TSC kernels: LDS for the rho rows of one source tile (39 KB:
+// four workgroups per CU; 19 rows of a 128-cell z run), filled from the
+// global table at each tile; a tile whose rows do not fit reads the global
+// table.  The table loads otherwise co-limit these kernels on the vector-L1
+// path (4 and 9 16-byte loads per pair).  At N = 524288 (cylinder, sources in
+// input order): CIC 345 -> 394 and TSC 192 -> 251 G pairs/s, bitwise the same
+// sums; windows of 52 KB (3 workgroups) and 26 KB (6) were slower (360 / 250,
+// 390 / 225), and NGP, one load per pair, lost 15% and keeps the global reads.
+constexpr int kS1R2WinNodes = 2496;
+
 template <int V, int NI, int UNR>
 __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const int split) {
     const double4 *__restrict__ irec = a.irec;
@@ -563,6 +622,18 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
         fx[q] = fy[q] = fz[q] = 0.0;
     }
 
+    // S1R2 lookup: per source tile, the rho rows every pair of the workgroup
+    // can read, staged in LDS (see kS1R2WinNodes); range slots alternate
+    // between tiles so that one barrier closes the atomics and the previous
+    // tile's reads of the window
+    constexpr bool SWIN = V == PV_S1R2_CIC || V == PV_S1R2_TSC;
+    __shared__ __align__(16) double2 s1win[SWIN ? kS1R2WinNodes : 1];
+    __shared__ int s1rng[4];
+    if (SWIN) {
+        if (tid < 4) s1rng[tid] = (tid & 1) ? INT32_MIN : INT32_MAX;
+        __syncthreads();
+    }
+
     // loop bounds in 32 bits, relative to the tile: the loop tests stay on the
     // SALU (a 64-bit signed compare would take two VALU instructions per trip)
     const int ntiles = (int)((j1 - j0 + kTile - 1) / kTile);
@@ -579,6 +650,34 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
         const int cnt = (ti + 1 < ntiles) ? kTile : (int)(j1 - t0);  // sources in this tile
         const double4 *__restrict__ tj = jrec + t0;
         int u0 = 0;
+        S1R2Win sw{s1win, 0, 0, false};
+        if (SWIN) {
+            // rho of every pair with this tile lies between the distances from
+            // the row to the near and far sides of the tile's xy box; the
+            // stencils read rho nodes floor(rho / drho) - 1 .. floor + 2
+            const double4 te = ext[t0 / kTile];
+            const double ax = fabs(xi[0] - tc.x), ay = fabs(yi[0] - tc.y);
+            const double gx = fmax(ax - te.x, 0.0), gy = fmax(ay - te.y, 0.0);
+            const double ex = ax + te.x, ey = ay + te.y;
+            const int lo = (int)floor(fmin(sqrt(fma(gx, gx, gy * gy)) * k.inv_drho, k.last_r)) - 1;
+            const int hi = (int)floor(fmin(sqrt(fma(ex, ex, ey * ey)) * k.inv_drho, k.last_r)) + 2;
+            int *rg = s1rng + 2 * (ti & 1);
+            atomicMin(&rg[0], lo);
+            atomicMax(&rg[1], hi);
+            __syncthreads();  // ranges complete; every wave is done with the previous window
+            const int w0 = max(rg[0], -1), w1 = min(rg[1], k.nrho);  // ghost rows -1 and Nrho
+            if (tid < 2) s1rng[2 * ((ti + 1) & 1) + tid] = tid ? INT32_MIN : INT32_MAX;  // next tile's slot
+            const unsigned nz3 = (unsigned)(k.nz + 3);
+            sw.w0 = w0;
+            sw.nw = w1 - w0 + 1;
+            sw.ok = (int64_t)sw.nw * nz3 <= kS1R2WinNodes;
+            if (sw.ok) {
+                const double2 *src = s1r2_row(table, k, w0);  // rows w0 .. w1 are contiguous
+                const unsigned nn = (unsigned)sw.nw * nz3;
+                for (unsigned o = tid; o < nn; o += kBlock) s1win[o] = src[o];
+            }
+            __syncthreads();  // the window is complete
+        }
         if (V == PV_R3 || V == PV_T3_NEAREST || V == PV_S1R2_NEAREST) {
             // Spatially ordered sources: when the tile's bounding box lies
             // outside every lane's softening bound (B widened by 1e-9 for
@@ -645,14 +744,15 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
             for (int u = 0; u < UNR; ++u)
 #pragma unroll
                 for (int q = 0; q < NI; ++q)
-                    pair_f64<V>(k, table, soft, t0 + u0 + u, pj[u], xi[q], yi[q], zi[q], hi[q], B[q], fx[q],
-                                fy[q], fz[q]);
+                    pair_f64<V, SWIN>(k, table, soft, t0 + u0 + u, pj[u], xi[q], yi[q], zi[q], hi[q], B[q], fx[q],
+                                      fy[q], fz[q], sw);
         }
         for (; u0 < cnt; ++u0) {
             const double4 pj = tj[u0];
 #pragma unroll
             for (int q = 0; q < NI; ++q)
-                pair_f64<V>(k, table, soft, t0 + u0, pj, xi[q], yi[q], zi[q], hi[q], B[q], fx[q], fy[q], fz[q]);
+                pair_f64<V, SWIN>(k, table, soft, t0 + u0, pj, xi[q], yi[q], zi[q], hi[q], B[q], fx[q], fy[q], fz[q],
+                                  sw);
         }
     }
 

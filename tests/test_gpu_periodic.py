@@ -325,6 +325,37 @@ def test_s1r2_lookup_forces(steps, oracle, cyl, order):
     assert np.percentile(e, 99) <= 1e-9
 
 
+@pytest.mark.parametrize("order", [2, 4])
+def test_s1r2_table_window_matches_global_reads(steps, oracle, cyl, order):
+    """The CIC / TSC kernels stage the rho rows of each source tile in LDS
+    when the workgroup's pairs fit a window of rows, else read the global
+    table.  Input order (windows fit), a row subset (other workgroups, other
+    windows) and shuffled rows and sources (windows mostly do not fit) give the
+    same values: bitwise for the subset, to summation order for the shuffle;
+    and the oracle agrees on sampled rows."""
+    print(f"case s1r2 table window order={order}", flush=True)
+    _, T, shape = cyl
+    w = wl.s1r2_cylinder(n_omega=64, n_phi=16, n_z=32, seed=1002)
+    kw = dict(is_periodic=2, L=w.L, Rsim=w.Rsim, cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere,
+              s1r2_lookup=1, interp_order=order, s1r2_table=T, s1r2_shape=shape)
+    d, keep = steps.make_desc(steps.TOPO_S1R2, **kw)
+    F = steps.force_eval(d, w.x, w.M, w.soft)
+    # a subset that keeps the row-block count keeps the source splits, hence
+    # the summation order
+    Fs = steps.force_eval(d, w.x, w.M, w.soft, id_min=37, id_max=w.N - 1)
+    report(f"S1R2 window order={order} row subset vs all rows", rel_err(Fs, F[37:]))
+    assert np.array_equal(Fs, F[37:])
+    rows = np.linspace(0, w.N - 1, 48).astype(np.int32)
+    e = rel_err(F[rows], oracle.forces_rows(oracle.params(2, **kw), w.x, w.M, w.soft, rows))
+    report(f"S1R2 window order={order} N={w.N} sampled rows", e)
+    assert np.median(e) <= 1e-12 and e.max() <= 1e-9
+    perm = np.random.default_rng(11).permutation(w.N)
+    Fp = steps.force_eval(d, w.x[perm], w.M[perm], w.soft[perm])
+    ep = rel_err(Fp, F[perm])
+    report(f"S1R2 window order={order} shuffled vs input order", ep)
+    assert np.median(ep) <= 1e-12 and np.percentile(ep, 99) <= 1e-10
+
+
 @pytest.mark.parametrize("mode", ["nearest", "images"])
 def test_s1r2_radial_corrected_forces(steps, oracle, cyl, mode):
     print(f"case s1r2 {mode}", flush=True)
