@@ -915,9 +915,19 @@ __device__ __forceinline__ const float2 *s1r2_row32(const float2 *__restrict__ T
 }
 
 // s1r2_interp in FP32 (CIC, TSC): returns q = D_rho / rho and D_z.
-template <int ORDER>
+// An LDS copy of the FP32 S1R2 table rows for rho nodes w0 .. w0 + nw - 1.
+struct S1R2Win32 {
+    const float2 *win;
+    int w0, nw;
+    bool ok;
+};
+
+// WIN: the rows come from the window when the stencil's rho nodes all lie in
+// it, else from the global table (the same values either way).
+template <int ORDER, bool WIN = false>
 __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, const KScalars &k, float dx,
-                                                float dy, float dz, float &q, float &Dz) {
+                                                float dy, float dz, float &q, float &Dz,
+                                                const S1R2Win32 sw = S1R2Win32{}) {
     const int Nz = k.nz;
     const float rxy2 = fmaf(dx, dx, dy * dy);
     const float inv_rho = (rxy2 > 0.0f) ? __builtin_amdgcn_rsqf(rxy2) : 0.0f;
@@ -926,6 +936,7 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
     const float lastr = k.last_rf;
     float Dr, Dzv;
     static_assert(ORDER == 2 || ORDER == 4, "FP32 interpolation is CIC or TSC");
+    auto gather = [&](auto rowf) {
     if (ORDER == 2) {
         const float ur = fminf(rho * k.inv_drhof, lastr);
         const float urf = floorf(ur);
@@ -934,8 +945,8 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
         const float uzf = floorf(uz);
         const float fz = uz - uzf;
         const int iz0 = s1r2_zfix((int)uzf, -1, Nz - 1, Nz);
-        const float2 *r0 = s1r2_row32(T, k, ir0) + (iz0 + 1);
-        const float2 *r1 = s1r2_row32(T, k, ir0 + 1) + (iz0 + 1);
+        const float2 *r0 = rowf(ir0) + (iz0 + 1);
+        const float2 *r1 = rowf(ir0 + 1) + (iz0 + 1);
         const float2 t00 = r0[0], t01 = r0[1], t10 = r1[0], t11 = r1[1];
         const float a0 = fmaf(fr, t10.x - t00.x, t00.x), b0 = fmaf(fr, t11.x - t01.x, t01.x);
         const float a1 = fmaf(fr, t10.y - t00.y, t00.y), b1 = fmaf(fr, t11.y - t01.y, t01.y);
@@ -955,11 +966,24 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
         for (int row = 0; row < 3; ++row)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float2 t = s1r2_row32(T, k, jr - 1 + c)[jz + row];
+                const float2 t = rowf(jr - 1 + c)[jz + row];
                 const float w = wr[c] * wz[row];
                 Dr = fmaf(w, t.x, Dr);
                 Dzv = fmaf(w, t.y, Dzv);
             }
+    }
+    };
+    bool in = false;
+    if (WIN && sw.ok) {
+        const float ur = fminf(rho * k.inv_drhof, lastr);
+        const int lo = (ORDER == 2) ? (int)floorf(ur) : (int)floorf(ur + 0.5f) - 1;
+        in = lo >= sw.w0 && lo + (ORDER == 2 ? 1 : 2) < sw.w0 + sw.nw;
+    }
+    if (in) {
+        const unsigned nz3 = (unsigned)(k.nz + 3);
+        gather([&](int ir) { return sw.win + (unsigned)(ir - sw.w0) * nz3; });
+    } else {
+        gather([&](int ir) { return s1r2_row32(T, k, ir); });
     }
     q = Dr * inv_rho;
     Dz = Dzv;
@@ -984,10 +1008,10 @@ __device__ __forceinline__ float mixed_r2_z(const KScalars &k, const float4 pj, 
     return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
 }
 
-template <int V>
+template <int V, bool WIN = false>
 __device__ __forceinline__ void mixed_pair_z(const KScalars &k, const float2 *__restrict__ T, const float4 pj,
                                              float xr, float yr, float zr, float B, bool &near, float &ax,
-                                             float &ay, float &az) {
+                                             float &ay, float &az, const S1R2Win32 sw = S1R2Win32{}) {
     float dx, dy, dz;
     const float r2 = mixed_r2_z<V>(k, pj, xr, yr, zr, dx, dy, dz);
     const bool nr = r2 < B;  // left to the FP64 redo (r2 may be 0 here)
@@ -1002,7 +1026,7 @@ __device__ __forceinline__ void mixed_pair_z(const KScalars &k, const float2 *__
         az = fmaf(s, dz, az);
     } else {
         float q, Dz;
-        s1r2_interp_f32<s1r2_order(V)>(T, k, dx, dy, dz, q, Dz);
+        s1r2_interp_f32<s1r2_order(V), WIN>(T, k, dx, dy, dz, q, Dz, sw);
         const float sxy = m * (y3 - q);
         ax = fmaf(sxy, dx, ax);
         ay = fmaf(sxy, dy, ay);
@@ -1078,6 +1102,20 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
         fx[q] = fy[q] = fz[q] = 0.0;
     }
 
+    // TSC: the rho rows of each source tile in LDS, as in body_f64 (FP32
+    // nodes: 4992 of them, 39 KB, up to 38 z runs of 131), with the sources
+    // in input order (xy-compact tiles on column-ordered input).  At N =
+    // 524288: 300 -> 431 G pairs/s, pair-sum p99 error 1.5e-5 -> 2.0e-5
+    // (comoving term included).  CIC kept Morton tiles and global reads:
+    // with the window it ran at 582 against 681.
+    constexpr bool SWIN = V == PV_S1R2_TSC;
+    __shared__ __align__(16) float2 s1win[SWIN ? 2 * kS1R2WinNodes : 1];
+    __shared__ int s1rng[4];
+    if (SWIN) {
+        if (tid < 4) s1rng[tid] = (tid & 1) ? INT32_MIN : INT32_MAX;
+        __syncthreads();
+    }
+
     const int ntiles = (int)((j1 - j0 + kTile - 1) / kTile);  // 32-bit loop bounds: SALU tests
     for (int ti = 0; ti < ntiles; ++ti) {
         const int64_t t0 = j0 + (int64_t)ti * kTile;
@@ -1107,6 +1145,32 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
         const int64_t tend = min(j1, t0 + kTile);
         const int cnt = (int)(tend - t0);
         const float4 *__restrict__ tr = jrel + t0;
+        S1R2Win32 sw{s1win, 0, 0, false};
+        if (SWIN) {
+            static_assert(!SWIN || NI == 1, "one row per lane");
+            const double4 te = ext[t0 / kTile];
+            const double axd = fabs(xi[0] - c.x), ayd = fabs(yi[0] - c.y);
+            const double gx = fmax(axd - te.x, 0.0), gy = fmax(ayd - te.y, 0.0);
+            const double ex = axd + te.x, ey = ayd + te.y;
+            const int lo = (int)floor(fmin(sqrt(fma(gx, gx, gy * gy)) * k.inv_drho, k.last_r)) - 1;
+            const int hi = (int)floor(fmin(sqrt(fma(ex, ex, ey * ey)) * k.inv_drho, k.last_r)) + 2;
+            int *rg = s1rng + 2 * (ti & 1);
+            atomicMin(&rg[0], lo);
+            atomicMax(&rg[1], hi);
+            __syncthreads();  // ranges complete; every wave is done with the previous window
+            const int w0 = max(rg[0], -1), w1 = min(rg[1], k.nrho);
+            if (tid < 2) s1rng[2 * ((ti + 1) & 1) + tid] = tid ? INT32_MIN : INT32_MAX;  // next tile's slot
+            const unsigned nz3 = (unsigned)(k.nz + 3);
+            sw.w0 = w0;
+            sw.nw = w1 - w0 + 1;
+            sw.ok = (int64_t)sw.nw * nz3 <= 2 * kS1R2WinNodes;
+            if (sw.ok) {
+                const float2 *src = s1r2_row32(tab32, k, w0);
+                const unsigned nn = (unsigned)sw.nw * nz3;
+                for (unsigned o = tid; o < nn; o += kBlock) s1win[o] = src[o];
+            }
+            __syncthreads();  // the window is complete
+        }
         if (V == PV_S1R2_NEAREST || V == PV_T3_NEAREST) {
             // Far tile (decided in FP64, with margins far above the FP32
             // rounding of the displacements): every lane sees the whole tile
@@ -1149,13 +1213,13 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
             for (int u = 0; u < UNR; ++u)
 #pragma unroll
                 for (int q = 0; q < NI; ++q)
-                    mixed_pair_z<V>(k, tab32, pj[u], xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q]);
+                    mixed_pair_z<V, SWIN>(k, tab32, pj[u], xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q], sw);
         }
         for (; j < cnt; ++j) {
             const float4 pj = tr[j];
 #pragma unroll
             for (int q = 0; q < NI; ++q)
-                mixed_pair_z<V>(k, tab32, pj, xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q]);
+                mixed_pair_z<V, SWIN>(k, tab32, pj, xr[q], yr[q], zr[q], B[q], near, ax[q], ay[q], az[q], sw);
         }
         if (__builtin_expect(near, 0)) {
             for (int64_t jj = t0; jj < tend; ++jj) {
@@ -1876,14 +1940,16 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 bool is_mixed(const EvalPlan &p) { return p.precision == STEPS_PREC_MIXED && mixed_supported(p.variant); }
 
 // The mixed kernels read spatially ordered sources (Morton order; their FP32
-// offsets need compact tiles), the R3 and nearest-image FP64 kernels from
+// offsets need compact tiles) except the S1R2 TSC one, whose table window
+// needs tiles compact in xy (input order), the R3 and nearest-image FP64 kernels from
 // N = 65536 on (their far-tile test gains 7% for R3 at N = 2^21; below, the
 // ~16 launches of the ordering cost more than the pair kernel saves), the T3
 // Ewald kernel sources in z pencils.
 constexpr int64_t kR3SortMinN = 65536;
 bool sorted_sources(const EvalPlan &p) {
     const bool far_tiles = p.variant == PV_R3 || p.variant == PV_T3_NEAREST || p.variant == PV_S1R2_NEAREST;
-    return is_mixed(p) || (far_tiles && p.N >= kR3SortMinN) || p.variant == PV_T3_EWALD;
+    return (is_mixed(p) && p.variant != PV_S1R2_TSC) || (far_tiles && p.N >= kR3SortMinN) ||
+           p.variant == PV_T3_EWALD;
 }
 bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
 // The T3 Ewald kernel also takes its rows in that order when it evaluates
@@ -2083,9 +2149,11 @@ size_t tile_extra_bytes(const EvalPlan &p) {
     const int64_t nt = (p.N + kTile - 1) / kTile;
     const size_t n = (size_t)std::max<int64_t>(p.N, 1);
     size_t b = 2 * align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4));
-    if (sorted_sources(p)) {
+    if (sorted_sources(p) || is_mixed(p)) {  // FP32 records relative to the tiles, FP32 table
         b += align256(n * sizeof(float4));
         b += align256(table32_bytes(p));
+    }
+    if (sorted_sources(p)) {
         b += align256(n * sizeof(double4)) + align256(n * sizeof(double));
         b += align256(order_scratch_bytes(p.N));
     }
@@ -2122,9 +2190,15 @@ hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, d
                                v.jrel, v.tile_ext, p.N);
     } else {
         // tile extents too: the R3 kernel's far-tile test reads them (on unsorted
-        // tiles it rarely passes, and the per-pair loop runs)
-        hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles, v.jrel,
-                           v.tile_ext, p.N);
+        // tiles it rarely passes, and the per-pair loop runs), and so do the
+        // S1R2 table windows; the mixed TSC kernel takes its FP32 records
+        // relative to input-order tiles
+        if (is_mixed(p))
+            hipLaunchKernelGGL((tiles_kernel<true>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles, v.jrel,
+                               v.tile_ext, p.N);
+        else
+            hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles,
+                               v.jrel, v.tile_ext, p.N);
     }
     return hipGetLastError();
 }
