@@ -1484,8 +1484,6 @@ constexpr int kT3WinNodes32 = 3264;
 // 37.7 and 37.3; shares over an even count (1/2, 3/4) left the global blocks
 // on a few XCDs and ran at 26.  Rows padded to 256 bytes ran at 31.5.
 constexpr unsigned kT3WinNum32 = 2u, kT3WinDen32 = 3u;
-constexpr unsigned kT3WinPad32 = 1u;  // window rows start at multiples of this many nodes
-__host__ __device__ constexpr unsigned t3_win_stride32(unsigned nz) { return (nz + kT3WinPad32 - 1u) / kT3WinPad32 * kT3WinPad32; }
 
 struct T3Stencil32 {
     float wx[4], wy[4], wz[4], wxs[4], wys[4];
@@ -1592,7 +1590,7 @@ __device__ __forceinline__ void body_t3_mixed(const PairArgs &a, const int bx, c
     const double xi = p.x, yi = p.y, zi = p.z, hi = isoft[k.id_min + r];
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int Ng = k.ng, lo = k.t3_lo;
-    const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3), WS = t3_win_stride32(NZ);
+    const unsigned NY = (unsigned)(Ng - lo), NZ = (unsigned)(Ng + 3);
     int bxmin = 0, bxmax = 0, bymin = 0, bymax = 0;
     const bool win_share = (unsigned)bx % kT3WinDen32 < kT3WinNum32;
     if (WIN) {
@@ -1638,14 +1636,14 @@ __device__ __forceinline__ void body_t3_mixed(const PairArgs &a, const int bx, c
                     cur = pk;
                     wok = win_share && t3_win_axis(pk >> 10, bxmin, bxmax, Ng, lo, fx0, nxw) &&
                           t3_win_axis(pencil_y(kj, Ng), bymin, bymax, Ng, lo, fy0, nyw) &&
-                          (int64_t)nxw * nyw * WS <= kT3WinNodes32;
+                          (int64_t)nxw * nyw * NZ <= kT3WinNodes32;
                     __syncthreads();  // every wave is done with the previous window
                     if (wok) {
                         const unsigned nn = (unsigned)(nxw * nyw) * NZ;
                         for (unsigned o = tid; o < nn; o += kBlock) {
                             const unsigned row = o / NZ, z = o - row * NZ;
                             const unsigned wa = row / (unsigned)nyw, wc = row - wa * (unsigned)nyw;
-                            win[row * WS + z] = T32[(((unsigned)fx0 + wa) * NY + (unsigned)fy0 + wc) * NZ + z];
+                            win[o] = T32[(((unsigned)fx0 + wa) * NY + (unsigned)fy0 + wc) * NZ + z];
                         }
                     }
                     __syncthreads();  // the window is complete
@@ -1671,7 +1669,7 @@ __device__ __forceinline__ void body_t3_mixed(const PairArgs &a, const int bx, c
                     st,
                     [&](int i, int jj, float4 *n) {
                         const float4 *w0 =
-                            win + ((st.mx[i] - (unsigned)fx0) * (unsigned)nyw + (st.my[jj] - (unsigned)fy0)) * WS +
+                            win + ((st.mx[i] - (unsigned)fx0) * (unsigned)nyw + (st.my[jj] - (unsigned)fy0)) * NZ +
                             (unsigned)st.iz0;
 #pragma unroll
                         for (int q = 0; q < 4; ++q) n[q] = w0[q];
@@ -1970,7 +1968,7 @@ bool t3_window(const EvalPlan &p) {
 // The same for the mixed T3 kernel (16-byte nodes; Ng <= 127).
 bool t3_window_mixed(const EvalPlan &p) {
     return p.variant == PV_T3_EWALD && sorted_rows(p) && is_mixed(p) &&
-           (int64_t)t3_win_stride32((unsigned)(p.t3_ng + 3)) * 25 <= kT3WinNodes32;
+           (int64_t)(p.t3_ng + 3) * 25 <= kT3WinNodes32;
 }
 
 size_t s1r2_f32_entries(const EvalPlan &p) {
