@@ -37,6 +37,11 @@ def make_sims(oracle):
                                   mass_in_unit_sphere=c.mass_in_unit_sphere, s1r2_lookup=1, interp_order=2,
                                   s1r2_table=T, s1r2_shape=shape)
     sims.append((c, d, keep + [T]))
+    # TSC: the table-window kernels (FP64, and mixed with input-order sources)
+    d, keep = steps_amd.make_desc(steps_amd.TOPO_S1R2, is_periodic=2, L=c.L, Rsim=c.Rsim, cosmology=1, comoving=1,
+                                  mass_in_unit_sphere=c.mass_in_unit_sphere, s1r2_lookup=1, interp_order=4,
+                                  s1r2_table=T, s1r2_shape=shape)
+    sims.append((c, d, keep + [T]))
     return sims
 
 
