@@ -1290,7 +1290,8 @@ constexpr int kT3WinDoubles = 10080;  // 78.75 KB of LDS per workgroup: two work
 // G pairs/s, with bank-conflict stalls on 57% of LDS cycles; three component
 // runs, 20.9; this layout, 26.7.  Rows padded to 256 bytes (so that a change
 // of row keeps the bank pattern) fit fewer windows: 27.9 against 30.2 with
-// serpentine pencils.
+// serpentine pencils, and on the current kernel 30.3 against 34.4 (rows at
+// odd multiples of 128 bytes: 32.4).
 __host__ __device__ constexpr unsigned t3_win_stride(unsigned nz3) { return (nz3 + 1u) & ~1u; }
 
 // Layout nodes one axis of the window needs: sources in cell column s, rows
