@@ -47,10 +47,12 @@ enum steps_topology {
 
 enum steps_precision {
     STEPS_PREC_FP64 = 0, /* all pair arithmetic in FP64                      */
-    STEPS_PREC_MIXED = 1 /* FP32 pair arithmetic on tile-recentred positions
-                            (spatially ordered sources), FP64 accumulation
-                            across tiles, FP64 for pairs inside the softening
-                            bound; R3 and S1R2 (nearest z-image, CIC, TSC)    */
+    STEPS_PREC_MIXED = 1 /* FP32 pair arithmetic (tile-recentred positions,
+                            or for the T3 Ewald table FP64 displacements
+                            rounded once), FP64 accumulation across tiles,
+                            FP64 for pairs inside the softening bound; R3, T3
+                            (nearest image, Ewald) and S1R2 (nearest z-image,
+                            CIC, TSC)                                         */
 };
 
 /* Everything the reference force routines read from globals. */
