@@ -443,3 +443,45 @@ def test_s1r2_mixed_precision_error(steps, oracle, cyl, order):
         assert np.percentile(eo, 99) <= bound
         assert np.median(e) <= 2e-6
         assert not np.array_equal(Fmx, F64)     # the FP32 path really ran
+
+
+@pytest.mark.parametrize("N", [2, 65, 1000])
+@pytest.mark.parametrize("mode", ["t3_ewald", "t3_nearest", "s1r2_cic", "s1r2_tsc"])
+def test_periodic_ragged_sizes(steps, oracle, t3_tab, cyl, mode, N):
+    """Sizes that fill neither a 64-source tile nor a 256-row block (the
+    clamped lanes and partial tiles of the window kernels): FP64 against the
+    oracle on every row, mixed against FP64."""
+    print(f"case periodic ragged {mode} N={N}", flush=True)
+    rng = np.random.default_rng(N)
+    M = rng.uniform(0.5, 2.0, N)
+    if mode.startswith("t3"):
+        L = L_T3
+        x = rng.uniform(0.0, L, (N, 3))
+        h = np.full(N, 0.02 * L / 64)
+        ip = 2 if mode == "t3_ewald" else 1
+        tab = t3_tab if ip == 2 else None
+        kw = dict(is_periodic=ip, L=L, t3_table=tab)
+        d, keep = steps.make_desc(steps.TOPO_T3, **kw)
+        dm, keepm = steps.make_desc(steps.TOPO_T3, precision=steps.PREC_MIXED, **kw)
+        p = oracle.params(1, interp_order=4, **kw)
+    else:
+        w, T, shape = cyl
+        r = w.Rsim * np.sqrt(rng.uniform(0.0, 1.0, N))
+        phi = rng.uniform(0.0, 2 * np.pi, N)
+        x = np.stack([r * np.cos(phi), r * np.sin(phi), rng.uniform(-w.L / 2, w.L / 2, N)], axis=1)
+        h = np.full(N, 0.5)
+        kw = dict(is_periodic=2, L=w.L, Rsim=w.Rsim, s1r2_lookup=1, interp_order=2 if mode == "s1r2_cic" else 4,
+                  s1r2_table=T, s1r2_shape=shape)
+        d, keep = steps.make_desc(steps.TOPO_S1R2, **kw)
+        dm, keepm = steps.make_desc(steps.TOPO_S1R2, precision=steps.PREC_MIXED, **kw)
+        p = oracle.params(2, **kw)
+    F = steps.force_eval(d, x, M, h)
+    e = rel_err(F, oracle.forces(p, x, M, h))
+    report(f"{mode} N={N} vs oracle", e)
+    assert np.median(e) <= 1e-12 and e.max() <= 1e-9
+    Fm = steps.force_eval(dm, x, M, h)
+    em = rel_err(Fm, F)
+    report(f"{mode} N={N} mixed vs fp64", em)
+    assert np.isfinite(Fm).all()
+    if N >= 65:  # with two particles one nearly cancelling force would decide the statistic
+        assert np.median(em) <= 1e-5 and np.percentile(em, 90) <= 1e-4
