@@ -9,7 +9,9 @@ are already resident in HBM.  The default workload is the compactified R^3
 configuration at N = 2^21 (seed 1003): the FP64 path is the headline value;
 the mixed-precision path, its accuracy against FP64, and the CPU baseline
 (the reference-faithful OpenMP restatement on this host's cores, on a bounded
-sample of rows) are reported on the same JSON line.
+sample of rows) are reported on the same JSON line, beside the periodic
+variants, the N = 16384 force-plus-KDK-steps case, the serving loop and the
+host-buffer entry point.
 """
 from __future__ import annotations
 
@@ -244,6 +246,38 @@ def variants_leg(reps: int) -> dict:
     return out
 
 
+def c0_kdk_leg(nsteps: int = 10) -> dict:
+    """Compactified R3 at N = 16384 (seed 1000): one force evaluation, then
+    `nsteps` KDK steps on the resident simulation (step.cc:100-312: kick,
+    drift, force, kick; each step returns errmax to the host, which the next
+    time step needs), FP64 and mixed; wall clock per step."""
+    w = wl.r3_compactified(N=16384, seed=1000)
+    kw = dict(cosmology=w.cosmology, comoving=w.comoving, mass_in_unit_sphere=w.mass_in_unit_sphere)
+    hub = lambda a: wl.H0 * math.sqrt(wl.OMEGA_M * a ** -3 + wl.OMEGA_L)
+    out = {"N": w.N, "kdk_steps": nsteps}
+    for name, prec in (("fp64", steps_amd.PREC_FP64), ("mixed", steps_amd.PREC_MIXED)):
+        d, keep = steps_amd.make_desc(steps_amd.TOPO_R3, precision=prec, **kw)
+        sim = steps_amd.Sim(d, w.N)
+        sim.set_state(w.x, w.M, w.soft, w.v)
+        sim.forces()
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.forces()
+        sim.synchronize()
+        t_force = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for k in range(nsteps):
+            a1 = w.a_start * (1 + 1e-3 * k)
+            a2 = w.a_start * (1 + 1e-3 * (k + 1))
+            sim.step(1e-4, a1, hub(a1), a2, hub(a2), w.rho_part)
+        dt = time.perf_counter() - t0
+        sim.close()
+        out[name] = {"force_ms": t_force * 1e3, "ms_per_step": dt * 1e3 / nsteps, "steps_per_s": nsteps / dt,
+                     "g_pairs_per_s": float(w.N) ** 2 * nsteps / dt / 1e9}
+    print(f"[bench] c0 KDK: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def host_path_leg(w, desc, calls: int) -> dict:
     """The host-buffer entry point (steps_force_eval, the reference routine's
     boundary): host positions, masses and softening lengths in, forces out,
@@ -373,6 +407,7 @@ def main() -> int:
 
     if not args.no_variants:
         out["variants"] = variants_leg(max(1, min(args.steps, 3)))
+        out["c0_kdk"] = c0_kdk_leg()
 
     if not args.no_serving:
         out["serving"] = serving_leg(args.serving_requests)
