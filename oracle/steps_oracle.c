@@ -627,6 +627,53 @@ double oracle_cylindrical_force_correction(double r, double R, const double *T, 
     return c;
 }
 
+/* utils.cc:39-56: Lagrange interpolation through three / four nodes */
+static double quad_interp(double X, double X1, double Y1, double X2, double Y2, double X3, double Y3)
+{
+    double L1 = (X - X2) * (X - X3) / ((X1 - X2) * (X1 - X3));
+    double L2 = (X - X1) * (X - X3) / ((X2 - X1) * (X2 - X3));
+    double L3 = (X - X1) * (X - X2) / ((X3 - X1) * (X3 - X2));
+    return Y1 * L1 + Y2 * L2 + Y3 * L3;
+}
+
+static double cubic_interp(double X, double X1, double Y1, double X2, double Y2, double X3, double Y3,
+                           double X4, double Y4)
+{
+    double L1 = (X - X2) * (X - X3) * (X - X4) / ((X1 - X2) * (X1 - X3) * (X1 - X4));
+    double L2 = (X - X1) * (X - X3) * (X - X4) / ((X2 - X1) * (X2 - X3) * (X2 - X4));
+    double L3 = (X - X1) * (X - X2) * (X - X4) / ((X3 - X1) * (X3 - X2) * (X3 - X4));
+    double L4 = (X - X1) * (X - X2) * (X - X3) / ((X4 - X1) * (X4 - X2) * (X4 - X3));
+    return Y1 * L1 + Y2 * L2 + Y3 * L3 + Y4 * L4;
+}
+
+/* forces.cc:279-316: the CPU correction of order 1, 2 or 3 (nodes assumed at
+ * R/size * i, as on the GPU).  Order 2 shifts its three nodes down by one at
+ * i == size-2; order 3 shifts its four nodes down by one at i == size-3 and
+ * returns the last entry from i == size-2 on.  Every force path of the
+ * reference calls order 1 (forces.cc:1136,1177,1296,1388). */
+double oracle_cylindrical_force_correction_order(double r, double R, const double *T, int size, int order)
+{
+    double step = R / (double)size;
+    int i = (int)floor(r / R * (size - 1));
+    double c = T[size - 1];
+    if (order == 1) {
+        if (i < size - 1) c = lin_interp(r, step * i, T[i], step * (i + 1), T[i + 1]);
+    } else if (order == 2) {
+        if (i < size - 2)
+            c = quad_interp(r, step * i, T[i], step * (i + 1), T[i + 1], step * (i + 2), T[i + 2]);
+        else if (i == size - 2)
+            c = quad_interp(r, step * (i - 1), T[i - 1], step * i, T[i], step * (i + 1), T[i + 1]);
+    } else if (order == 3) {
+        if (i < size - 3)
+            c = cubic_interp(r, step * i, T[i], step * (i + 1), T[i + 1], step * (i + 2), T[i + 2],
+                             step * (i + 3), T[i + 3]);
+        else if (i == size - 3)
+            c = cubic_interp(r, step * (i - 1), T[i - 1], step * i, T[i], step * (i + 1), T[i + 1],
+                             step * (i + 2), T[i + 2]);
+    }
+    return c;
+}
+
 /* ------------------------------------------------------------------------ */
 /* direct sums                                                               */
 /* ------------------------------------------------------------------------ */

@@ -18,19 +18,27 @@
  *   t3 table + forces      antisymmetry, O_h symmetry, independent Ewald sum
  *                          with another alpha, perfect-lattice zero force,
  *                          Catmull-Rom node reproduction
- *   s1r2 table + forces    direct real-space z-image sum, interpolation
- *                          weights, node reproduction
+ *   s1r2 table + forces    direct real-space z-image sum; an exact-erfc
+ *                          numpy 1P Ewald (itself checked against the image
+ *                          sum) to 1e-11 max|D| where the A&S fit carries no
+ *                          weight, within the fit's bound elsewhere; uniform
+ *                          periodic cylinder (comoving term cancels the pair
+ *                          sum in every mode); CIC bilinear / TSC linear data
+ *                          off the nodes, NGP half-way rounding
  *   radial table           independent adaptive quadrature of the integrand,
- *                          long-cylinder limit
- *   softening setup, kdk   closed forms, two-body orbit
+ *                          long-cylinder limit; its correction of order 1-3
+ *                          reproduces linear / quadratic / cubic data
+ *   softening setup, kdk   closed forms, two-body orbit, force-free Hubble
+ *                          drag against v a^2 = const and x(t) by quadrature,
+ *                          periodic wraps
  *   r3 fp32 stand-in      two-body case exact to float rounding, error
  *                          against the FP64 oracle within float bounds and
  *                          growing with N (tests/test_oracle_r3.py); it is a
  *                          comparison point, not a parity oracle
- *   fast_erfcx branch of the S1R2 table (ewald_space.cc:539-561): parity
- *                          unpinned beyond the direct-image-sum bound, because
- *                          the reference's own A&S 7.1.26 fit is only
- *                          accurate to ~1e-7 and no reference output exists.
+ *   fast_erfcx             the published A&S 7.1.26 bound (1.5e-7 on erfc)
+ *                          and its reflection for x < 0; its contribution to
+ *                          the S1R2 table is bounded node by node (<= ~1e-10
+ *                          max|D| at the S1R2 workload's box).
  */
 #ifndef STEPS_ORACLE_H
 #define STEPS_ORACLE_H
@@ -130,6 +138,9 @@ void oracle_cylindrical_force_table(double *table, double R, double Lz, int size
                                     int accuracy);
 double oracle_cylindrical_force_correction(double r, double R, const double *table,
                                            int size);
+/* the CPU twin with interpolation order 1, 2 or 3 (forces.cc:279-316) */
+double oracle_cylindrical_force_correction_order(double r, double R, const double *table,
+                                                 int size, int order);
 
 /* softening set-up: h_i = cbrt(M_i*3/(4 pi rho_part)) */
 void oracle_softening_lengths(const double *M, int N, double particle_radius,

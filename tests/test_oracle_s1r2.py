@@ -2,15 +2,16 @@
 forces_cuda.cu:177-455, :652-864, utils.cc:162-228) independently.
 
 * The Tornberg 1P Ewald table against a direct real-space sum over 2*10^5
-  z-images written here in numpy.  They agree to ~2e-7 of max|D|: that is the
-  accuracy of the reference's own Ewald sum (its erfcx is the A&S 7.1.26 fit,
-  good to ~1e-7, ewald_space.cc:539-561), so below that level the table is
-  "parity unpinned" (no reference output exists to compare with).
+  z-images written here in numpy, to 1e-6 of max|D| (the finer pins, an
+  exact-erfc numpy Ewald at 1e-11 max|D| and the A&S fit's bound, are in
+  test_oracle_pins.py).
 * The reference's real-space table mode (PERIODIC_Z_RSPACELOOKUP) against the
   same numpy sum truncated identically (exact).
 * NGP/CIC reproduce table nodes; TSC reproduces linear data.
-* The radial (finite-cylinder) table against adaptive quadrature.
+* The radial (finite-cylinder) table against adaptive quadrature; its
+  correction of order 1-3 (forces.cc:279-316) against polynomial data.
 """
+import ctypes as C
 import math
 
 import numpy as np
@@ -185,3 +186,32 @@ def test_radial_correction_interpolation(oracle):
         else:
             expect = T[-1]
         assert c == pytest.approx(expect, rel=1e-13)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_radial_correction_orders_reproduce_polynomials(oracle, order):
+    """forces.cc:279-316: the order-k correction is Lagrange interpolation on
+    the nodes R/size * i, so it reproduces polynomials of degree k exactly;
+    past its last full stencil it shifts the stencil down (order 2 at
+    i = size-2, order 3 at i = size-3) or returns the last entry."""
+    lib = oracle.lib()
+    fn = lib.oracle_cylindrical_force_correction_order
+    fn.restype = C.c_double
+    fn.argtypes = [C.c_double, C.c_double, oracle._DP, C.c_int, C.c_int]
+    R, size = 50.0, 40
+    step = R / size
+    coef = [0.7, -0.03, 0.004, -1.1e-4][: order + 1]
+    poly = lambda r: sum(c * r ** k for k, c in enumerate(coef))
+    T = np.array([poly(step * i) for i in range(size)])
+    rng = np.random.default_rng(order)
+    last_full = {1: size - 2, 2: size - 2, 3: size - 3}[order]   # highest i with a stencil
+    for r in rng.uniform(0.0, R, 400):
+        i = int(math.floor(r / R * (size - 1)))
+        c = fn(r, R, oracle.dp(T), size, order)
+        if i <= last_full:
+            assert c == pytest.approx(poly(r), rel=1e-11, abs=1e-13), (r, i)
+        else:
+            assert c == T[-1], (r, i)
+    # the order-1 function is the one the force paths use
+    for r in (0.0, 3.3, 49.0, 60.0):
+        assert fn(r, R, oracle.dp(T), size, 1) == lib.oracle_cylindrical_force_correction(r, R, oracle.dp(T), size)
