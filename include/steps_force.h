@@ -130,7 +130,11 @@ int steps_sim_last_kernel_ms(steps_sim *s, float *pair_ms, float *total_ms);
  * errmax = max_i |A'_i| / cbrt(M_i 3/(4 pi rho_part)).  a_kick1/hubble1 are the
  * scale factor and H(a) before the step, a_kick2/hubble2 after the host's
  * Friedmann update (step.cc:233-243); G is +1.  The stored F must hold the
- * forces of the current positions (after steps_sim_forces). */
+ * forces of the current positions for all N rows: after steps_sim_forces(s,
+ * 0, N-1) or a previous step; otherwise (no evaluation yet, a sub-range
+ * evaluation, new positions) STEPS_ERR_STATE.  After a step F holds the
+ * step's own full-range forces, so steps_sim_get_forces works without
+ * another evaluation; steps_sim_last_kernel_ms times only steps_sim_forces. */
 typedef struct steps_step_params {
     double h;
     double a_kick1, hubble1;
@@ -139,7 +143,8 @@ typedef struct steps_step_params {
     double rho_part;
 } steps_step_params;
 int steps_sim_step(steps_sim *s, const steps_step_params *p, double *errmax);
-/* errmax of the stored forces without stepping (step.cc:35-98). */
+/* errmax of the stored forces without stepping (step.cc:35-98); needs the
+ * full-range forces as steps_sim_step does. */
 int steps_sim_errmax(steps_sim *s, double a, double hubble, double G, double rho_part,
                      double *errmax);
 

@@ -299,6 +299,7 @@ class Sim:
         p = StepParams(h, a_kick1, hubble1, a_kick2, hubble2, G, rho_part)
         e = C.c_double()
         _check(self._lib.steps_sim_step(self.h, C.byref(p), C.byref(e)))
+        self._last_rows = self.N  # F now holds the step's full-range forces
         return e.value
 
     def errmax(self, a, hubble, rho_part, G=1.0) -> float:

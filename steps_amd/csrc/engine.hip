@@ -64,19 +64,33 @@ void DevBuf::release() {
 }
 
 namespace {
+// Hash of every word of a host table (four independent multiply-rotate lanes
+// in the manner of xxHash64's round, then folded): a table edited in place
+// between calls, at any entry, gets a new fingerprint and is uploaded again.
+// It reads the table once (about 1 ms for the 6 MB T3 table of IS_PERIODIC 2),
+// which is small next to the O(N^2) evaluation that follows.
 uint64_t fingerprint(const double *h, size_t n) {
-    uint64_t f = 1469598103934665603ull ^ (uint64_t)n;
-    const size_t step = std::max<size_t>(1, n / 4096);
-    for (size_t i = 0; i < n; i += step) {
-        uint64_t b;
-        std::memcpy(&b, &h[i], sizeof b);
-        f = (f ^ b) * 1099511628211ull;
+    constexpr uint64_t P1 = 11400714785074694791ull, P2 = 14029467366897019727ull, P3 = 1609587929392839161ull;
+    auto rotl = [](uint64_t x, int r) { return (x << r) | (x >> (64 - r)); };
+    auto round = [&](uint64_t acc, uint64_t w) { return rotl(acc + w * P2, 31) * P1; };
+    uint64_t a[4] = {P1 + P2, P2, 0, 0 - P1};
+    const unsigned char *b = reinterpret_cast<const unsigned char *>(h);
+    size_t i = 0;
+    for (; i + 4 <= n; i += 4) {
+        uint64_t w[4];
+        std::memcpy(w, b + i * 8, sizeof w);
+        for (int l = 0; l < 4; ++l) a[l] = round(a[l], w[l]);
     }
-    if (n) {
-        uint64_t b;
-        std::memcpy(&b, &h[n - 1], sizeof b);
-        f = (f ^ b) * 1099511628211ull;
+    uint64_t f = rotl(a[0], 1) + rotl(a[1], 7) + rotl(a[2], 12) + rotl(a[3], 18);
+    for (; i < n; ++i) {
+        uint64_t w;
+        std::memcpy(&w, b + i * 8, sizeof w);
+        f = rotl(f ^ round(0, w), 27) * P1 + P3;
     }
+    f ^= (uint64_t)n * P3;
+    f ^= f >> 33;
+    f *= P2;
+    f ^= f >> 29;
     return f;
 }
 }  // namespace
@@ -422,7 +436,10 @@ int sim_upload(steps_sim *s, const double *x, const double *v, const double *M, 
     if (F) STEPS_HIP(hipMemcpyAsync(s->F.ptr, F, 3 * nb, hipMemcpyHostToDevice, s->stream), "copy F");
     else STEPS_HIP(hipMemsetAsync(s->F.ptr, 0, 3 * nb, s->stream), "zero F");
     STEPS_HIP(hipStreamSynchronize(s->stream), "upload state");
-    s->evaluated = false;
+    // forces handed in with the state are the full-range forces of these positions
+    s->evaluated = s->full = (F != nullptr);
+    s->last_nrows = F ? s->N : 0;
+    s->timed = false;
     return STEPS_OK;
 }
 
@@ -567,7 +584,7 @@ int steps_sim_set_positions(steps_sim *s, const double *x) {
     STEPS_HIP(hipMemcpyAsync(s->stage.ptr, x, 3 * nb, hipMemcpyHostToDevice, s->stream), "copy x");
     STEPS_HIP(launch_pack_positions(s->stage.as<double>(), s->N, s->dp, s->stream), "pack positions");
     STEPS_HIP(hipStreamSynchronize(s->stream), "set positions");
-    s->evaluated = false;
+    s->evaluated = s->full = s->timed = false;
     return STEPS_OK;
 }
 
@@ -582,9 +599,11 @@ int steps_sim_forces(steps_sim *s, int64_t id_min, int64_t id_max) {
     int rc = sim_plan(s, id_min, id_max, s->desc.precision, p);
     if (rc) return rc;
     STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
+    s->evaluated = s->full = s->timed = false;  // F is overwritten from here on
     STEPS_HIP(sim_enqueue_forces(s, p, s->stream, true), "force kernels");
     s->last_nrows = p.nrows;
-    s->evaluated = true;
+    s->evaluated = s->timed = true;
+    s->full = p.id_min == 0 && p.nrows == s->N;
     return STEPS_OK;
 }
 
@@ -608,7 +627,7 @@ int steps_sim_synchronize(steps_sim *s) {
 
 int steps_sim_last_kernel_ms(steps_sim *s, float *pair_ms, float *total_ms) {
     if (!s) return fail(STEPS_ERR_ARG, "null sim");
-    if (!s->evaluated) return fail(STEPS_ERR_STATE, "no force evaluation queued");
+    if (!s->timed) return fail(STEPS_ERR_STATE, "no timed force evaluation queued (steps carry no timing events)");
     STEPS_HIP(hipEventSynchronize(s->ev2), "event sync");
     float a = 0, b = 0;
     STEPS_HIP(hipEventElapsedTime(&a, s->ev0, s->ev1), "elapsed");
@@ -625,10 +644,15 @@ int steps_sim_step(steps_sim *s, const steps_step_params *sp, double *errmax) {
     int rc = sim_plan(s, 0, s->N - 1, s->desc.precision, p);
     if (rc) return rc;
     if ((rc = sim_fill_params(s, *sp))) return rc;
+    // step.cc:128-140 kicks with the forces of the current positions, all N rows
+    if (!s->full)
+        return fail(STEPS_ERR_STATE, "a KDK step needs the full-range forces of the current positions "
+                                     "(call steps_sim_forces(s, 0, N-1) first)");
+    s->evaluated = s->full = s->timed = false;
     STEPS_HIP(sim_enqueue_step(s, p, s->stream), "KDK step");
     STEPS_HIP(hipStreamSynchronize(s->stream), "KDK step");
     s->last_nrows = p.nrows;
-    s->evaluated = false;  // the step's force kernels carry no timing events
+    s->evaluated = s->full = true;  // the step's second force evaluation, all rows (no timing events)
     if (errmax) *errmax = *s->herrmax;
     return STEPS_OK;
 }
@@ -636,6 +660,7 @@ int steps_sim_step(steps_sim *s, const steps_step_params *sp, double *errmax) {
 int steps_sim_errmax(steps_sim *s, double a, double hubble, double G, double rho_part, double *errmax) {
     // step.cc:35-98: errmax of the stored forces, no kick
     if (!s) return fail(STEPS_ERR_ARG, "null sim");
+    if (!s->full) return fail(STEPS_ERR_STATE, "errmax needs the full-range forces of the current positions");
     STEPS_HIP(hipSetDevice(s->device), "hipSetDevice");
     steps_step_params sp{0.0, a, hubble, a, hubble, G, rho_part};
     int rc = sim_fill_params(s, sp);

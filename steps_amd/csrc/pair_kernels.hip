@@ -16,8 +16,10 @@
 //     replaces the per-pair h_i + h_j test, and the exact cubic-spline branch
 //     (r < h_i + h_j, forces_cuda.cu:466-519, the reference's own formula)
 //     runs only when some lane of the wave is that close;
-//   * 1/r^3 uses v_rsq_f64 (2^-24 accurate on gfx950) plus one second-order
-//     Newton correction.
+//   * 1/r^3 uses v_rsq_f64 (2^-24 accurate on gfx950) plus one first-order
+//     Newton correction; the R3 and nearest-image FP64 kernels read source
+//     copies whose mass slot holds -1.5 m, which folds the mass into that
+//     correction (mr3_scaled: 5 VALU instead of 6).
 #include "internal.h"
 
 #include <algorithm>
@@ -44,6 +46,8 @@ struct KScalars {
     // Newton-correction constant, passed in so that it lives in a register
     // for the whole loop instead of being rematerialised per pair
     double c15;
+    // -5/3, the constant of mr3_scaled
+    double cm53;
     // reciprocals of the table grid spacings: T3 Ng/L, S1R2 1/drho and Nz/Lz
     double inv_gs, inv_drho, inv_dzg;
     double last_r;  // S1R2: index of the last rho node, Nrho - 1 (Nrho >= 2)
@@ -104,6 +108,23 @@ __device__ __forceinline__ double inv_r3(double r2, double c15) {
     const double e = fma(-r2, t, 1.0);
     return (t * y) * fma(e, c15, 1.0);
 }
+
+// m r2^(-3/2) for a source whose record carries mp = -1.5 m (the FP64 R3 and
+// nearest-image kernels' source copies, tiles_kernel): with y = rsq(r2) and
+// t = y^2, mp y^3 (r2 t - 5/3) = m y^3 (2.5 - 1.5 r2 t) = m y^3 (1 + 1.5 e),
+// e = 1 - r2 t, which is inv_r3's first-order correction times m, in 5 VALU
+// (y m', t (y m'), r2 t - 5/3, the product, and t) instead of inv_r3's 5 plus
+// the mass multiply.  r2 t - 5/3 lies near -2/3, so its rounding adds 1 ulp.
+__device__ __forceinline__ double mr3_scaled(double r2, double mp, double cm53) {
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double t = y * y;
+    const double b = t * (y * mp);
+    return b * fma(r2, t, cm53);
+}
+
+// The mass of a source whose record carries -1.5 m (within 2 ulp of m); only
+// the rare pairs inside the softening length need it.
+__device__ __forceinline__ double unscaled_mass(double mp) { return mp * (-2.0 / 3.0); }
 
 // forces_cuda.cu:466-519, written out as the reference does (beta powers and
 // divisions); only reached for r < beta, so its cost does not matter.
@@ -419,6 +440,17 @@ __device__ __forceinline__ double pair_w_tile(double r2, double B, double hi,
     return w;
 }
 
+// pair_w_tile times the source mass, for a source carrying mp = -1.5 m.
+__device__ __forceinline__ double pair_s_tile(double r2, double B, double hi, double mp,
+                                              const double *__restrict__ soft, int64_t j, const KScalars &k) {
+    double s = mr3_scaled(r2, mp, k.cm53);
+    if (r2 < B) {
+        const double beta = hi + soft[j];
+        if (r2 < beta * beta) s = unscaled_mass(mp) * softening_exact(sqrt(r2), beta);
+    }
+    return s;
+}
+
 // An LDS copy of the S1R2 table rows for rho nodes w0 .. w0 + nw - 1 (each
 // the full z run of the device layout), valid when ok.
 struct S1R2Win {
@@ -429,8 +461,10 @@ struct S1R2Win {
 
 // One pair (i = lane row, j = uniform source) for variant V, FP64.  WIN: an
 // S1R2 lookup pair reads its rho rows from the window when they all lie in
-// it, else from the global table (the same values either way).
-template <int V, bool WIN = false>
+// it, else from the global table (the same values either way).  MS: the
+// source record carries -1.5 m instead of m (R3 and nearest-image variants;
+// the FP64 kernel's source copies, mr3_scaled).
+template <int V, bool WIN = false, bool MS = false>
 __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__restrict__ table,
                                          const double *__restrict__ soft, const int64_t j,
                                          const double4 pj, const double xi, const double yi,
@@ -439,7 +473,7 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
     double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
     if (V == PV_R3) {
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double s = pj.w * pair_w_tile(r2, B, hi, soft, j, k);
+        const double s = MS ? pair_s_tile(r2, B, hi, pj.w, soft, j, k) : pj.w * pair_w_tile(r2, B, hi, soft, j, k);
         fx = fma(s, dx, fx);
         fy = fma(s, dy, fy);
         fz = fma(s, dz, fz);
@@ -449,13 +483,13 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
         if (fabs(dy) > k.halfL) dy -= copysign(k.L, dy);
         if (fabs(dz) > k.halfL) dz -= copysign(k.L, dz);
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double w = pair_w_tile(r2, B, hi, soft, j, k);
         if (V == PV_T3_NEAREST) {
-            const double s = pj.w * w;
+            const double s = MS ? pair_s_tile(r2, B, hi, pj.w, soft, j, k) : pj.w * pair_w_tile(r2, B, hi, soft, j, k);
             fx = fma(s, dx, fx);
             fy = fma(s, dy, fy);
             fz = fma(s, dz, fz);
         } else {
+            const double w = pair_w_tile(r2, B, hi, soft, j, k);
             double Dx, Dy, Dz;
             t3_interp(table, k, dx, dy, dz, Dx, Dy, Dz);
             fx = fma(pj.w, fma(w, dx, -Dx), fx);
@@ -483,13 +517,13 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
         if (dz > k.halfL) dz -= k.L;
         else if (dz < -k.halfL) dz += k.L;
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double w = pair_w_tile(r2, B, hi, soft, j, k);
         if (V == PV_S1R2_NEAREST) {
-            const double s = pj.w * w;
+            const double s = MS ? pair_s_tile(r2, B, hi, pj.w, soft, j, k) : pj.w * pair_w_tile(r2, B, hi, soft, j, k);
             fx = fma(s, dx, fx);
             fy = fma(s, dy, fy);
             fz = fma(s, dz, fz);
         } else {
+            const double w = pair_w_tile(r2, B, hi, soft, j, k);
             // m (w d - D) with D_x = q dx, D_y = q dy: x and y share m (w - q)
             constexpr int ORD = (V == PV_S1R2_NGP) ? 0 : ((V == PV_S1R2_CIC) ? 2 : 4);
             double rho, inv_rho, uz, Dr, Dzv;
@@ -521,12 +555,12 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
 // scalar loads and are applied to the NI rows of the lane.
 // ---------------------------------------------------------------------------
 // R3 pair of a tile that no row of the wave can reach within its softening
-// bound: the plain r^-3 law, no per-pair test.
+// bound: the plain r^-3 law, no per-pair test (pj.w = -1.5 m).
 __device__ __forceinline__ void pair_r3_far(const KScalars &k, const double4 pj, const double xi, const double yi,
                                             const double zi, double &fx, double &fy, double &fz) {
     const double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const double s = pj.w * inv_r3(r2, k.c15);
+    const double s = mr3_scaled(r2, pj.w, k.cm53);
     fx = fma(s, dx, fx);
     fy = fma(s, dy, fy);
     fz = fma(s, dz, fz);
@@ -568,7 +602,7 @@ __device__ __forceinline__ void pair_far_shift(const KScalars &k, const double4 
                                                double &fx, double &fy, double &fz) {
     const double dx = (pj.x - xi) + sx, dy = (pj.y - yi) + sy, dz = (pj.z - zi) + sz;
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const double s = pj.w * inv_r3(r2, k.c15);
+    const double s = mr3_scaled(r2, pj.w, k.cm53);  // pj.w = -1.5 m
     fx = fma(s, dx, fx);
     fy = fma(s, dy, fy);
     fz = fma(s, dz, fz);
@@ -627,6 +661,8 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
     // between tiles so that one barrier closes the atomics and the previous
     // tile's reads of the window
     constexpr bool SWIN = V == PV_S1R2_CIC || V == PV_S1R2_TSC;
+    // sources carry -1.5 m (scaled_sources, mr3_scaled)
+    constexpr bool MSC = V == PV_R3 || V == PV_T3_NEAREST || V == PV_S1R2_NEAREST;
     __shared__ __align__(16) double2 s1win[SWIN ? kS1R2WinNodes : 1];
     __shared__ int s1rng[4];
     if (SWIN) {
@@ -744,14 +780,14 @@ __device__ __forceinline__ void body_f64(const PairArgs &a, const int bx, const 
             for (int u = 0; u < UNR; ++u)
 #pragma unroll
                 for (int q = 0; q < NI; ++q)
-                    pair_f64<V, SWIN>(k, table, soft, t0 + u0 + u, pj[u], xi[q], yi[q], zi[q], hi[q], B[q], fx[q],
+                    pair_f64<V, SWIN, MSC>(k, table, soft, t0 + u0 + u, pj[u], xi[q], yi[q], zi[q], hi[q], B[q], fx[q],
                                       fy[q], fz[q], sw);
         }
         for (; u0 < cnt; ++u0) {
             const double4 pj = tj[u0];
 #pragma unroll
             for (int q = 0; q < NI; ++q)
-                pair_f64<V, SWIN>(k, table, soft, t0 + u0, pj, xi[q], yi[q], zi[q], hi[q], B[q], fx[q], fy[q], fz[q],
+                pair_f64<V, SWIN, MSC>(k, table, soft, t0 + u0, pj, xi[q], yi[q], zi[q], hi[q], B[q], fx[q], fy[q], fz[q],
                                   sw);
         }
     }
@@ -1851,13 +1887,16 @@ __global__ void unpack_positions_kernel(const double4 *__restrict__ jrec, double
 
 // One wave64 per source tile: the tile's bounding-box centre and largest
 // softening length (double4 (cx, cy, cz, h_max)); with MIXED also the FP32
-// offsets of each source from that centre plus its mass.
+// offsets of each source from that centre plus its mass.  A non-null `scaled`
+// receives each source as (x, y, z, -1.5 m), the records mr3_scaled reads; it
+// may be jrec itself (each lane reads its record before it stores, and the
+// padding lanes that re-read the first record of a tile store nothing).
 template <bool MIXED>
-__global__ __launch_bounds__(kTile) void tiles_kernel(const double4 *__restrict__ jrec,
+__global__ __launch_bounds__(kTile) void tiles_kernel(const double4 *jrec,
                                                      const double *__restrict__ soft,
                                                      double4 *__restrict__ tiles,
                                                      float4 *__restrict__ jrel, double4 *__restrict__ ext,
-                                                     int64_t n) {
+                                                     double4 *scaled, int64_t n) {
     const int64_t t = blockIdx.x;
     const int lane = threadIdx.x;
     const int64_t i = t * kTile + lane;
@@ -1883,6 +1922,7 @@ __global__ __launch_bounds__(kTile) void tiles_kernel(const double4 *__restrict_
                               (hi[2] - lo[2]) * (0.5 + 1e-15), 0.0);
     if (MIXED && ok)
         jrel[i] = make_float4((float)(p.x - cx), (float)(p.y - cy), (float)(p.z - cz), (float)p.w);
+    if (scaled && ok) scaled[i] = make_double4(p.x, p.y, p.z, -1.5 * p.w);
 }
 
 // Sources per inner-loop step (scalar-loaded together), chosen by measurement
@@ -1951,6 +1991,12 @@ bool sorted_sources(const EvalPlan &p) {
            p.variant == PV_T3_EWALD;
 }
 bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
+// The FP64 R3 and nearest-image kernels read source records whose mass slot
+// holds -1.5 m (mr3_scaled): the sorted copy, scaled in place, or a scaled
+// copy of the input-order records.
+bool scaled_sources(const EvalPlan &p) {
+    return !is_mixed(p) && (p.variant == PV_R3 || p.variant == PV_T3_NEAREST || p.variant == PV_S1R2_NEAREST);
+}
 // The T3 Ewald kernel also takes its rows in that order when it evaluates
 // every particle, and so do the nearest-image FP64 kernels (Morton order: a
 // wave's rows then sit in one small region, so whole tiles pass the far /
@@ -2030,6 +2076,7 @@ KScalars make_scalars(const EvalPlan &p) {
     k.images = p.images;
     k.image_cut = (((double)p.images) - 0.4) * p.L;
     k.c15 = 1.5;
+    k.cm53 = -5.0 / 3.0;
     k.inv_gs = (p.t3_ng > 0 && p.L > 0) ? 1.0 / (p.L / (double)p.t3_ng) : 0.0;
     const double drho = k.rho_max / (double)std::max(1, p.s1r2_nrho - 1);
     k.inv_drho = (drho > 0.0) ? 1.0 / drho : 0.0;
@@ -2050,7 +2097,7 @@ PairArgs make_args(const EvalPlan &p, const DeviceParticles &dp, const ScratchVi
     const bool sorted = sorted_sources(p);
     a.irec = sorted_rows(p) ? v.jrec_s : dp.jrec;
     a.isoft = sorted_rows(p) ? v.soft_s : dp.soft;
-    a.jrec = sorted ? v.jrec_s : dp.jrec;
+    a.jrec = (sorted || scaled_sources(p)) ? v.jrec_s : dp.jrec;
     a.soft = sorted ? v.soft_s : dp.soft;
     a.jrel = v.jrel;
     a.tiles = v.tiles;
@@ -2148,12 +2195,12 @@ size_t tile_extra_bytes(const EvalPlan &p) {
     const int64_t nt = (p.N + kTile - 1) / kTile;
     const size_t n = (size_t)std::max<int64_t>(p.N, 1);
     size_t b = 2 * align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4));
-    if (sorted_sources(p) || is_mixed(p)) {  // FP32 records relative to the tiles, FP32 table
+    if (sorted_sources(p) || is_mixed(p) || scaled_sources(p)) {  // FP32 records relative to the tiles, FP32 table
         b += align256(n * sizeof(float4));
         b += align256(table32_bytes(p));
     }
+    if (sorted_sources(p) || scaled_sources(p)) b += align256(n * sizeof(double4)) + align256(n * sizeof(double));
     if (sorted_sources(p)) {
-        b += align256(n * sizeof(double4)) + align256(n * sizeof(double));
         b += align256(order_scratch_bytes(p.N));
     }
     return b + 256;
@@ -2181,23 +2228,25 @@ hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, d
         if ((e = launch_spatial_order(dp.jrec, dp.soft, p.N, v.order, v.jrec_s, v.soft_s, st,
                                       pencil_order(p) ? p.L : 0.0, pencil_order(p) ? p.t3_ng : 0)) != hipSuccess)
             return e;
+        double4 *scaled = scaled_sources(p) ? v.jrec_s : nullptr;  // in place
         if (is_mixed(p))
             hipLaunchKernelGGL((tiles_kernel<true>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles,
-                               v.jrel, v.tile_ext, p.N);
+                               v.jrel, v.tile_ext, scaled, p.N);
         else
             hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles,
-                               v.jrel, v.tile_ext, p.N);
+                               v.jrel, v.tile_ext, scaled, p.N);
     } else {
         // tile extents too: the R3 kernel's far-tile test reads them (on unsorted
         // tiles it rarely passes, and the per-pair loop runs), and so do the
         // S1R2 table windows; the mixed TSC kernel takes its FP32 records
         // relative to input-order tiles
+        double4 *scaled = scaled_sources(p) ? v.jrec_s : nullptr;
         if (is_mixed(p))
             hipLaunchKernelGGL((tiles_kernel<true>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles, v.jrel,
-                               v.tile_ext, p.N);
+                               v.tile_ext, scaled, p.N);
         else
             hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles,
-                               v.jrel, v.tile_ext, p.N);
+                               v.jrel, v.tile_ext, scaled, p.N);
     }
     return hipGetLastError();
 }

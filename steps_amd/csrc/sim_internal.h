@@ -27,8 +27,8 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
-// Device copy of a host table, re-uploaded only when its content changes
-// (size + strided-sample fingerprint).
+// Device copy of a host table, re-uploaded only when its pointer, size or
+// content changes (a hash of every entry).
 struct TableCopy {
     const double *host = nullptr;
     size_t n = 0;
@@ -72,7 +72,9 @@ struct steps_sim {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [kind][precision]
     int64_t last_nrows = 0;
-    bool evaluated = false;
+    bool evaluated = false;  // F holds last_nrows rows of a force evaluation of the current positions
+    bool full = false;       // ... and those rows are all N particles in input order (what a KDK step kicks with)
+    bool timed = false;      // ev0..ev2 bracket the last force evaluation
 };
 
 namespace steps {

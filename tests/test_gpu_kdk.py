@@ -97,3 +97,42 @@ def test_c0_force_and_ten_steps(steps, oracle):
     print(f"[c0 after 10 steps] median {np.median(rel):.3e} p99 {np.percentile(rel, 99):.3e}", flush=True)
     assert np.median(rel) <= 1e-12
     sim.close()
+
+
+def test_step_needs_full_range_forces(steps, oracle):
+    """A KDK step kicks every particle with the stored forces of the current
+    positions (step.cc:128-140), so it refuses to run on forces of a row
+    sub-range, on forces of older positions, or before any evaluation; after a
+    step the stored forces are the step's own, all N rows, and equal a fresh
+    evaluation of the new positions bitwise."""
+    print("case kdk state checks N=2048", flush=True)
+    w = wl.r3_compactified(N=2048, seed=31)
+    kw = dict(cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere)
+    desc, keep = steps.make_desc(steps.TOPO_R3, **kw)
+    sim = steps.Sim(desc, w.N)
+    sim.set_state(w.x, w.M, w.soft, w.v)
+    args = (1e-4, w.a_start, 1.0, w.a_start * 1.001, 1.0, w.rho_part)
+    with pytest.raises(steps.StepsError) as ei:
+        sim.step(*args)
+    assert ei.value.code == steps.STEPS_ERR_STATE
+    sim.forces(100, 199)
+    with pytest.raises(steps.StepsError):
+        sim.step(*args)
+    with pytest.raises(steps.StepsError):
+        sim.errmax(w.a_start, 1.0, w.rho_part)
+    assert sim.get_forces().shape == (100, 3)        # the sub-range evaluation is still readable
+    sim.forces()
+    sim.set_positions(w.x * 1.0001)                  # new positions: the stored forces are stale
+    with pytest.raises(steps.StepsError):
+        sim.step(*args)
+    sim.set_positions(w.x)
+    sim.forces()
+    sim.step(*args)
+    Fs = sim.get_forces()                            # no evaluation in between
+    assert Fs.shape == (w.N, 3)
+    with pytest.raises(steps.StepsError):
+        sim.last_kernel_ms()                         # a step carries no timing events
+    sim.forces()
+    assert np.array_equal(sim.get_forces(), Fs)
+    sim.step(*args)                                  # a step after a step
+    sim.close()

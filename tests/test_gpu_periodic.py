@@ -485,3 +485,36 @@ def test_periodic_ragged_sizes(steps, oracle, t3_tab, cyl, mode, N):
     assert np.isfinite(Fm).all()
     if N >= 65:  # with two particles one nearly cancelling force would decide the statistic
         assert np.median(em) <= 1e-5 and np.percentile(em, 90) <= 1e-4
+
+
+def test_table_edited_in_place_is_uploaded_again(steps, t3_tab):
+    """The one-shot entry point keeps the device copy of a table between calls
+    and re-uploads it when the caller's table changes.  A single entry edited
+    in place (same pointer, same size) at a node a pair interpolates from, and
+    not on any stride a sampled fingerprint would read, must reach the device:
+    the forces then equal those of a fresh copy of the edited table bitwise."""
+    print("case t3 table edited in place", flush=True)
+    L, ng = L_T3, 63
+    x = np.array([[10.0, 20.0, 30.0], [27.3, 25.1, 109.8]])
+    M = np.array([1.0, 2.0])
+    h = np.array([0.01, 0.01])
+    tab = t3_tab.copy()
+    d, keep = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=L, t3_table=tab)
+    live = keep[0]
+    assert np.shares_memory(live, tab)
+    F0 = steps.force_eval(d, x, M, h)
+    dd = x[1] - x[0]
+    dd -= L * np.round(dd / L)                       # nearest image, as the kernel takes it
+    u = (dd + 0.5 * L) * ng / L - 0.5                # grid coordinate of the displacement
+    ix, iy, iz = (int(round(c)) for c in u)          # a node inside the tricubic stencil
+    idx = ((ix * ng + iy) * ng + iz) * 3 + 0
+    n = live.size
+    assert idx % max(1, n // 4096) != 0 and idx != n - 1
+    live[idx] += 50.0 * np.abs(t3_tab).max()
+    F1 = steps.force_eval(d, x, M, h)
+    assert not np.array_equal(F1, F0)
+    d2, keep2 = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=L, t3_table=live.copy())
+    F2 = steps.force_eval(d2, x, M, h)
+    assert np.array_equal(F1, F2)
+    live[idx] = t3_tab[idx]                           # and back
+    assert np.array_equal(steps.force_eval(d, x, M, h), F0)
