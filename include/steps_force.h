@@ -198,6 +198,7 @@ typedef struct steps_server_stats {
     int64_t resident_sims, bytes_resident, max_backlog;
     int64_t packed_launches, packed_requests;  /* launches holding >= 2 requests, and those requests */
     int64_t tickets_held;      /* requests whose result has not been collected yet */
+    int64_t admitting;         /* admissions in progress (device copies being made) */
 } steps_server_stats;
 
 void steps_server_config_init(steps_server_config *c);
@@ -219,7 +220,9 @@ int steps_server_remove_sim(steps_server *s, int64_t sim_id);
 /* Device bytes the simulation takes while resident (state + scratch + tables). */
 int64_t steps_server_sim_bytes(steps_server *s, int64_t sim_id);
 /* kind: STEPS_REQ_FORCE (all rows) or STEPS_REQ_STEP (p required); precision
- * per request (enum steps_precision). */
+ * per request (enum steps_precision).  A step kicks with the forces of the
+ * current positions, so a simulation's first request must be a force
+ * evaluation: a step before it finishes with STEPS_ERR_STATE. */
 int steps_server_submit(steps_server *s, int64_t sim_id, int32_t kind, int32_t precision,
                         const steps_step_params *p, int64_t *ticket);
 /* Blocks until the request is done and returns its result once: the ticket

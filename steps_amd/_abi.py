@@ -98,7 +98,7 @@ class RequestResult(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("submitted", "completed", "failed", "admissions", "evictions",
                                         "graph_captures", "resident_sims", "bytes_resident", "max_backlog",
-                                        "packed_launches", "packed_requests", "tickets_held")]
+                                        "packed_launches", "packed_requests", "tickets_held", "admitting")]
 
 
 _lib = None

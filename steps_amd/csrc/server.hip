@@ -19,6 +19,10 @@
 //     pair variant and precision are packed: their pair kernels run as one
 //     launch (a flat grid over all their row blocks and splits), the cheap
 //     per-simulation kernels around it are launched directly.
+// The scheduler takes its decisions under the server mutex and does the
+// device work they imply (allocation, state copies, graph capture, launches)
+// after releasing it, so submit, wait, get_stats and get_state never wait for
+// device work; a simulation whose device state is in use is marked busy.
 // A request's result does not depend on the mix: its kernels, plan and
 // reduction order are those of the simulation alone.
 #include "sim_internal.h"
@@ -56,7 +60,11 @@ struct SimEntry {
     std::vector<double> t3, s1r2, radial;   // host tables, needed to (re)admit
     steps_sim *res = nullptr;
     size_t bytes = 0;
-    bool busy = false, has_F = false, removed = false;
+    // busy: its device state is in use (a request in flight, an admission
+    // or eviction in progress, a get_state copy); has_F: the host F copy is
+    // the forces of the host positions; lost: an eviction failed to copy the
+    // state back, so its requests fail
+    bool busy = false, has_F = false, removed = false, lost = false;
     double last_used = 0;
     std::deque<int64_t> pending;
 };
@@ -141,10 +149,15 @@ struct steps_server {
     std::condition_variable cv_work, cv_done;
     std::thread thr;
     bool stop = false;
-    std::vector<SimEntry> sims;
+    // Simulations by id.  A deque: add_sim appends while the scheduler works
+    // on entries through pointers taken under the lock, and appending to a
+    // deque leaves existing elements where they are.
+    std::deque<SimEntry> sims;
     // Requests by ticket: reqs[t - base].  Requests that are done and whose
     // result was collected leave the front, so a long-running server keeps
-    // only the window of tickets still in use.
+    // only the window of tickets still in use.  Erasing at the front and
+    // appending leave the other elements in place, so the scheduler may hold
+    // a pointer to a request it has taken until it publishes the outcome.
     std::deque<Request> reqs;
     int64_t base = 0;
     Request &req(int64_t t) { return reqs[(size_t)(t - base)]; }
@@ -155,8 +168,9 @@ struct steps_server {
             ++base;
         }
     }
-    std::vector<Slot> slots;
-    size_t used = 0;
+    std::vector<Slot> slots;  // the scheduler thread's own (destroy touches them after the join)
+    size_t used = 0;          // device bytes of resident simulations, admissions in progress included
+    int64_t admitting = 0;    // admissions between decision and publication
     steps_server_stats st{};
     Clock::time_point t0 = Clock::now();
 
@@ -168,47 +182,6 @@ struct steps_server {
         return (int)n;
     }
 
-    void evict(SimEntry &e) {
-        // state back to host, then free the device copy
-        if (sim_download(e.res, e.x.data(), e.v.data(), e.F.data()) == STEPS_OK) e.has_F = true;
-        sim_destroy(e.res);
-        e.res = nullptr;
-        used -= e.bytes;
-        st.evictions++;
-    }
-
-    // 0: resident; 1: cannot fit now (retry later); <0: permanent error
-    int ensure_resident(SimEntry &e, int64_t self) {
-        if (e.res) return 0;
-        if ((int64_t)e.bytes > cfg.hbm_budget_bytes) return STEPS_ERR_ADMISSION;
-        while ((int64_t)(used + e.bytes) > cfg.hbm_budget_bytes) {
-            SimEntry *lru = nullptr;
-            for (size_t k = 0; k < sims.size(); ++k) {
-                SimEntry &o = sims[k];
-                if ((int64_t)k == self || !o.res || o.busy) continue;
-                if (!lru || o.last_used < lru->last_used) lru = &o;
-            }
-            if (!lru) return 1;
-            evict(*lru);
-        }
-        steps_force_desc d = e.desc;
-        d.t3_table = e.t3.empty() ? nullptr : e.t3.data();
-        d.s1r2_table = e.s1r2.empty() ? nullptr : e.s1r2.data();
-        d.radial_table = e.radial.empty() ? nullptr : e.radial.data();
-        steps_sim *s = nullptr;
-        int rc = sim_create(d, e.N, true, &s);
-        if (rc) return rc;
-        rc = sim_upload(s, e.x.data(), e.v.data(), e.M.data(), e.soft.data(), e.has_F ? e.F.data() : nullptr);
-        if (rc) {
-            sim_destroy(s);
-            return rc;
-        }
-        e.res = s;
-        used += e.bytes;
-        st.admissions++;
-        return 0;
-    }
-
     void finish(Request &r, int status) {
         r.status = status;
         r.t_end = now();
@@ -218,11 +191,160 @@ struct steps_server {
         cv_done.notify_all();
     }
 
-    // Launch request r of entry e on slot sl; returns a status (not thrown).
-    int issue(SimEntry &e, Request &r, Slot &sl) {
-        steps_sim *s = e.res;
+    // ---- the scheduler's decisions (taken under mu) and the device work they
+    // imply (done without it).  Every simulation a decision names is marked
+    // busy until the outcome is published, so submit / wait / get_state /
+    // get_stats never wait for device work, and no other thread touches its
+    // device state meanwhile.
+    struct Evict {
+        SimEntry *e;
+        steps_sim *res;
+        int rc = STEPS_OK;
+    };
+    struct Action {
+        Slot *slot = nullptr;
+        std::vector<SimEntry *> es;   // one, or the members of a packed launch
+        std::vector<Request *> rs;    // their head requests, taken off `pending`
+        std::vector<Evict> victims;   // made room for an admission
+        bool admit = false;           // es[0] has to become resident first
+        steps_sim *admitted = nullptr;
+        int rc = STEPS_OK;            // admission or launch status
+    };
+
+    // Room for e (not resident) within the budget: least-recently-used idle
+    // residents are detached as victims (their device state is downloaded and
+    // freed with the action).  0: reserved; 1: no room now (victims already
+    // detached still go); <0: it can never fit.
+    int plan_admission(SimEntry &e, std::vector<Evict> &victims) {
+        if ((int64_t)e.bytes > cfg.hbm_budget_bytes) return fail(STEPS_ERR_ADMISSION, "simulation larger than the budget");
+        while ((int64_t)(used + e.bytes) > cfg.hbm_budget_bytes) {
+            SimEntry *lru = nullptr;
+            for (SimEntry &o : sims) {
+                if (&o == &e || !o.res || o.busy) continue;
+                if (!lru || o.last_used < lru->last_used) lru = &o;
+            }
+            if (!lru) return 1;
+            victims.push_back(Evict{lru, lru->res});
+            lru->res = nullptr;
+            lru->busy = true;
+            used -= lru->bytes;
+        }
+        used += e.bytes;
+        return 0;
+    }
+
+    bool step_without_forces(const SimEntry &e) {
+        return req(e.pending.front()).kind == STEPS_REQ_STEP && !e.has_F;
+    }
+
+    // Ready resident requests that can share one packed pair launch with the
+    // head of ready[qi] (small, same variant and precision, valid plan).
+    std::vector<SimEntry *> form_pack(const std::vector<SimEntry *> &ready, size_t qi) {
+        SimEntry *e = ready[qi];
+        std::vector<SimEntry *> group{e};
+        if (cfg.pack_max_n <= 0 || e->N > cfg.pack_max_n || !e->res) return group;
+        const int prec = req(e->pending.front()).precision;
         EvalPlan p;
-        int rc = sim_plan(s, 0, e.N - 1, r.precision, p);
+        if (sim_plan(e->res, 0, e->N - 1, prec, p) != STEPS_OK) return group;
+        for (size_t qj = qi + 1; qj < ready.size() && (int)group.size() < kMaxPack; ++qj) {
+            SimEntry *e2 = ready[qj];
+            if (e2->busy || !e2->res || e2->removed || e2->lost || e2->N > cfg.pack_max_n || e2->variant != e->variant)
+                continue;
+            if (req(e2->pending.front()).precision != prec || step_without_forces(*e2)) continue;
+            EvalPlan p2;
+            if (sim_plan(e2->res, 0, e2->N - 1, prec, p2) != STEPS_OK) continue;
+            e2->busy = true;  // taken: the loop over `ready` skips it
+            group.push_back(e2);
+        }
+        return group;
+    }
+
+    // Under mu: requests of removed or lost simulations fail, removed idle
+    // residents are detached for destruction, and each free slot gets the
+    // oldest ready head request (with its admission) or a packed group.
+    bool plan(std::vector<Action> &acts, std::vector<steps_sim *> &reap) {
+        bool progressed = false;
+        for (SimEntry &e : sims) {
+            if (!(e.removed || e.lost)) continue;
+            while (!e.pending.empty()) {
+                Request &r = req(e.pending.front());
+                e.pending.pop_front();
+                finish(r, e.removed ? fail(STEPS_ERR_STATE, "simulation removed")
+                                    : fail(STEPS_ERR_DEVICE, "simulation state lost in an eviction"));
+                progressed = true;
+            }
+            if (e.removed && !e.busy && e.res) {
+                reap.push_back(e.res);
+                e.res = nullptr;
+                used -= e.bytes;
+                progressed = true;
+            }
+        }
+        std::vector<SimEntry *> ready;
+        for (SimEntry &e : sims)
+            if (!e.busy && !e.pending.empty()) ready.push_back(&e);
+        std::sort(ready.begin(), ready.end(),
+                  [](const SimEntry *a, const SimEntry *b) { return a->pending.front() < b->pending.front(); });
+        for (size_t qi = 0; qi < ready.size(); ++qi) {
+            SimEntry &e = *ready[qi];
+            if (e.busy) continue;  // packed with an earlier head
+            if (step_without_forces(e)) {
+                // step.cc:128-140 kicks with the forces of the current
+                // positions: a step needs a force request before it
+                Request &r = req(e.pending.front());
+                e.pending.pop_front();
+                finish(r, fail(STEPS_ERR_STATE, "step request before any force evaluation of the simulation"));
+                progressed = true;
+                continue;
+            }
+            Slot *free_slot = nullptr;
+            for (Slot &sl : slots)
+                if (sl.ticket < 0) { free_slot = &sl; break; }
+            if (!free_slot) break;
+            Action a;
+            a.slot = free_slot;
+            if (!e.res) {
+                const int adm = plan_admission(e, a.victims);
+                if (adm != 0) {
+                    if (adm < 0) {
+                        Request &r = req(e.pending.front());
+                        e.pending.pop_front();
+                        finish(r, adm);
+                        progressed = true;
+                    }
+                    // no room until something finishes; evictions already
+                    // decided still run (an action without a slot)
+                    if (!a.victims.empty()) {
+                        a.slot = nullptr;
+                        acts.push_back(std::move(a));
+                        progressed = true;
+                    }
+                    continue;
+                }
+                a.admit = true;
+                admitting++;
+            }
+            e.busy = true;
+            a.es = a.admit ? std::vector<SimEntry *>{&e} : form_pack(ready, qi);
+            const double ts = now();
+            for (SimEntry *m : a.es) {
+                Request &r = req(m->pending.front());
+                m->pending.pop_front();
+                r.t_start = ts;
+                a.rs.push_back(&r);
+            }
+            free_slot->ticket = a.rs[0]->ticket;  // reserved; the launch is published later
+            acts.push_back(std::move(a));
+            progressed = true;
+        }
+        return progressed;
+    }
+
+    // Launch request r on slot sl for resident simulation s (graph captured
+    // on first use); returns a status.
+    int issue(steps_sim *s, const Request &r, Slot &sl, int64_t &captures) {
+        EvalPlan p;
+        int rc = sim_plan(s, 0, s->N - 1, r.precision, p);
         if (rc) return rc;
         if (r.kind == STEPS_REQ_STEP && (rc = sim_fill_params(s, r.sp))) return rc;
         hipError_t he;
@@ -242,7 +364,7 @@ struct steps_server {
                     g = nullptr;
                     return hip_fail(he, "instantiate");
                 }
-                st.graph_captures++;
+                captures++;
             }
             STEPS_HIP(hipGraphLaunch(g, sl.stream), "graph launch");
         } else {
@@ -254,191 +376,180 @@ struct steps_server {
         return STEPS_OK;
     }
 
-    // Ready requests that can share one packed pair launch with the head of
-    // ready[qi] (resident, small, same variant and precision, valid plan); the
-    // head itself first.  Members are kept from eviction while the group forms.
-    std::vector<int64_t> form_pack(const std::vector<int64_t> &ready, size_t qi, const std::vector<char> &taken) {
-        const int64_t k = ready[qi];
-        std::vector<int64_t> group{k};
-        SimEntry &e = sims[k];
-        if (cfg.pack_max_n <= 0 || e.N > cfg.pack_max_n) return group;
-        const int prec = req(e.pending.front()).precision;
-        EvalPlan p;
-        if (sim_plan(e.res, 0, e.N - 1, prec, p) != STEPS_OK) return group;
-        e.busy = true;
-        for (size_t qj = qi + 1; qj < ready.size() && (int)group.size() < kMaxPack; ++qj) {
-            const int64_t k2 = ready[qj];
-            SimEntry &e2 = sims[k2];
-            if (taken[k2] || e2.removed || e2.busy || e2.N > cfg.pack_max_n || e2.variant != e.variant) continue;
-            if (req(e2.pending.front()).precision != prec) continue;
-            if (ensure_resident(e2, k2) != 0) continue;
-            EvalPlan p2;
-            if (sim_plan(e2.res, 0, e2.N - 1, prec, p2) != STEPS_OK) continue;
-            e2.busy = true;
-            group.push_back(k2);
-        }
-        for (const int64_t g : group) sims[g].busy = false;
-        return group;
-    }
-
-    // One packed launch for the head requests of `group`: per simulation the
-    // step's first half and the source preparation, one pair kernel over all,
-    // per simulation the reduction and the step's second half.  No graphs
+    // One packed launch for the head requests of the group: per simulation
+    // the step's first half and the source preparation, one pair kernel over
+    // all, per simulation the reduction and the step's second half.  No graphs
     // (the composition changes from launch to launch).
-    void issue_batch(const std::vector<int64_t> &group, Slot &sl) {
-        const size_t n = group.size();
+    int issue_batch(const Action &a) {
+        const size_t n = a.es.size();
         std::vector<EvalPlan> plans(n);
-        std::vector<int64_t> tickets(n);
-        const double t0 = now();
         for (size_t i = 0; i < n; ++i) {
-            SimEntry &e = sims[group[i]];
-            tickets[i] = e.pending.front();
-            e.pending.pop_front();
-            req(tickets[i]).t_start = t0;
+            steps_sim *s = a.es[i]->res;
+            const Request &r = *a.rs[i];
+            int rc = sim_plan(s, 0, s->N - 1, r.precision, plans[i]);
+            if (!rc && r.kind == STEPS_REQ_STEP) rc = sim_fill_params(s, r.sp);
+            if (rc) return rc;
         }
-        auto fail_all = [&](int code) {
-            for (const int64_t t : tickets) finish(req(t), code);
-        };
-        for (size_t i = 0; i < n; ++i) {
-            SimEntry &e = sims[group[i]];
-            const Request &r = req(tickets[i]);
-            int rc = sim_plan(e.res, 0, e.N - 1, r.precision, plans[i]);
-            if (!rc && r.kind == STEPS_REQ_STEP) rc = sim_fill_params(e.res, r.sp);
-            if (rc) return fail_all(rc);
-        }
+        Slot &sl = *a.slot;
         const hipStream_t qs = sl.stream;
         hipError_t he = hipSuccess;
         for (size_t i = 0; i < n && he == hipSuccess; ++i)
-            if (req(tickets[i]).kind == STEPS_REQ_STEP) he = sim_enqueue_step_before(sims[group[i]].res, qs);
+            if (a.rs[i]->kind == STEPS_REQ_STEP) he = sim_enqueue_step_before(a.es[i]->res, qs);
         for (size_t i = 0; i < n && he == hipSuccess; ++i) {
-            steps_sim *s = sims[group[i]].res;
+            steps_sim *s = a.es[i]->res;
             he = launch_forces_prepare(plans[i], s->dp, s->part.as<double>(), qs);
         }
         int64_t blocks = 0;
         for (size_t i = 0; i < n && he == hipSuccess; ++i) {
-            steps_sim *s = sims[group[i]].res;
+            steps_sim *s = a.es[i]->res;
             blocks += pack_describe(plans[i], s->dp, s->part.as<double>(), sl.hseg, (int)i, blocks);
         }
         if (he == hipSuccess)
             he = hipMemcpyAsync(sl.dseg.ptr, sl.hseg, n * pack_desc_bytes(), hipMemcpyHostToDevice, qs);
         if (he == hipSuccess) he = launch_pair_packed(plans[0].variant, plans[0].precision, sl.dseg.ptr, (int)n, blocks, qs);
         for (size_t i = 0; i < n && he == hipSuccess; ++i) {
-            steps_sim *s = sims[group[i]].res;
+            steps_sim *s = a.es[i]->res;
             he = launch_forces_finish(plans[i], s->dp, s->part.as<double>(), s->F.as<double>(), qs);
         }
         for (size_t i = 0; i < n && he == hipSuccess; ++i)
-            if (req(tickets[i]).kind == STEPS_REQ_STEP) he = sim_enqueue_step_after(sims[group[i]].res, qs);
+            if (a.rs[i]->kind == STEPS_REQ_STEP) he = sim_enqueue_step_after(a.es[i]->res, qs);
         if (he == hipSuccess) he = hipEventRecord(sl.done, qs);
         if (he != hipSuccess) {
             // work already queued on this stream must drain before the
             // simulations can be used again
             (void)hipStreamSynchronize(qs);
-            return fail_all(hip_fail(he, "packed launch"));
+            return hip_fail(he, "packed launch");
         }
-        for (const int64_t g : group) sims[g].busy = true;
-        sl.batch = tickets;
-        sl.ticket = tickets[0];
-        st.packed_launches++;
-        st.packed_requests += (int64_t)n;
+        return STEPS_OK;
+    }
+
+    // Without mu: evictions (state to host, device copy freed), the
+    // admission (device copy created from the host state), the launch.
+    // Touches only the simulations and requests the action names.
+    void execute(Action &a, int64_t &captures) {
+        for (Evict &v : a.victims) {
+            SimEntry &o = *v.e;
+            v.rc = sim_download(v.res, o.x.data(), o.v.data(), o.F.data());
+            sim_destroy(v.res);
+        }
+        if (!a.slot) return;
+        SimEntry &e = *a.es[0];
+        if (a.admit) {
+            steps_force_desc d = e.desc;
+            d.t3_table = e.t3.empty() ? nullptr : e.t3.data();
+            d.s1r2_table = e.s1r2.empty() ? nullptr : e.s1r2.data();
+            d.radial_table = e.radial.empty() ? nullptr : e.radial.data();
+            steps_sim *s = nullptr;
+            a.rc = sim_create(d, e.N, true, &s);
+            if (!a.rc) {
+                a.rc = sim_upload(s, e.x.data(), e.v.data(), e.M.data(), e.soft.data(), e.has_F ? e.F.data() : nullptr);
+                if (a.rc) sim_destroy(s);
+            }
+            if (a.rc) return;
+            a.admitted = s;
+        }
+        a.rc = (a.es.size() >= 2) ? issue_batch(a) : issue(a.admit ? a.admitted : e.res, *a.rs[0], *a.slot, captures);
+    }
+
+    // Under mu: outcomes of the actions.
+    void publish(std::vector<Action> &acts) {
+        for (Action &a : acts) {
+            for (Evict &v : a.victims) {
+                SimEntry &o = *v.e;
+                o.busy = false;
+                if (v.rc == STEPS_OK) o.has_F = true;
+                else o.lost = true;  // its requests fail from now on
+                st.evictions++;
+            }
+            if (!a.slot) continue;
+            if (a.admit) {
+                admitting--;
+                SimEntry &e = *a.es[0];
+                if (a.admitted) {
+                    e.res = a.admitted;
+                    st.admissions++;
+                } else {
+                    used -= e.bytes;
+                }
+            }
+            if (a.rc == STEPS_OK) {
+                if (a.es.size() >= 2) {
+                    st.packed_launches++;
+                    st.packed_requests += (int64_t)a.es.size();
+                }
+                a.slot->batch.clear();
+                for (Request *r : a.rs) a.slot->batch.push_back(r->ticket);
+                a.slot->ticket = a.rs[0]->ticket;
+            } else {
+                for (size_t i = 0; i < a.es.size(); ++i) {
+                    finish(*a.rs[i], a.rc);
+                    a.es[i]->busy = false;
+                }
+                a.slot->ticket = -1;
+                a.slot->batch.clear();
+            }
+        }
+        cv_done.notify_all();  // get_state waits for !busy
+    }
+
+    // Under mu: requests whose slot event completed.
+    bool complete(const std::vector<hipError_t> &q) {
+        bool progressed = false;
+        for (size_t i = 0; i < slots.size(); ++i) {
+            Slot &sl = slots[i];
+            if (sl.ticket < 0 || q[i] == hipErrorNotReady) continue;
+            for (const int64_t t : sl.batch) {
+                Request &r = req(t);
+                SimEntry &e = sims[(size_t)r.sim];
+                if (q[i] == hipSuccess) {
+                    if (r.kind == STEPS_REQ_STEP) r.errmax = *e.res->herrmax;
+                    e.has_F = true;
+                    finish(r, STEPS_OK);
+                } else {
+                    finish(r, hip_fail(q[i], "request"));
+                }
+                e.busy = false;
+                e.last_used = now();
+            }
+            sl.batch.clear();
+            sl.ticket = -1;
+            progressed = true;
+        }
+        return progressed;
     }
 
     void loop() {
         (void)hipSetDevice(cfg.device);
-        std::unique_lock<std::mutex> lk(mu);
+        std::unique_lock<std::mutex> lk(mu, std::defer_lock);
+        std::vector<hipError_t> q(slots.size());
         for (;;) {
-            bool progressed = false;
+            // the slots are this thread's: their events are queried unlocked
+            for (size_t i = 0; i < slots.size(); ++i)
+                q[i] = (slots[i].ticket >= 0) ? hipEventQuery(slots[i].done) : hipErrorNotReady;
+            lk.lock();
+            bool progressed = complete(q);
+            std::vector<Action> acts;
+            std::vector<steps_sim *> reap;
+            progressed = plan(acts, reap) || progressed;
+            lk.unlock();
+            for (steps_sim *s : reap) sim_destroy(s);
+            int64_t captures = 0;
+            for (Action &a : acts) execute(a, captures);
+            lk.lock();
+            publish(acts);
+            st.graph_captures += captures;
             int inflight = 0;
-            // completions
-            for (Slot &sl : slots) {
-                if (sl.ticket < 0) continue;
-                const hipError_t q = hipEventQuery(sl.done);
-                if (q == hipErrorNotReady) {
-                    inflight++;
-                    continue;
-                }
-                if (sl.batch.empty()) sl.batch.push_back(sl.ticket);
-                for (const int64_t t : sl.batch) {
-                    Request &r = req(t);
-                    SimEntry &e = sims[r.sim];
-                    if (q == hipSuccess) {
-                        if (r.kind == STEPS_REQ_STEP) r.errmax = *e.res->herrmax;
-                        e.has_F = true;
-                        finish(r, STEPS_OK);
-                    } else {
-                        finish(r, hip_fail(q, "request"));
-                    }
-                    e.busy = false;
-                    e.last_used = now();
-                    if (e.removed && e.pending.empty() && e.res) {
-                        sim_destroy(e.res);
-                        e.res = nullptr;
-                        used -= e.bytes;
-                    }
-                }
-                sl.batch.clear();
-                sl.ticket = -1;
-                progressed = true;
-            }
-            // issue: oldest head request first, while slots are free
-            std::vector<int64_t> ready;
-            for (size_t k = 0; k < sims.size(); ++k)
-                if (!sims[k].busy && !sims[k].pending.empty()) ready.push_back((int64_t)k);
-            std::sort(ready.begin(), ready.end(),
-                      [&](int64_t a, int64_t b) { return sims[a].pending.front() < sims[b].pending.front(); });
-            std::vector<char> taken(sims.size(), 0);
-            for (size_t qi = 0; qi < ready.size(); ++qi) {
-                const int64_t k = ready[qi];
-                if (taken[k]) continue;
-                Slot *free_slot = nullptr;
-                for (Slot &sl : slots)
-                    if (sl.ticket < 0) { free_slot = &sl; break; }
-                if (!free_slot) break;
-                SimEntry &e = sims[k];
-                const int64_t t = e.pending.front();
-                Request &r = req(t);
-                if (e.removed) {
-                    e.pending.pop_front();
-                    finish(r, fail(STEPS_ERR_STATE, "simulation removed"));
-                    progressed = true;
-                    continue;
-                }
-                const int adm = ensure_resident(e, k);
-                if (adm == 1) continue;  // no room until something finishes
-                if (adm < 0) {
-                    e.pending.pop_front();
-                    progressed = true;
-                    finish(r, adm);
-                    continue;
-                }
-                std::vector<int64_t> group = form_pack(ready, qi, taken);
-                for (const int64_t g : group) taken[g] = 1;
-                progressed = true;
-                if (group.size() >= 2) {
-                    issue_batch(group, *free_slot);
-                    if (free_slot->ticket >= 0) inflight++;
-                    continue;
-                }
-                e.pending.pop_front();
-                r.t_start = now();
-                const int rc = issue(e, r, *free_slot);
-                if (rc != STEPS_OK) {
-                    finish(r, rc);
-                    continue;
-                }
-                e.busy = true;
-                free_slot->ticket = t;
-                inflight++;
-            }
+            for (const Slot &sl : slots) inflight += sl.ticket >= 0;
             if (stop && inflight == 0) break;
             if (!progressed) {
                 if (inflight > 0) {
                     lk.unlock();
                     std::this_thread::sleep_for(std::chrono::microseconds(20));
-                    lk.lock();
-                } else {
-                    cv_work.wait_for(lk, std::chrono::milliseconds(50));
+                    continue;
                 }
+                cv_work.wait_for(lk, std::chrono::milliseconds(50));
             }
+            lk.unlock();
         }
     }
 };
@@ -555,13 +666,9 @@ int steps_server_remove_sim(steps_server *S, int64_t sim) {
     if (!S) return fail(STEPS_ERR_ARG, "null server");
     std::lock_guard<std::mutex> lk(S->mu);
     if (sim < 0 || sim >= (int64_t)S->sims.size()) return fail(STEPS_ERR_ARG, "unknown simulation");
-    SimEntry &e = S->sims[sim];
-    e.removed = true;
-    if (!e.busy && e.pending.empty() && e.res) {
-        sim_destroy(e.res);
-        e.res = nullptr;
-        S->used -= e.bytes;
-    }
+    // the scheduler fails its pending requests and frees its device copy once
+    // nothing of it is in flight
+    S->sims[(size_t)sim].removed = true;
     S->cv_work.notify_all();
     return STEPS_OK;
 }
@@ -570,7 +677,7 @@ int64_t steps_server_sim_bytes(steps_server *S, int64_t sim) {
     if (!S) return fail(STEPS_ERR_ARG, "null server");
     std::lock_guard<std::mutex> lk(S->mu);
     if (sim < 0 || sim >= (int64_t)S->sims.size()) return fail(STEPS_ERR_ARG, "unknown simulation");
-    return (int64_t)S->sims[sim].bytes;
+    return (int64_t)S->sims[(size_t)sim].bytes;
 }
 
 int steps_server_submit(steps_server *S, int64_t sim, int32_t kind, int32_t precision, const steps_step_params *sp,
@@ -580,7 +687,7 @@ int steps_server_submit(steps_server *S, int64_t sim, int32_t kind, int32_t prec
     if (precision != STEPS_PREC_FP64 && precision != STEPS_PREC_MIXED) return fail(STEPS_ERR_ARG, "unknown precision");
     if (kind == STEPS_REQ_STEP && !sp) return fail(STEPS_ERR_ARG, "step request without parameters");
     std::lock_guard<std::mutex> lk(S->mu);
-    if (sim < 0 || sim >= (int64_t)S->sims.size() || S->sims[sim].removed)
+    if (sim < 0 || sim >= (int64_t)S->sims.size() || S->sims[(size_t)sim].removed)
         return fail(STEPS_ERR_ARG, "unknown simulation");
     if (S->stop) return fail(STEPS_ERR_STATE, "server stopping");
     Request r;
@@ -591,7 +698,7 @@ int steps_server_submit(steps_server *S, int64_t sim, int32_t kind, int32_t prec
     if (sp) r.sp = *sp;
     r.t_submit = S->now();
     S->reqs.push_back(r);
-    S->sims[sim].pending.push_back(r.ticket);
+    S->sims[(size_t)sim].pending.push_back(r.ticket);
     S->st.submitted++;
     S->st.max_backlog = std::max<int64_t>(S->st.max_backlog, S->backlog());
     *ticket = r.ticket;
@@ -636,10 +743,19 @@ int steps_server_get_state(steps_server *S, int64_t sim, double *x, double *v, d
     if (!S) return fail(STEPS_ERR_ARG, "null server");
     std::unique_lock<std::mutex> lk(S->mu);
     if (sim < 0 || sim >= (int64_t)S->sims.size()) return fail(STEPS_ERR_ARG, "unknown simulation");
-    S->cv_done.wait(lk, [&] { return !S->sims[sim].busy && S->sims[sim].pending.empty(); });
-    SimEntry &e = S->sims[sim];
+    SimEntry &e = S->sims[(size_t)sim];
+    S->cv_done.wait(lk, [&] { return !e.busy && e.pending.empty(); });
+    if (e.lost) return fail(STEPS_ERR_DEVICE, "simulation state lost in an eviction");
     if (e.res) {
-        int rc = sim_download(e.res, x, v, F);
+        // copied without the lock; busy keeps the scheduler off this simulation
+        e.busy = true;
+        steps_sim *res = e.res;
+        lk.unlock();
+        const int rc = sim_download(res, x, v, F);
+        lk.lock();
+        e.busy = false;
+        S->cv_done.notify_all();
+        S->cv_work.notify_all();
         return rc;
     }
     const size_t n3 = 3 * (size_t)e.N;
@@ -654,6 +770,7 @@ int steps_server_get_stats(steps_server *S, steps_server_stats *out) {
     std::lock_guard<std::mutex> lk(S->mu);
     *out = S->st;
     out->bytes_resident = (int64_t)S->used;
+    out->admitting = S->admitting;
     out->tickets_held = (int64_t)S->reqs.size();
     int64_t n = 0;
     for (const auto &e : S->sims) n += e.res ? 1 : 0;

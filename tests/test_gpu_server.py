@@ -226,3 +226,94 @@ def test_collected_tickets_are_forgotten():
     t = srv.submit(sid, steps_amd.REQ_FORCE)
     assert t == tickets[-1] + 1 and srv.wait(t).status == 0
     srv.close()
+
+
+def test_submit_and_wait_return_during_an_admission():
+    """The scheduler does device work (allocation, state upload) after
+    releasing the server lock: while a large simulation is being admitted,
+    get_stats, submit and wait return at once."""
+    print("case server admission outside the lock (N = 2^24)", flush=True)
+    srv = steps_amd.Server(streams=2)
+    w = wl.r3_compactified(N=2000, seed=9)
+    d, keep = steps_amd.make_desc(steps_amd.TOPO_R3)
+    small = srv.add_sim(d, w.x, w.M, w.soft)
+    t_done = srv.submit(small, steps_amd.REQ_FORCE)
+    srv.drain()                                       # done, not collected
+    n = 1 << 24
+    rng = np.random.default_rng(3)
+    xb = rng.normal(size=(n, 3)) * 100.0
+    Mb, hb = np.ones(n), np.full(n, 0.01)
+    # the S1R2 image-sum variant has no mixed-precision kernel: the request is
+    # admitted (device copies made) and then fails before any pair work
+    db, kb = steps_amd.make_desc(steps_amd.TOPO_S1R2, is_periodic=2, L=200.0, Rsim=1000.0, s1r2_lookup=0)
+    big = srv.add_sim(db, xb, Mb, hb)
+    del xb
+    t_big = srv.submit(big, steps_amd.REQ_FORCE, steps_amd.PREC_MIXED)
+    seen = False
+    for _ in range(2_000_000):
+        if srv.stats()["admitting"] == 1:
+            seen = True
+            break
+        if srv.stats()["failed"] > 0:
+            break
+    assert seen, "the admission was never observed in progress"
+    t_new = srv.submit(small, steps_amd.REQ_FORCE)
+    r = srv.wait(t_done)
+    still = srv.stats()["admitting"]
+    print(f"[server admission] admitting after submit + wait: {still}", flush=True)
+    assert r.status == 0 and still == 1
+    assert srv.wait(t_big).status != 0
+    assert srv.wait(t_new).status == 0
+    srv.close()
+
+
+def test_step_before_force_and_removal():
+    """A step request before any force evaluation of its simulation fails
+    (the kick needs the forces of the current positions); removing a
+    simulation fails its pending requests and frees its device copy."""
+    print("case server step-before-force / removal", flush=True)
+    w = wl.r3_compactified(N=65536, seed=1)          # ~2 ms a step: the later requests are still queued
+    d, keep = steps_amd.make_desc(steps_amd.TOPO_R3, cosmology=1, comoving=1,
+                                  mass_in_unit_sphere=w.mass_in_unit_sphere)
+    srv = steps_amd.Server(streams=1)
+    sid = srv.add_sim(d, w.x, w.M, w.soft, w.v)
+    r = srv.wait(srv.submit(sid, steps_amd.REQ_STEP, 0, step_params(w, 0)))
+    assert r.status == steps_amd.STEPS_ERR_STATE
+    assert srv.wait(srv.submit(sid, steps_amd.REQ_FORCE)).status == 0
+    assert srv.wait(srv.submit(sid, steps_amd.REQ_STEP, 0, step_params(w, 0))).status == 0
+    tickets = [srv.submit(sid, steps_amd.REQ_STEP, 0, step_params(w, k)) for k in range(1, 6)]
+    srv.remove_sim(sid)
+    res = [srv.wait(t) for t in tickets]
+    assert res[-1].status == steps_amd.STEPS_ERR_STATE      # never issued
+    st = srv.stats()
+    assert st["resident_sims"] == 0 and st["bytes_resident"] == 0, st
+    srv.close()
+
+
+def test_c4_ensemble_server_equals_sequential():
+    """The serving configuration itself: 64 simulations (R3 / S1R2 CIC
+    alternating, N log-uniform in [8192, 524288], seed 1004) and a seeded
+    Poisson stream of 48 KDK-step requests (200 req/s, FP64 / mixed 50/50),
+    served by the server -- with the 200 GB budget, and with a budget that
+    holds a third of the ensemble, so that simulations are evicted and
+    re-admitted -- and one request at a time: every simulation's final
+    (x, v) is bitwise the same."""
+    from steps_amd import serving as sv
+    print("case server c4 ensemble 64 sims, 48 requests", flush=True)
+    sims = sv.make_ensemble()
+    reqs = sv.make_requests(sims, nreq=48)
+    seq, fin_q = sv.run_sequential(sims, reqs)
+    print(f"[c4 sequential] {seq}", flush=True)
+    srv, fin_s = sv.run_server(sims, reqs)
+    print(f"[c4 server] {srv}", flush=True)
+    assert srv["failed"] == 0 and srv["admissions"] == len(sims) and srv["evictions"] == 0
+    for i, (a, b) in enumerate(zip(fin_s, fin_q)):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
+    probe = steps_amd.Server(streams=1)
+    total = sum(probe.sim_bytes(probe.add_sim(d, w.x, w.M, w.soft)) for (w, d, _) in sims)
+    probe.close()
+    small, fin_b = sv.run_server(sims, reqs, budget=total // 3)
+    print(f"[c4 server, budget {total // 3} of {total}] {small}", flush=True)
+    assert small["failed"] == 0 and small["evictions"] > 0
+    for i, (a, b) in enumerate(zip(fin_b, fin_q)):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
