@@ -107,7 +107,53 @@ def cpu_baseline(w, target_s: float = 12.0) -> dict:
         "cpu_model": cpu_model,
         "nproc": os.cpu_count(),
         "optimised_variant_value": nrows * N / tf / 1e9,
+        "configs": cpu_configs(lib, oracle_lib),
     }
+
+
+def cpu_configs(lib, oracle_lib, target_s: float = 3.0) -> dict:
+    """The same faithful OpenMP restatement on the other configurations:
+    the R3 N = 16384 case in full (median of three runs), and strided row
+    samples of the T3 64^3 lattice (63^3 Ewald table, the CPU build's
+    default trilinear interpolation, EWALD_INTERPOLATION_ORDER 2) and of the
+    S1xR2 cylinder (N = 524288, CIC table), each sized to about target_s."""
+    threads = int(lib.oracle_max_threads())
+    out = {}
+    w0 = wl.r3_compactified(**WORKLOADS["r3_16k"])
+    p0 = oracle_lib.params(0, cosmology=w0.cosmology, comoving=w0.comoving,
+                           mass_in_unit_sphere=w0.mass_in_unit_sphere)
+    rows = np.arange(w0.N, dtype=np.int32)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle_lib.forces_rows(p0, w0.x, w0.M, w0.soft, rows, kind="faithful")
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    out["r3_16k"] = {"value": w0.N * w0.N / t / 1e9, "unit": "G pair-interactions/s", "cores": threads,
+                     "seconds_per_evaluation": t, "sample": f"all {w0.N} rows, median of 3"}
+    t3 = wl.t3_lattice()
+    T = steps_amd.t3_table(2, t3.L)
+    c = wl.s1r2_cylinder()
+    Ts, shape = steps_amd.s1r2_table(2, c.L, c.Rsim)
+    cases = [("t3_64cubed", t3, oracle_lib.params(1, is_periodic=2, L=t3.L, t3_table=T, interp_order=2),
+              "T3 Ewald (63^3 table, trilinear)"),
+             ("s1r2_cic_524288", c, oracle_lib.params(2, is_periodic=2, L=c.L, Rsim=c.Rsim, cosmology=1, comoving=1,
+                                                      mass_in_unit_sphere=c.mass_in_unit_sphere, interp_order=2,
+                                                      s1r2_table=Ts, s1r2_shape=shape),
+              "S1xR2 CIC table")]
+    for name, w, p, what in cases:
+        nrows = max(16, threads * 2)
+        for _ in range(4):
+            r = (np.arange(nrows, dtype=np.int64) * w.N // nrows).astype(np.int32)
+            t0 = time.perf_counter()
+            oracle_lib.forces_rows(p, w.x, w.M, w.soft, r, kind="faithful")
+            t = time.perf_counter() - t0
+            if t >= 0.6 * target_s or nrows >= w.N:
+                break
+            nrows = int(min(w.N, max(nrows + 1, nrows * target_s / max(t, 1e-3))))
+        out[name] = {"value": nrows * w.N / t / 1e9, "unit": "G pair-interactions/s", "cores": threads,
+                     "sample": f"{nrows} strided rows x all {w.N} sources, {what}, {t:.1f} s"}
+    return out
 
 
 def time_path(sim, steps: int, warmup: int):
@@ -171,13 +217,32 @@ def serving_leg(nreq: int) -> dict:
     print(f"[bench] serving: sequential done {seq}", file=sys.stderr, flush=True)
     same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(fin_s, fin_q))
     pairs = float(sum(sims[r.sim][0].N ** 2 for r in reqs))
+    # At 200 req/s the stream offers ~15x what the device serves, so latency
+    # is queueing time and grows with the stream's length: the same stream cut
+    # at shorter lengths shows by how much.  Latency under a load the device
+    # can sustain comes from the same ensemble at 70% of the measured
+    # capacity, over 20 s of arrivals.
+    by_len = []
+    for n in (nreq // 4, nreq // 2):
+        r, _ = sv.run_server(sims, reqs[:n])
+        by_len.append({k: r[k] for k in ("requests", "wall_s", "p50_ms", "p99_ms", "max_backlog")})
+        print(f"[bench] serving: {n} requests {by_len[-1]}", file=sys.stderr, flush=True)
+    by_len.append({k: srv[k] for k in ("requests", "wall_s", "p50_ms", "p99_ms", "max_backlog")})
+    rate = 0.7 * srv["requests_per_s"]
+    sus_reqs = sv.make_requests(sims, nreq=max(1, int(rate * 20.0)), rate=rate, seed=1005)
+    sus, _ = sv.run_server(sims, sus_reqs)
+    print(f"[bench] serving: sustainable {rate:.1f} req/s {sus}", file=sys.stderr, flush=True)
+    sus["offered_req_per_s"] = rate
+    sus["arrivals_s"] = sus_reqs[-1].t_arrival
     return {"workload": "64 sims R3/S1R2, N log-uniform [8192, 524288], Poisson 200 req/s seed 1004, "
                         "KDK steps, FP64 / mixed 50/50",
-            "requests": nreq, "pair_interactions": pairs,
+            "requests": nreq, "arrivals_s": reqs[-1].t_arrival, "pair_interactions": pairs,
             "steps_per_s": srv["requests_per_s"], "p50_ms": srv["p50_ms"], "p99_ms": srv["p99_ms"],
             "server": srv, "sequential": seq,
             "speedup_vs_sequential": srv["requests_per_s"] / seq["requests_per_s"],
-            "results_identical": bool(same)}
+            "results_identical": bool(same),
+            "latency_vs_stream_length": by_len,
+            "sustainable_load": sus}
 
 
 def variants_leg(reps: int) -> dict:
@@ -189,11 +254,32 @@ def variants_leg(reps: int) -> dict:
     warm-up, then `reps` evaluations timed with HIP events."""
     out = {}
     t3 = wl.t3_lattice()
-    T = steps_amd.t3_table(2, t3.L)
+    c = wl.s1r2_cylinder()
+    # table builds on the device (each the first of its parameters in this
+    # process: the library caches built tables by parameters)
+    tables = {}
+    for ip in (2, 3, 4):
+        t0 = time.perf_counter()
+        Tt = steps_amd.t3_table(ip, t3.L)
+        tables[f"t3_is_periodic_{ip}"] = {"seconds": time.perf_counter() - t0,
+                                          "ngrid": round((Tt.size // 3) ** (1 / 3)), "bytes": Tt.nbytes}
+        if ip == 2:
+            T = Tt
+        elif ip == 3:
+            T127 = Tt
+        t0 = time.perf_counter()
+        Tc, shp = steps_amd.s1r2_table(ip, c.L, c.Rsim)
+        tables[f"s1r2_is_periodic_{ip}"] = {"seconds": time.perf_counter() - t0, "nrho_nz": list(shp),
+                                            "bytes": Tc.nbytes}
+        if ip == 2:
+            Ts, shape = Tc, shp
+    t0 = time.perf_counter()
+    steps_amd.radial_table(c.Rsim, 0.5 * c.L, 1000, 1000)
+    tables["radial_1000"] = {"seconds": time.perf_counter() - t0}
+    out["table_builds"] = tables
+    print(f"[bench] table builds: {tables}", file=sys.stderr, flush=True)
     d, keep = steps_amd.make_desc(steps_amd.TOPO_T3, is_periodic=2, L=t3.L, t3_table=T)
     cases = [("t3_ewald", t3, d, [T, keep])]
-    c = wl.s1r2_cylinder()
-    Ts, shape = steps_amd.s1r2_table(2, c.L, c.Rsim)
     for name, order in (("s1r2_cic", 2), ("s1r2_tsc", 4)):
         ds, ks = steps_amd.make_desc(steps_amd.TOPO_S1R2, is_periodic=2, L=c.L, Rsim=c.Rsim, cosmology=1,
                                      comoving=1, mass_in_unit_sphere=c.mass_in_unit_sphere, s1r2_lookup=1,
@@ -243,6 +329,17 @@ def variants_leg(reps: int) -> dict:
         sim.close()
         out[name] = r
         print(f"[bench] variant {name}: {r}", file=sys.stderr, flush=True)
+    # T3 with the 127^3 table of IS_PERIODIC = 3 (the global-table kernel; the
+    # LDS window needs Ng <= 119), on a 40^3 lattice to bound the run time
+    w40 = wl.t3_lattice(n_side=40, seed=1001, is_periodic=3)
+    T40 = steps_amd.t3_table(3, w40.L) if abs(w40.L - t3.L) > 0 else T127
+    d3, k3 = steps_amd.make_desc(steps_amd.TOPO_T3, is_periodic=3, L=w40.L, t3_table=T40)
+    sim = steps_amd.Sim(d3, w40.N)
+    sim.set_state(w40.x, w40.M, w40.soft, w40.v)
+    _, pair_ms, _ = time_path(sim, 1, 1)
+    sim.close()
+    out["t3_ewald_127"] = {"N": w40.N, "kernel_ms": pair_ms, "g_pairs_per_s": float(w40.N) ** 2 / (pair_ms * 1e-3) / 1e9}
+    print(f"[bench] variant t3_ewald_127: {out['t3_ewald_127']}", file=sys.stderr, flush=True)
     return out
 
 

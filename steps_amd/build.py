@@ -79,7 +79,25 @@ def build(verbose: bool = False) -> Path:
     _run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)], verbose)
     os.replace(tmp, out)
     build_shims(verbose)
+    build_host(verbose)
     return out
+
+
+def build_host(verbose: bool = False) -> None:
+    """The host side of a StePS run (include/steps_host.h): the parameter-file
+    and snapshot library libsteps_host.so, and the steps_run driver linked
+    with it and with libsteps_amd.so."""
+    host = CSRC / "host"
+    common = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", f"-I{host}"]
+    lib = LIBDIR / "libsteps_host.so"
+    tmp = LIBDIR / "libsteps_host.so.tmp"
+    _run([*common, "-fPIC", "-shared", str(host / "params.cc"), str(host / "snapio.cc"), "-o", str(tmp)], verbose)
+    os.replace(tmp, lib)
+    exe = LIBDIR / "steps_run"
+    tmp = LIBDIR / "steps_run.tmp"
+    _run([*common, str(host / "run.cc"), f"-L{LIBDIR}", "-lsteps_host", "-lsteps_amd", "-Wl,-rpath,$ORIGIN",
+          "-o", str(tmp)], verbose)
+    os.replace(tmp, exe)
 
 
 def build_shims(verbose: bool = False) -> None:

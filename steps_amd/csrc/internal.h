@@ -87,6 +87,7 @@ struct EvalPlan {
 size_t partial_bytes(const EvalPlan &p);
 size_t tile_extra_bytes(const EvalPlan &p);
 constexpr int kTile = 64;  // sources per tile (one wave64 prepares one tile)
+constexpr int kSub = 8;    // sources per sub-tile of the mixed S1R2 table kernels
 
 // Launch the tile pass, the pair kernel and the fixed-order reduction; F is
 // device AoS [3*nrows].  Non-null events are recorded right before and after

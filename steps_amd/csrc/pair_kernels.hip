@@ -67,6 +67,7 @@ struct PairArgs {
     const float4 *jrel;
     const double4 *tiles;
     const double4 *tile_ext;  // half extents of the tiles' bounding boxes (sorted sources)
+    const double4 *sub;       // centres of the 8-source sub-tiles (mixed S1R2 CIC / TSC)
     const double *table;
     const float2 *tab32;      // FP32 S1R2 table (mixed)
     const float4 *t3tab32;    // FP32 T3 table, (Dx, Dy, Dz, 0) per node (mixed)
@@ -1145,6 +1146,7 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
     // (comoving term included).  CIC kept Morton tiles and global reads:
     // with the window it ran at 582 against 681.
     constexpr bool SWIN = V == PV_S1R2_TSC;
+    constexpr bool SUBT = V == PV_S1R2_CIC || V == PV_S1R2_TSC;  // sub_tiles(): jrel per 8-source sub-tile
     __shared__ __align__(16) float2 s1win[SWIN ? 2 * kS1R2WinNodes : 1];
     __shared__ int s1rng[4];
     if (SWIN) {
@@ -1238,6 +1240,74 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
                 }
                 continue;
             }
+        }
+        if (SUBT) {
+            // S1R2 CIC / TSC: per sub-tile of kSub sources, the row's FP32
+            // offset from the sub-tile's centre (its z image nearest that
+            // centre) and FP32 sums over kSub pairs, added to the FP64
+            // accumulators after each sub-tile.  Against whole-tile offsets
+            // and sums this cut the error against FP64 at N = 524288 with the
+            // comoving term (CIC p99 1.5e-5 -> 8.9e-6, TSC 2.0e-5 -> 9.2e-6;
+            // the TSC tiles span half the box in z, so their offsets carried
+            // ulp(Lz/4)), for 4-7% of the speed (1.76x / 1.64x FP64).
+            for (int s0 = 0; s0 < cnt; s0 += kSub) {
+                const int scnt = min(kSub, cnt - s0);
+                const double4 cs = a.sub[(t0 + s0) / kSub];
+                float xs[NI], ys[NI], zs[NI];
+#pragma unroll
+                for (int q = 0; q < NI; ++q) {
+                    xs[q] = (float)(xi[q] - cs.x);
+                    ys[q] = (float)(yi[q] - cs.y);
+                    zs[q] = (float)(near_image(zi[q], cs.z, k) - cs.z);
+                    ax[q] = ay[q] = az[q] = 0.0f;
+                }
+                const float4 *__restrict__ ts = tr + s0;
+                bool near = false;
+                if (scnt == kSub) {
+#pragma unroll
+                    for (int j = 0; j < kSub; j += UNR) {
+                        float4 pj[UNR];
+#pragma unroll
+                        for (int u = 0; u < UNR; ++u) pj[u] = ts[j + u];
+#pragma unroll
+                        for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                            for (int q = 0; q < NI; ++q)
+                                mixed_pair_z<V, SWIN>(k, tab32, pj[u], xs[q], ys[q], zs[q], B[q], near, ax[q], ay[q],
+                                                      az[q], sw);
+                    }
+                } else {
+                    for (int j = 0; j < scnt; ++j) {
+                        const float4 pj = ts[j];
+#pragma unroll
+                        for (int q = 0; q < NI; ++q)
+                            mixed_pair_z<V, SWIN>(k, tab32, pj, xs[q], ys[q], zs[q], B[q], near, ax[q], ay[q], az[q],
+                                                  sw);
+                    }
+                }
+                if (__builtin_expect(near, 0)) {
+                    for (int jj = 0; jj < scnt; ++jj) {
+                        const float4 pr = ts[jj];
+#pragma unroll
+                        for (int q = 0; q < NI; ++q) {
+                            float dx, dy, dz;
+                            if (mixed_r2_z<V>(k, pr, xs[q], ys[q], zs[q], dx, dy, dz) < B[q]) {
+                                const double b = hi[q] + c.w;
+                                const int64_t jg = t0 + s0 + jj;
+                                pair_f64<V>(k, table, soft, jg, jrec[jg], xi[q], yi[q], zi[q], hi[q], b * b, fx[q],
+                                            fy[q], fz[q]);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NI; ++q) {
+                    fx[q] += (double)ax[q];
+                    fy[q] += (double)ay[q];
+                    fz[q] += (double)az[q];
+                }
+            }
+            continue;
         }
         bool near = false;
         int j = 0;
@@ -1891,17 +1961,20 @@ __global__ void unpack_positions_kernel(const double4 *__restrict__ jrec, double
 // receives each source as (x, y, z, -1.5 m), the records mr3_scaled reads; it
 // may be jrec itself (each lane reads its record before it stores, and the
 // padding lanes that re-read the first record of a tile store nothing).
-template <bool MIXED>
+template <bool MIXED, bool SUB = false>
 __global__ __launch_bounds__(kTile) void tiles_kernel(const double4 *jrec,
                                                      const double *__restrict__ soft,
                                                      double4 *__restrict__ tiles,
                                                      float4 *__restrict__ jrel, double4 *__restrict__ ext,
-                                                     double4 *scaled, int64_t n) {
+                                                     double4 *scaled, double4 *__restrict__ sub, int64_t n) {
     const int64_t t = blockIdx.x;
     const int lane = threadIdx.x;
     const int64_t i = t * kTile + lane;
     const bool ok = i < n;
-    const int64_t src = ok ? i : t * kTile;  // t*kTile < n: padding lanes repeat the first source
+    // padding lanes repeat the first source of their sub-tile, or of the tile
+    // (t*kTile < n) when the whole sub-tile is padding
+    const int64_t g0 = i - (lane % kSub);
+    const int64_t src = ok ? i : (g0 < n ? g0 : t * kTile);
     const double4 p = jrec[src];
     double lo[3] = {p.x, p.y, p.z}, hi[3] = {p.x, p.y, p.z};
     double hm = soft[src];
@@ -1920,8 +1993,23 @@ __global__ __launch_bounds__(kTile) void tiles_kernel(const double4 *jrec,
     if (ext && lane == 0)
         ext[t] = make_double4((hi[0] - lo[0]) * (0.5 + 1e-15), (hi[1] - lo[1]) * (0.5 + 1e-15),
                               (hi[2] - lo[2]) * (0.5 + 1e-15), 0.0);
-    if (MIXED && ok)
+    if (MIXED && !SUB && ok)
         jrel[i] = make_float4((float)(p.x - cx), (float)(p.y - cy), (float)(p.z - cz), (float)p.w);
+    if (SUB) {
+        // the bounding-box centre of each group of kSub consecutive sources,
+        // and each source's FP32 offset from its group's centre
+        double sl[3] = {p.x, p.y, p.z}, sh[3] = {p.x, p.y, p.z};
+#pragma unroll
+        for (int m = kSub / 2; m > 0; m >>= 1)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                sl[a] = fmin(sl[a], __shfl_xor(sl[a], m, kTile));
+                sh[a] = fmax(sh[a], __shfl_xor(sh[a], m, kTile));
+            }
+        const double sx = 0.5 * (sl[0] + sh[0]), sy = 0.5 * (sl[1] + sh[1]), sz = 0.5 * (sl[2] + sh[2]);
+        if (ok && lane % kSub == 0) sub[i / kSub] = make_double4(sx, sy, sz, 0.0);
+        if (ok) jrel[i] = make_float4((float)(p.x - sx), (float)(p.y - sy), (float)(p.z - sz), (float)p.w);
+    }
     if (scaled && ok) scaled[i] = make_double4(p.x, p.y, p.z, -1.5 * p.w);
 }
 
@@ -1991,6 +2079,14 @@ bool sorted_sources(const EvalPlan &p) {
            p.variant == PV_T3_EWALD;
 }
 bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
+// The mixed S1R2 CIC / TSC kernels take their FP32 source offsets relative to
+// the centres of 8-source sub-tiles (tiles_kernel<true, true>).
+bool sub_tiles(const EvalPlan &p) {
+    return is_mixed(p) && (p.variant == PV_S1R2_CIC || p.variant == PV_S1R2_TSC);
+}
+size_t sub_bytes(const EvalPlan &p) {
+    return align256((size_t)std::max<int64_t>((p.N + kSub - 1) / kSub, 1) * sizeof(double4));
+}
 // The FP64 R3 and nearest-image kernels read source records whose mass slot
 // holds -1.5 m (mr3_scaled): the sorted copy, scaled in place, or a scaled
 // copy of the input-order records.
@@ -2037,6 +2133,7 @@ struct ScratchView {
     double *part;
     double4 *tiles;
     double4 *tile_ext;
+    double4 *sub;
     float4 *jrel;
     float2 *tab32;
     double4 *jrec_s;
@@ -2052,7 +2149,8 @@ ScratchView scratch_view(const EvalPlan &p, double *partials) {
     v.part = partials;
     v.tiles = (double4 *)((char *)partials + align256(partial_bytes(p)));
     v.tile_ext = (double4 *)((char *)v.tiles + tb);
-    v.jrel = (float4 *)((char *)v.tile_ext + tb);
+    v.sub = (double4 *)((char *)v.tile_ext + tb);
+    v.jrel = (float4 *)((char *)v.sub + sub_bytes(p));
     v.tab32 = (float2 *)((char *)v.jrel + align256(n * sizeof(float4)));
     v.jrec_s = (double4 *)((char *)v.tab32 + align256(table32_bytes(p)));
     v.soft_s = (double *)((char *)v.jrec_s + align256(n * sizeof(double4)));
@@ -2102,6 +2200,7 @@ PairArgs make_args(const EvalPlan &p, const DeviceParticles &dp, const ScratchVi
     a.jrel = v.jrel;
     a.tiles = v.tiles;
     a.tile_ext = v.tile_ext;
+    a.sub = v.sub;
     a.table = (p.variant == PV_T3_EWALD) ? p.t3 : (is_s1r2_lookup(p.variant) ? p.s1r2 : nullptr);
     a.tab32 = v.tab32;
     a.t3tab32 = (const float4 *)v.tab32;  // the same scratch region: one mixed table per evaluation
@@ -2194,7 +2293,7 @@ size_t partial_bytes(const EvalPlan &p) {
 size_t tile_extra_bytes(const EvalPlan &p) {
     const int64_t nt = (p.N + kTile - 1) / kTile;
     const size_t n = (size_t)std::max<int64_t>(p.N, 1);
-    size_t b = 2 * align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4));
+    size_t b = 2 * align256((size_t)std::max<int64_t>(nt, 1) * sizeof(double4)) + sub_bytes(p);
     if (sorted_sources(p) || is_mixed(p) || scaled_sources(p)) {  // FP32 records relative to the tiles, FP32 table
         b += align256(n * sizeof(float4));
         b += align256(table32_bytes(p));
@@ -2229,24 +2328,30 @@ hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, d
                                       pencil_order(p) ? p.L : 0.0, pencil_order(p) ? p.t3_ng : 0)) != hipSuccess)
             return e;
         double4 *scaled = scaled_sources(p) ? v.jrec_s : nullptr;  // in place
-        if (is_mixed(p))
+        if (sub_tiles(p))
+            hipLaunchKernelGGL((tiles_kernel<true, true>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles,
+                               v.jrel, v.tile_ext, scaled, v.sub, p.N);
+        else if (is_mixed(p))
             hipLaunchKernelGGL((tiles_kernel<true>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles,
-                               v.jrel, v.tile_ext, scaled, p.N);
+                               v.jrel, v.tile_ext, scaled, v.sub, p.N);
         else
             hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, v.jrec_s, v.soft_s, v.tiles,
-                               v.jrel, v.tile_ext, scaled, p.N);
+                               v.jrel, v.tile_ext, scaled, v.sub, p.N);
     } else {
         // tile extents too: the R3 kernel's far-tile test reads them (on unsorted
         // tiles it rarely passes, and the per-pair loop runs), and so do the
         // S1R2 table windows; the mixed TSC kernel takes its FP32 records
         // relative to input-order tiles
         double4 *scaled = scaled_sources(p) ? v.jrec_s : nullptr;
-        if (is_mixed(p))
+        if (sub_tiles(p))
+            hipLaunchKernelGGL((tiles_kernel<true, true>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles,
+                               v.jrel, v.tile_ext, scaled, v.sub, p.N);
+        else if (is_mixed(p))
             hipLaunchKernelGGL((tiles_kernel<true>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles, v.jrel,
-                               v.tile_ext, scaled, p.N);
+                               v.tile_ext, scaled, v.sub, p.N);
         else
             hipLaunchKernelGGL((tiles_kernel<false>), dim3(nt), dim3(kTile), 0, st, dp.jrec, dp.soft, v.tiles,
-                               v.jrel, v.tile_ext, scaled, p.N);
+                               v.jrel, v.tile_ext, scaled, v.sub, p.N);
     }
     return hipGetLastError();
 }

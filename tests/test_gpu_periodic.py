@@ -518,3 +518,80 @@ def test_table_edited_in_place_is_uploaded_again(steps, t3_tab):
     assert np.array_equal(F1, F2)
     live[idx] = t3_tab[idx]                           # and back
     assert np.array_equal(steps.force_eval(d, x, M, h), F0)
+
+
+@pytest.mark.parametrize("is_periodic", [3, 4])
+def test_device_t3_table_finer_grids(steps, oracle, is_periodic):
+    """IS_PERIODIC 3 and 4 (main.cc:430-447): 127^3 and 255^3 tables with
+    wider cuts, built on the device; sampled nodes against the oracle's Ewald
+    correction evaluated directly at the node (ewald_space.cc:210-286)."""
+    print(f"case t3 table is_periodic={is_periodic}", flush=True)
+    L = L_T3
+    T = steps.t3_table(is_periodic, L)
+    alpha, rel, rec, ng = oracle.t3_params(is_periodic, L)
+    assert ng == (127 if is_periodic == 3 else 255) and T.size == ng ** 3 * 3
+    T = T.reshape(ng, ng, ng, 3)
+    scale = np.abs(T).max()
+    rng = np.random.default_rng(is_periodic)
+    idx = rng.integers(0, ng, size=(120, 3))
+    idx[:4] = [[0, 0, 0], [ng - 1, ng // 2, 3], [ng // 2, ng // 2, ng // 2], [1, ng - 2, ng // 3]]
+    worst = 0.0
+    for i, j, k in idx:
+        r = (np.array([i, j, k]) + 0.5) * (L / ng) - 0.5 * L
+        D = oracle.t3_correction(r, L, alpha, rel, rec)
+        worst = max(worst, np.abs(T[i, j, k] - D).max() / scale)
+    print(f"[T3 table is_periodic={is_periodic}] sampled max |dT|/max|T| = {worst:.3e}", flush=True)
+    assert worst <= 1e-11
+
+
+@pytest.mark.parametrize("is_periodic", [3, 4])
+def test_device_s1r2_table_finer_grids(steps, oracle, is_periodic):
+    """IS_PERIODIC 3 and 4 (main.cc:574-602): Nz = 256 and 512 with wider
+    cuts; sampled nodes against the oracle's singly periodic pair force minus
+    the nearest-image Newton force (ewald_space.cc:754-798)."""
+    print(f"case s1r2 table is_periodic={is_periodic}", flush=True)
+    Lz, Rsim = 200.0, 1000.0
+    T, (nrho, nz) = steps.s1r2_table(is_periodic, Lz, Rsim)
+    alpha, rel, rec, nz_o, nrho_o = oracle.s1r2_params(is_periodic, Lz, Rsim)
+    assert (nrho, nz) == (nrho_o, nz_o) and nz == (256 if is_periodic == 3 else 512)
+    T = T.reshape(nrho, nz, 2)
+    scale = np.abs(T).max()
+    drho, dzg = 2.25 * Rsim / (nrho - 1), Lz / nz
+    rng = np.random.default_rng(10 + is_periodic)
+    worst = 0.0
+    for ir, iz in list(zip(rng.integers(0, nrho, 150), rng.integers(0, nz, 150))) + [(0, 0), (nrho - 1, nz - 1)]:
+        rho, z = ir * drho, (iz + 0.5) * dzg - 0.5 * Lz
+        Fp = oracle.s1r2_force_pair((rho, 0.0, z), Lz, alpha, rel, rec)
+        r2 = rho * rho + z * z                       # |z| < Lz/2: z is its own nearest image
+        newton = np.array([-rho, -z]) * r2 ** -1.5 if r2 > 0 else np.zeros(2)
+        worst = max(worst, np.abs(T[ir, iz] - (np.array([Fp[0], Fp[2]]) - newton)).max() / scale)
+    print(f"[S1R2 table is_periodic={is_periodic}] sampled max |dT|/max|T| = {worst:.3e}", flush=True)
+    assert worst <= 1e-10
+
+
+def test_forces_with_finer_tables(steps, oracle):
+    """The pair kernels on the IS_PERIODIC = 3 tables (T3 127^3: the global-
+    table kernel, the LDS window needs Ng <= 119; S1R2 Nz = 256, CIC and TSC)
+    against the oracle on the same tables, sampled rows."""
+    print("case forces with is_periodic=3 tables", flush=True)
+    w = wl.t3_lattice(n_side=24, seed=1001, is_periodic=3)
+    T = steps.t3_table(3, w.L)
+    d, keep = steps.make_desc(steps.TOPO_T3, is_periodic=3, L=w.L, t3_table=T)
+    F = steps.force_eval(d, w.x, w.M, w.soft)
+    rows = np.linspace(0, w.N - 1, 48).astype(np.int32)
+    p = oracle.params(1, is_periodic=3, L=w.L, t3_table=T, interp_order=4)
+    e = rel_err(F[rows], oracle.forces_rows(p, w.x, w.M, w.soft, rows))
+    report("T3 is_periodic=3 (127^3) sampled rows", e)
+    assert np.median(e) <= 1e-12 and e.max() <= 1e-10
+    c = wl.s1r2_cylinder(n_omega=16, n_phi=8, n_z=16, seed=6, is_periodic=3)
+    Ts, shape = steps.s1r2_table(3, c.L, c.Rsim)
+    kw = dict(is_periodic=3, L=c.L, Rsim=c.Rsim, cosmology=1, comoving=1, mass_in_unit_sphere=c.mass_in_unit_sphere)
+    rows = np.linspace(0, c.N - 1, 64).astype(np.int32)
+    for order in (2, 4):
+        d, keep = steps.make_desc(steps.TOPO_S1R2, s1r2_lookup=1, interp_order=order, s1r2_table=Ts,
+                                  s1r2_shape=shape, **kw)
+        F = steps.force_eval(d, c.x, c.M, c.soft)
+        p = oracle.params(2, s1r2_lookup=1, interp_order=order, s1r2_table=Ts, s1r2_shape=shape, **kw)
+        e = rel_err(F[rows], oracle.forces_rows(p, c.x, c.M, c.soft, rows))
+        report(f"S1R2 is_periodic=3 (Nz=256) order={order} sampled rows", e)
+        assert np.median(e) <= 1e-12 and e.max() <= 1e-10

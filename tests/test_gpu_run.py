@@ -1,0 +1,76 @@
+"""steps_run, the driver that runs a StePS parameter file on the device: a
+compactified R3 comoving run from an ASCII initial-condition file, checked
+against the oracle advanced the same way (the same step lengths from the
+same errmax rule, the scale factor by RK4), and the snapshot's units."""
+import ctypes as C
+import math
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from steps_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parent.parent
+UNIT_T, UNIT_V = 47.14829951063323, 20.738652969925447
+
+
+def test_steps_run_r3_comoving_matches_oracle(steps, oracle, tmp_path):
+    print("case steps_run R3 comoving N=2048 from an ASCII IC", flush=True)
+    w = wl.r3_compactified(N=2048, seed=21)
+    a0, a_max, acc = 1.0 / 32, 0.034, 0.0125
+    h_min, h_max = 2e-5 / UNIT_T, 0.03125 / UNIT_T
+    ic = tmp_path / "ic.dat"
+    # the IC convention: velocities in km/s * sqrt(a) (main.cc:1147-1154)
+    with open(ic, "w") as f:
+        for i in range(w.N):
+            row = list(w.x[i]) + list(w.v[i] * math.sqrt(a0) * UNIT_V) + [w.M[i]]
+            f.write("\t".join("%.17g" % c for c in row) + "\n")
+    out = tmp_path / "out"
+    out.mkdir()
+    par = tmp_path / "run.param"
+    par.write_text("\n".join([
+        "Omega_b\t0.0", f"Omega_lambda\t{wl.OMEGA_L!r}", f"Omega_m\t{wl.OMEGA_M!r}", "Omega_r\t0.0",
+        f"HubbleConstant\t{wl.H0 * UNIT_V!r}", f"a_start\t{a0!r}", f"a_max\t{a_max!r}",
+        "COSMOLOGY\t1", "IS_PERIODIC\t0", "COMOVING_INTEGRATION\t1", "L_BOX\t100.0", "R_SIM\t1000.0",
+        f"IC_FILE\t{ic}", "IC_FORMAT\t0", f"OUT_DIR\t{out}", "OUTPUT_TIME_VARIABLE\t0", "OUTPUT_FORMAT\t0",
+        f"ACC_PARAM\t{acc!r}", "STEP_MIN\t0.00002", "STEP_MAX\t0.03125", f"PARTICLE_RADII\t{w.R0!r}",
+        "FIRST_T_OUT\t13.0", "H_OUT\t13.0", ""]))
+    exe = ROOT / "steps_amd" / "lib" / "steps_run"
+    r = subprocess.run([str(exe), str(par), "--max-steps", "40"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-600:], r.stderr[-600:], flush=True)
+    assert r.returncode == 0
+    log = np.loadtxt(out / "Logfile.dat")
+    nsteps = log.shape[0]
+    assert 2 <= nsteps < 40
+
+    # the oracle, the same run
+    kw = dict(cosmology=1, comoving=1, mass_in_unit_sphere=4 * math.pi / 3 * 3 * wl.H0 ** 2 / (8 * math.pi) * wl.OMEGA_M)
+    p = oracle.params(0, **kw)
+    cosmo = oracle.OracleCosmo(wl.H0, wl.OMEGA_M, wl.OMEGA_L, 0.0, 1.0 - wl.OMEGA_M - wl.OMEGA_L)
+    lib = oracle.lib()
+    soft, M_min, rho_part = steps.softening_lengths(w.M, w.R0)   # utils.cc:59-82, as the driver does
+    x = w.x.reshape(-1).copy()
+    v = (w.v.reshape(-1) * math.sqrt(a0) * UNIT_V) / math.sqrt(a0) / UNIT_V
+    F = oracle.forces(p, w.x, w.M, soft).reshape(-1).copy()
+    st = oracle.OracleStepState(a0, lib.oracle_hubble(C.byref(cosmo), a0), 0.0, rho_part, 1.0)
+    e = lib.oracle_initial_errmax(C.byref(p), C.byref(st), w.N, oracle.dp(x.copy()), oracle.dp(v), oracle.dp(F),
+                                  oracle.dp(w.M))
+    hs = []
+    while st.a < a_max and len(hs) < 40:
+        h = min(max(math.sqrt(2 * acc / e), h_min), h_max)
+        hs.append(h)
+        st.h = h
+        e = lib.oracle_kdk_step(C.byref(p), C.byref(cosmo), C.byref(st), w.N, oracle.dp(x), oracle.dp(v),
+                                oracle.dp(F), oracle.dp(w.M), oracle.dp(soft))
+    assert len(hs) == nsteps
+    assert np.allclose(log[:, 5], hs, rtol=1e-9)
+    assert log[-1, 2] == pytest.approx(st.a, rel=1e-12)
+    snap = np.loadtxt(out / "t1300.dat")
+    assert snap.shape == (w.N, 7)
+    assert np.allclose(snap[:, :3], x.reshape(-1, 3), rtol=0, atol=1e-10 * np.abs(x).max())
+    vs = v.reshape(-1, 3) * math.sqrt(st.a) * UNIT_V
+    assert np.allclose(snap[:, 3:6], vs, rtol=1e-8, atol=1e-10 * np.abs(vs).max())
+    assert np.allclose(snap[:, 6], w.M, rtol=1e-15)
