@@ -99,6 +99,22 @@ const char *steps_version(void);
 int steps_force_eval(const steps_force_desc *d, int64_t N, const double *x, const double *M,
                      const double *soft, double *F, int64_t id_min, int64_t id_max);
 
+/* steps_force_eval without any device allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * (bitwise) and error behaviour as steps_force_eval, but every device buffer
+ * the evaluation needs (staged inputs, packed records, forces, the pair
+ * kernels' scratch, the tables' device copies) comes from one static device
+ * array of STEPS_NOALLOC_SCRATCH_MB MiB per device, held until the process
+ * ends; an evaluation that needs more (R3 FP64: up to N of about 2^18)
+ * fails with STEPS_ERR_ARG and says how many bytes it needed.  Calls are
+ * serialised within the process; all calls on a device share the array,
+ * so calls that overlap on one device (other processes) race.  Work runs on
+ * the null stream. */
+#define STEPS_NOALLOC_SCRATCH_MB 256
+int steps_force_eval_noalloc(const steps_force_desc *d, int64_t N, const double *x, const double *M,
+                             const double *soft, double *F, int64_t id_min, int64_t id_max);
+
 /* ---- resident simulation (state kept in HBM between calls) ---------------- */
 typedef struct steps_sim steps_sim;
 

@@ -29,7 +29,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 # every symbol include/steps_force.h declares (checked by tests/test_abi.py)
 EXPORTED = [
     "steps_force_desc_init", "steps_device_count", "steps_last_error", "steps_version",
-    "steps_force_eval", "steps_sim_create", "steps_sim_destroy", "steps_sim_set_precision", "steps_sim_set_state",
+    "steps_force_eval", "steps_force_eval_noalloc", "steps_sim_create", "steps_sim_destroy", "steps_sim_set_precision", "steps_sim_set_state",
     "steps_sim_set_positions", "steps_sim_get_state", "steps_sim_forces", "steps_sim_get_forces",
     "steps_sim_synchronize", "steps_sim_last_kernel_ms", "steps_sim_step", "steps_sim_errmax",
     "steps_t3_table", "steps_s1r2_table", "steps_radial_table", "steps_softening_lengths",
@@ -121,6 +121,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "steps_last_error": (C.c_char_p, []),
         "steps_version": (C.c_char_p, []),
         "steps_force_eval": (C.c_int, [C.POINTER(ForceDesc), i64, _DP, _DP, _DP, _DP, i64, i64]),
+        "steps_force_eval_noalloc": (C.c_int, [C.POINTER(ForceDesc), i64, _DP, _DP, _DP, _DP, i64, i64]),
         "steps_sim_create": (C.c_int, [C.POINTER(ForceDesc), i64, C.POINTER(vp)]),
         "steps_sim_set_precision": (C.c_int, [vp, C.c_int32]),
         "steps_sim_destroy": (C.c_int, [vp]),

@@ -166,6 +166,8 @@ int kdk_reduce_blocks(int64_t n);
 // kernel reads a node below lo at its mirror Ng-1-i with that component
 // negated), else 0.  Returns lo.
 int t3_device_layout(const double *T, int ng, std::vector<double> &out);
+// The lo t3_device_layout picks for this table (Ng/2 when it folds, else 0).
+int t3_layout_lo(const double *T, int ng);
 inline size_t t3_device_entries(int ng, int lo) {
     return (size_t)(ng - lo) * (size_t)(ng - lo) * (size_t)(ng + 3) * 3u;
 }

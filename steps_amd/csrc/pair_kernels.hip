@@ -2251,8 +2251,10 @@ static bool t3_antisymmetric_xy(const double *T, int ng) {
     return true;
 }
 
+int t3_layout_lo(const double *T, int ng) { return t3_antisymmetric_xy(T, ng) ? ng / 2 : 0; }
+
 int t3_device_layout(const double *T, int ng, std::vector<double> &out) {
-    const int lo = t3_antisymmetric_xy(T, ng) ? ng / 2 : 0;
+    const int lo = t3_layout_lo(T, ng);
     const size_t n = (size_t)ng, nx = (size_t)(ng - lo), nz = n + 3;
     out.resize(t3_device_entries(ng, lo));
     for (size_t mx = 0; mx < nx; ++mx)
