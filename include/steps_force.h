@@ -79,6 +79,10 @@ typedef struct steps_force_desc {
     int32_t radial_table_size;
     int32_t precision;           /* enum steps_precision                                  */
     int32_t device;              /* HIP device ordinal                                    */
+    double bh_theta;             /* > 0: Barnes-Hut octree forces with this opening angle
+                                    (USE_BH, main.cc:303; R3 and FP64 only); 0: direct sum */
+    double bh_particle_radius;   /* ParticleRadi: a node's softening length is
+                                    cbrt(mass / M_min) * ParticleRadi + h_i (forces.cc:386) */
 } steps_force_desc;
 
 /* Fill *d with the defaults (R3, FP64, no cosmology, device 0). */

@@ -1057,3 +1057,129 @@ double oracle_kdk_step(const oracle_params *p, const oracle_cosmo *c, oracle_ste
     }
     return errmax;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Barnes-Hut octree (forces.cc:97-187, 321-506)                            */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double cx, cy, cz, size;   /* cube centre and side */
+    double mass, com[3];
+    int pidx;                  /* particle of a leaf, -1 for internal or empty */
+    int child[8];              /* -1: none */
+} orc_bh_node;
+
+typedef struct {
+    orc_bh_node *n;
+    int count, cap;
+} orc_bh_tree;
+
+static int orc_bh_new(orc_bh_tree *t, double cx, double cy, double cz, double size)
+{
+    if (t->count == t->cap) {
+        int cap = t->cap ? 2 * t->cap : 1024;
+        orc_bh_node *m = (orc_bh_node *)realloc(t->n, sizeof(orc_bh_node) * (size_t)cap);
+        if (!m) return -1;
+        t->n = m;
+        t->cap = cap;
+    }
+    orc_bh_node *q = &t->n[t->count];
+    q->cx = cx; q->cy = cy; q->cz = cz; q->size = size;
+    q->mass = 0; q->com[0] = q->com[1] = q->com[2] = 0;
+    q->pidx = -1;
+    for (int k = 0; k < 8; ++k) q->child[k] = -1;
+    return t->count++;
+}
+
+/* forces.cc:121-128: x > cx sets bit 0, y > cy bit 1, z > cz bit 2 */
+static int orc_bh_octant(const orc_bh_node *q, const double *x, int i)
+{
+    return (x[3 * i] > q->cx ? 1 : 0) | (x[3 * i + 1] > q->cy ? 2 : 0) | (x[3 * i + 2] > q->cz ? 4 : 0);
+}
+
+/* forces.cc:130-174 (recursive insertion; the incremental centre of mass) */
+static int orc_bh_insert(orc_bh_tree *t, int node, const double *x, const double *M, int i, int depth)
+{
+    if (depth > 64) return -5;
+    orc_bh_node *q = &t->n[node];
+    if (q->mass == 0 && q->pidx == -1) {
+        q->pidx = i;
+        q->mass = M[i];
+        q->com[0] = x[3 * i]; q->com[1] = x[3 * i + 1]; q->com[2] = x[3 * i + 2];
+        return 0;
+    }
+    if (q->pidx != -1) {
+        int existing = q->pidx;
+        q->pidx = -1;
+        for (int j = 0; j < 8; ++j) {
+            q = &t->n[node];
+            double off = q->size / 4;
+            int c = orc_bh_new(t, q->cx + ((j & 1) ? off : -off), q->cy + ((j & 2) ? off : -off),
+                               q->cz + ((j & 4) ? off : -off), q->size / 2);
+            if (c < 0) return -1;
+            t->n[node].child[j] = c;
+        }
+        q = &t->n[node];
+        int rc = orc_bh_insert(t, q->child[orc_bh_octant(q, x, existing)], x, M, existing, depth + 1);
+        if (rc) return rc;
+    }
+    q = &t->n[node];
+    int rc = orc_bh_insert(t, q->child[orc_bh_octant(q, x, i)], x, M, i, depth + 1);
+    if (rc) return rc;
+    q = &t->n[node];
+    double total = q->mass + M[i];
+    for (int a = 0; a < 3; ++a) q->com[a] = (q->com[a] * q->mass + x[3 * i + a] * M[i]) / total;
+    q->mass = total;
+    return 0;
+}
+
+/* forces.cc:374-398 */
+static void orc_bh_force(const orc_bh_tree *t, int node, const double *x, int i, const double *h, double theta,
+                         double pr, double M_min, double *f)
+{
+    if (node < 0) return;
+    const orc_bh_node *q = &t->n[node];
+    if (q->mass == 0 || q->pidx == i) return;
+    double dx = q->com[0] - x[3 * i], dy = q->com[1] - x[3 * i + 1], dz = q->com[2] - x[3 * i + 2];
+    double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (q->pidx != -1 || q->size / dist < theta) {
+        double beta = cbrt(q->mass / M_min) * pr + h[i];
+        double w = q->mass * oracle_force_softening(dist, beta);
+        f[0] += w * dx; f[1] += w * dy; f[2] += w * dz;
+    } else {
+        for (int j = 0; j < 8; ++j) orc_bh_force(t, q->child[j], x, i, h, theta, pr, M_min, f);
+    }
+}
+
+int oracle_forces_bh(const oracle_params *p, double theta, double particle_radius, int N, const double *x,
+                     const double *M, const double *h, double *F, int id_min, int id_max)
+{
+    if (!p || N <= 0 || id_min < 0 || id_max >= N || id_max < id_min - 1) return -1;
+    if (p->topology != ORC_R3) return -4;
+    double rmax = 0, M_min = M[0];
+    for (int i = 0; i < N; ++i) {
+        double r = sqrt(pow(x[3 * i], 2) + pow(x[3 * i + 1], 2) + pow(x[3 * i + 2], 2));
+        if (r > rmax) rmax = r;
+        if (M_min > M[i]) M_min = M[i];
+    }
+    orc_bh_tree t = {NULL, 0, 0};
+    int rc = orc_bh_new(&t, 0.0, 0.0, 0.0, 2.00002 * rmax) < 0 ? -1 : 0; /* forces.cc:437-438 */
+    for (int i = 0; i < N && !rc; ++i) rc = orc_bh_insert(&t, 0, x, M, i, 0);
+    if (rc) {
+        free(t.n);
+        return rc;
+    }
+    const double DE = p->H0 * p->H0 * p->Omega_lambda;
+    int kk;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (kk = id_min; kk <= id_max; ++kk) {
+        double f[3] = {0, 0, 0};
+        orc_bh_force(&t, 0, x, kk, h, theta, particle_radius, M_min, f);
+        if (p->cosmology == 1) { /* forces.cc:474-491 */
+            double c = p->comoving == 1 ? p->mass_in_unit_sphere : DE;
+            for (int a = 0; a < 3; ++a) f[a] += c * x[3 * kk + a];
+        }
+        for (int a = 0; a < 3; ++a) F[3 * (kk - id_min) + a] = f[a];
+    }
+    free(t.n);
+    return 0;
+}

@@ -167,6 +167,19 @@ double oracle_initial_errmax(const oracle_params *p, oracle_step_state *s, int N
                              double *x, const double *v, const double *F,
                              const double *M);
 
+/* Barnes-Hut octree forces, free boundary (USE_BH build; forces.cc:97-187,
+ * 321-506): the reference's pointer octree built by inserting particles
+ * 0..N-1 into a root cube centred on the origin of side 2.00002 max|x_i|,
+ * then per row a recursive traversal (children 0..7) that takes a leaf, or a
+ * node with nodesize / dist < theta, at its centre of mass with softening
+ * cbrt(mass / M_min) * particle_radius + h_i, and opens the others; the own
+ * leaf and empty nodes are skipped.  Then the external term as for the
+ * direct sum (the radial BH correction table is not restated).  Rows
+ * id_min..id_max, F overwritten.  Returns 0, or -5 when two particles are too
+ * close to separate within 64 levels. */
+int oracle_forces_bh(const oracle_params *p, double theta, double particle_radius, int N, const double *x,
+                     const double *M, const double *h, double *F, int id_min, int id_max);
+
 #ifdef __cplusplus
 }
 #endif

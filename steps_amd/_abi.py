@@ -69,6 +69,8 @@ class ForceDesc(C.Structure):
         ("radial_table_size", C.c_int32),
         ("precision", C.c_int32),
         ("device", C.c_int32),
+        ("bh_theta", C.c_double),
+        ("bh_particle_radius", C.c_double),
     ]
 
 
@@ -184,9 +186,10 @@ def device_count() -> int:
 def make_desc(topology=TOPO_R3, *, is_periodic=0, cosmology=0, comoving=0, L=0.0, Rsim=0.0,
               mass_in_unit_sphere=0.0, H0=0.0, Omega_lambda=0.0, s1r2_lookup=1, interp_order=2,
               t3_table=None, s1r2_table=None, s1r2_shape=(0, 0), radial_table=None,
-              precision=PREC_FP64, device=0):
+              precision=PREC_FP64, device=0, bh_theta=0.0, bh_particle_radius=0.0):
     """Build a ForceDesc; returns (desc, keepalive) -- keep the second alive
-    as long as the desc is used (it owns the table arrays)."""
+    as long as the desc is used (it owns the table arrays).  bh_theta > 0
+    selects the Barnes-Hut octree (USE_BH) with that opening angle."""
     d = ForceDesc()
     load_library().steps_force_desc_init(C.byref(d))
     d.topology = topology
@@ -202,6 +205,8 @@ def make_desc(topology=TOPO_R3, *, is_periodic=0, cosmology=0, comoving=0, L=0.0
     d.interp_order = interp_order
     d.precision = precision
     d.device = device
+    d.bh_theta = bh_theta
+    d.bh_particle_radius = bh_particle_radius
     keep = []
     if t3_table is not None:
         t = f64(t3_table)

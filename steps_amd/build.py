@@ -28,7 +28,7 @@ SHIM_VARIANTS = {"r3": [], "t3": ["-DPERIODIC"], "s1r2": ["-DPERIODIC_Z", "-DEWA
                  "s1r2_nolookup": ["-DPERIODIC_Z", "-DPERIODIC_Z_NOLOOKUP"]}
 
 HIP_SOURCES = ["pair_kernels.hip", "engine.hip", "kdk_kernels.hip", "table_kernels.hip", "order_kernels.hip",
-               "server.hip", "noalloc.hip"]
+               "server.hip", "noalloc.hip", "bh_kernels.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

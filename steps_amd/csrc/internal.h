@@ -28,6 +28,7 @@ enum PairVariant : int {
     PV_S1R2_CIC = 5,      //   ... order 2
     PV_S1R2_TSC = 6,      //   ... order 4
     PV_S1R2_IMAGES = 7,   // forces_cuda.cu:711-759   (PERIODIC_Z_NOLOOKUP, IS_PERIODIC >= 2)
+    PV_R3_BH = 8,         // forces.cc:378-506        (USE_BH, free boundary; CPU-only in the reference)
 };
 
 constexpr bool is_s1r2_lookup(int v) { return v == PV_S1R2_NGP || v == PV_S1R2_CIC || v == PV_S1R2_TSC; }
@@ -79,6 +80,7 @@ struct EvalPlan {
     const double *s1r2 = nullptr; int s1r2_nrho = 0, s1r2_nz = 0;
     const double *radial = nullptr; int radial_size = 0;
     int images = 0;        // S1R2 image-sum: ewald_max
+    double bh_theta = 0, bh_particle_radius = 0;  // Barnes-Hut opening angle, ParticleRadi
 };
 
 // Scratch of one evaluation: partial sums [splits][3][nrows] (FP64), followed
@@ -131,6 +133,22 @@ hipError_t launch_spatial_order(const double4 *jrec, const double *soft, int64_t
 const uint32_t *spatial_order_perm(const void *scratch, int64_t n);
 // ... and spatial_order_keys(scratch, n)[k] its sort key.
 const uint32_t *spatial_order_keys(const void *scratch, int64_t n);
+
+// Stable sort of n 64-bit keys given as high and low 32-bit halves (device):
+// *perm receives (inside scratch, sort_u64_scratch_bytes(n) bytes) the input
+// index of the k-th smallest key.
+size_t sort_u64_scratch_bytes(int64_t n);
+hipError_t launch_sort_u64(const uint32_t *hi, const uint32_t *lo, int64_t n, void *scratch, const uint32_t **perm,
+                           hipStream_t stream);
+
+// Barnes-Hut octree forces (bh_kernels.hip): tree build and traversal for
+// rows id_min .. id_min+nrows-1 into F (device AoS, external term included);
+// `scratch` holds bh_scratch_bytes(N) bytes.  A non-null ev_tree_end is
+// recorded between the tree build and the traversal.
+int64_t bh_item_capacity(int64_t n);
+size_t bh_scratch_bytes(int64_t n);
+hipError_t launch_bh_forces(const EvalPlan &p, const DeviceParticles &dp, void *scratch, double *F,
+                            hipStream_t stream, hipEvent_t ev_tree_end);
 
 // AoS x [3n] + M [n] + soft [n] (device) -> DeviceParticles layout.
 hipError_t launch_pack(const double *x, const double *M, const double *soft, int64_t n,

@@ -241,7 +241,71 @@ __global__ __launch_bounds__(256) void gather_kernel(const double4 *__restrict__
     soft_s[i] = soft[s];
 }
 
+__global__ __launch_bounds__(256) void iota_copy_kernel(const uint32_t *__restrict__ kin, int64_t n,
+                                                        uint32_t *__restrict__ kout, uint32_t *__restrict__ v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    kout[i] = kin[i];
+    v[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void gather_u32_kernel(const uint32_t *__restrict__ src,
+                                                         const uint32_t *__restrict__ perm, int64_t n,
+                                                         uint32_t *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = src[perm[i]];
+}
+
+// One stable LSD pass over 8 bits of key[cur] (moving val[cur] along).
+hipError_t radix_pass(uint32_t *key[2], uint32_t *val[2], int &cur, int64_t n, int shift, uint32_t *counts,
+                      hipStream_t st) {
+    const int nblk = sort_blocks(n);
+    hipLaunchKernelGGL(radix_hist_kernel, dim3(nblk), dim3(kSortThreads), 0, st, key[cur], n, shift, nblk, counts);
+    hipError_t e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, counts, (int64_t)256 * nblk);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3(nblk), dim3(kSortThreads), 0, st, key[cur], val[cur], n, shift,
+                       counts, nblk, key[1 - cur], val[1 - cur]);
+    cur = 1 - cur;
+    return hipGetLastError();
+}
+
 }  // namespace
+
+size_t sort_u64_scratch_bytes(int64_t n) {
+    const size_t nn = (size_t)std::max<int64_t>(n, 1);
+    return 4 * a256(nn * 4) + a256((size_t)256 * sort_blocks(n) * 4) + 256;
+}
+
+hipError_t launch_sort_u64(const uint32_t *hi, const uint32_t *lo, int64_t n, void *scratch, const uint32_t **perm,
+                           hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const size_t nn = (size_t)n;
+    char *p = (char *)scratch;
+    uint32_t *key[2], *val[2];
+    key[0] = (uint32_t *)p;
+    key[1] = (uint32_t *)(p + a256(nn * 4));
+    val[0] = (uint32_t *)(p + 2 * a256(nn * 4));
+    val[1] = (uint32_t *)(p + 3 * a256(nn * 4));
+    uint32_t *counts = (uint32_t *)(p + 4 * a256(nn * 4));
+    const unsigned ng = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(iota_copy_kernel, dim3(ng), dim3(256), 0, st, lo, n, key[0], val[0]);
+    hipError_t e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    int cur = 0;
+    for (int shift = 0; shift < 32; shift += 8)
+        if ((e = radix_pass(key, val, cur, n, shift, counts, st)) != hipSuccess) return e;
+    // the high halves, in the order the low-half passes left: a stable sort
+    // by them completes the sort by (hi, lo)
+    hipLaunchKernelGGL(gather_u32_kernel, dim3(ng), dim3(256), 0, st, hi, val[cur], n, key[cur]);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    for (int shift = 0; shift < 32; shift += 8)
+        if ((e = radix_pass(key, val, cur, n, shift, counts, st)) != hipSuccess) return e;
+    *perm = val[cur];
+    return hipSuccess;
+}
 
 size_t order_scratch_bytes(int64_t n) {
     const size_t nn = (size_t)std::max<int64_t>(n, 1);

@@ -47,6 +47,7 @@ struct TableCopy {
 int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_max, EvalPlan &p);
 // The descriptor selects a variant with a mixed-precision kernel.
 inline bool desc_mixed_capable(const steps_force_desc &d) {
+    if (d.bh_theta != 0.0) return false;  // the octree is FP64 only
     return d.topology == STEPS_TOPO_R3 || d.topology == STEPS_TOPO_T3 ||
            (d.topology == STEPS_TOPO_S1R2 &&
             (d.is_periodic == 1 || (d.is_periodic >= 2 && d.s1r2_lookup && d.interp_order != 0)));

@@ -78,6 +78,9 @@ def lib() -> C.CDLL:
         getattr(L, name).restype = C.c_int
         getattr(L, name).argtypes = [C.POINTER(OracleParams), C.c_int, _DP, _DP, _DP, _DP, _IP, C.c_int]
     L.oracle_max_threads.restype = C.c_int
+    L.oracle_forces_bh.restype = C.c_int
+    L.oracle_forces_bh.argtypes = [C.POINTER(OracleParams), C.c_double, C.c_double, C.c_int, _DP, _DP, _DP, _DP,
+                                   C.c_int, C.c_int]
     L.oracle_forces_r3_longdouble.restype = None
     L.oracle_forces_r3_longdouble.argtypes = [C.POINTER(OracleParams), C.c_int, _DP, _DP, _DP,
                                               C.POINTER(C.c_longdouble)]
@@ -184,6 +187,21 @@ def forces(p: OracleParams, x, M, h, id_min=0, id_max=None) -> np.ndarray:
     n = max(0, id_max - id_min + 1)
     F = np.zeros(3 * max(n, 1))
     rc = lib().oracle_forces(C.byref(p), N, dp(x), dp(M), dp(h), dp(F), id_min, id_max)
+    assert rc == 0, rc
+    return F[: 3 * n].reshape(n, 3)
+
+
+def forces_bh(p: OracleParams, theta: float, particle_radius: float, x, M, h, id_min=0, id_max=None) -> np.ndarray:
+    """Barnes-Hut octree forces of the reference's USE_BH build (free boundary)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    N = M.size
+    if id_max is None:
+        id_max = N - 1
+    n = max(0, id_max - id_min + 1)
+    F = np.zeros(3 * max(n, 1))
+    rc = lib().oracle_forces_bh(C.byref(p), theta, particle_radius, N, dp(x), dp(M), dp(h), dp(F), id_min, id_max)
     assert rc == 0, rc
     return F[: 3 * n].reshape(n, 3)
 

@@ -1,0 +1,107 @@
+"""Barnes-Hut octree forces on the device (the reference's USE_BH engine,
+forces.cc:97-187, 378-506, CPU-only there) against the oracle's restatement
+of the reference's pointer octree.  The device builds the same tree (cubes
+holding two or more particles, children in octant order) and makes the same
+accept / open decision per (row, node), so the forces agree to the rounding
+of the centres of mass and of the sums: per-row relative error <= 1e-12,
+for several opening angles, row ranges, the external-term modes and
+particle counts from 1 up; and the octree with an opening angle so small
+that no node is accepted is the direct sum."""
+import numpy as np
+import pytest
+
+from conftest import rel_err, report
+from steps_amd import _abi
+from steps_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _desc(steps, w, theta, mode="comoving"):
+    kw = {}
+    if mode == "comoving":
+        kw = dict(cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere)
+    elif mode == "dark_energy":
+        kw = dict(cosmology=1, comoving=0, H0=wl.H0, Omega_lambda=wl.OMEGA_L)
+    return steps.make_desc(steps.TOPO_R3, bh_theta=theta, bh_particle_radius=w.particle_radius, **kw), kw
+
+
+@pytest.mark.parametrize("N,seed,theta,mode", [
+    (4096, 11, 0.5, "comoving"),
+    (4096, 12, 0.8, "none"),
+    (6000, 13, 0.25, "dark_energy"),
+    (65536, 14, 0.5, "comoving"),
+])
+def test_bh_matches_oracle(steps, oracle, N, seed, theta, mode):
+    print(f"case bh N={N} seed={seed} theta={theta} {mode}", flush=True)
+    w = wl.r3_compactified(N=N, seed=seed)
+    (d, keep), kw = _desc(steps, w, theta, mode)
+    p = oracle.params(0, **kw)
+    Fo = oracle.forces_bh(p, theta, w.particle_radius, w.x, w.M, w.soft)
+    F = steps.force_eval(d, w.x, w.M, w.soft)
+    e = rel_err(F, Fo)
+    report(f"bh N={N} theta={theta}", e)
+    assert e.max() <= 1e-12
+    lo, hi = N // 3, N // 3 + 777  # a row range: rows outside are not written
+    Fr = steps.force_eval(d, w.x, w.M, w.soft, id_min=lo, id_max=hi)
+    assert np.array_equal(Fr, F[lo:hi + 1])
+
+
+def test_bh_tiny_theta_is_the_direct_sum(steps):
+    print("case bh theta 1e-12 vs direct sum", flush=True)
+    w = wl.r3_compactified(N=3000, seed=15)
+    (d, keep), kw = _desc(steps, w, 1e-12)
+    dd, keep2 = steps.make_desc(steps.TOPO_R3, **kw)
+    e = rel_err(steps.force_eval(d, w.x, w.M, w.soft), steps.force_eval(dd, w.x, w.M, w.soft))
+    report("bh tiny theta vs direct", e)
+    assert e.max() <= 1e-11
+
+
+def test_bh_small_counts(steps, oracle):
+    for N in (1, 2, 3, 9, 64, 65, 257):
+        print(f"case bh small N={N}", flush=True)
+        w = wl.r3_compactified(N=max(N, 16), seed=16)
+        x, M, h = w.x[:N].copy(), w.M[:N].copy(), w.soft[:N].copy()
+        (d, keep), kw = _desc(steps, w, 0.6, "none")
+        Fo = oracle.forces_bh(oracle.params(0), 0.6, w.particle_radius, x, M, h)
+        F = steps.force_eval(d, x, M, h)
+        if N == 1:
+            assert np.array_equal(F, np.zeros((1, 3)))
+        else:
+            assert rel_err(F, Fo).max() <= 1e-12, N
+
+
+def test_bh_resident_sim_and_repeatability(steps, oracle):
+    print("case bh resident sim", flush=True)
+    w = wl.r3_compactified(N=20000, seed=17)
+    (d, keep), kw = _desc(steps, w, 0.5)
+    F1 = steps.force_eval(d, w.x, w.M, w.soft)
+    sim = steps.Sim(d, w.N)
+    sim.set_state(w.x, w.M, w.soft, w.v)
+    for _ in range(2):  # the scratch keeps the previous tree: a second evaluation is the same
+        sim.forces()
+        assert np.array_equal(sim.get_forces(), F1)
+    assert sim.last_kernel_ms()[0] > 0
+    with pytest.raises(_abi.StepsError) as ei:
+        sim.set_precision(steps.PREC_MIXED)
+    assert ei.value.code == _abi.STEPS_ERR_ARG
+    sim.close()
+
+
+def test_bh_refusals(steps):
+    print("case bh refusals", flush=True)
+    w = wl.r3_compactified(N=100, seed=18)
+    bad = [steps.make_desc(steps.TOPO_R3, bh_theta=-0.5),
+           steps.make_desc(steps.TOPO_R3, bh_theta=0.5, bh_particle_radius=-1.0),
+           steps.make_desc(steps.TOPO_R3, bh_theta=0.5, precision=steps.PREC_MIXED),
+           steps.make_desc(steps.TOPO_T3, is_periodic=1, L=10.0, bh_theta=0.5)]
+    for d, keep in bad:
+        with pytest.raises(_abi.StepsError) as ei:
+            steps.force_eval(d, w.x, w.M, w.soft)
+        assert ei.value.code == _abi.STEPS_ERR_ARG
+    srv = steps.Server(streams=1)
+    d, keep = steps.make_desc(steps.TOPO_R3, bh_theta=0.5, bh_particle_radius=w.particle_radius)
+    with pytest.raises(_abi.StepsError) as ei:
+        srv.add_sim(d, w.x, w.M, w.soft)
+    assert ei.value.code == _abi.STEPS_ERR_ARG
+    srv.close()

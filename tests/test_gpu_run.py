@@ -20,7 +20,7 @@ UNIT_T, UNIT_V = 47.14829951063323, 20.738652969925447
 def test_steps_run_r3_comoving_matches_oracle(steps, oracle, tmp_path):
     print("case steps_run R3 comoving N=2048 from an ASCII IC", flush=True)
     w = wl.r3_compactified(N=2048, seed=21)
-    a0, a_max, acc = 1.0 / 32, 0.034, 0.0125
+    a0, a_max, acc = 1.0 / 32, 0.033, 0.0125
     h_min, h_max = 2e-5 / UNIT_T, 0.03125 / UNIT_T
     ic = tmp_path / "ic.dat"
     # the IC convention: velocities in km/s * sqrt(a) (main.cc:1147-1154)
@@ -36,7 +36,7 @@ def test_steps_run_r3_comoving_matches_oracle(steps, oracle, tmp_path):
         f"HubbleConstant\t{wl.H0 * UNIT_V!r}", f"a_start\t{a0!r}", f"a_max\t{a_max!r}",
         "COSMOLOGY\t1", "IS_PERIODIC\t0", "COMOVING_INTEGRATION\t1", "L_BOX\t100.0", "R_SIM\t1000.0",
         f"IC_FILE\t{ic}", "IC_FORMAT\t0", f"OUT_DIR\t{out}", "OUTPUT_TIME_VARIABLE\t0", "OUTPUT_FORMAT\t0",
-        f"ACC_PARAM\t{acc!r}", "STEP_MIN\t0.00002", "STEP_MAX\t0.03125", f"PARTICLE_RADII\t{w.R0!r}",
+        f"ACC_PARAM\t{acc!r}", "STEP_MIN\t0.00002", "STEP_MAX\t0.03125", f"PARTICLE_RADII\t{w.particle_radius!r}",
         "FIRST_T_OUT\t13.0", "H_OUT\t13.0", ""]))
     exe = ROOT / "steps_amd" / "lib" / "steps_run"
     r = subprocess.run([str(exe), str(par), "--max-steps", "40"], capture_output=True, text=True, timeout=120)
@@ -51,7 +51,7 @@ def test_steps_run_r3_comoving_matches_oracle(steps, oracle, tmp_path):
     p = oracle.params(0, **kw)
     cosmo = oracle.OracleCosmo(wl.H0, wl.OMEGA_M, wl.OMEGA_L, 0.0, 1.0 - wl.OMEGA_M - wl.OMEGA_L)
     lib = oracle.lib()
-    soft, M_min, rho_part = steps.softening_lengths(w.M, w.R0)   # utils.cc:59-82, as the driver does
+    soft, M_min, rho_part = steps.softening_lengths(w.M, w.particle_radius)   # utils.cc:59-82, as the driver does
     x = w.x.reshape(-1).copy()
     v = (w.v.reshape(-1) * math.sqrt(a0) * UNIT_V) / math.sqrt(a0) / UNIT_V
     F = oracle.forces(p, w.x, w.M, soft).reshape(-1).copy()

@@ -35,6 +35,8 @@ def cases(steps, oracle):
     for prec in (steps.PREC_FP64, steps.PREC_MIXED):
         d, keep = steps.make_desc(steps.TOPO_R3, precision=prec, **kw)
         out.append((f"r3 prec={prec}", d, keep, r3a, r3b))
+    d, keep = steps.make_desc(steps.TOPO_R3, bh_theta=0.5, bh_particle_radius=r3a.particle_radius, **kw)
+    out.append(("r3 barnes-hut", d, keep, r3a, r3b))
     ta, tb = wl.t3_lattice(n_side=16, seed=1001), wl.t3_lattice(n_side=10, seed=1002)
     T3 = steps.t3_table(2, ta.L)
     d, keep = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=ta.L, t3_table=T3)
