@@ -9,9 +9,9 @@ are already resident in HBM.  The default workload is the compactified R^3
 configuration at N = 2^21 (seed 1003): the FP64 path is the headline value;
 the mixed-precision path, its accuracy against FP64, and the CPU baseline
 (the reference-faithful OpenMP restatement on this host's cores, on a bounded
-sample of rows) are reported on the same JSON line, beside the periodic
-variants, the N = 16384 force-plus-KDK-steps case, the serving loop and the
-host-buffer entry point.
+sample of rows) are reported on the same JSON line, beside the Barnes-Hut
+octree on the same particles, the periodic variants, the N = 16384
+force-plus-KDK-steps case, the serving loop and the host-buffer entry point.
 """
 from __future__ import annotations
 
@@ -375,6 +375,49 @@ def c0_kdk_leg(nsteps: int = 10) -> dict:
     return out
 
 
+def bh_leg(w, kw, F64, direct_ms: float, reps: int) -> dict:
+    """Barnes-Hut octree forces (the reference's USE_BH engine, forces.cc:
+    378-506, which the reference runs on the CPU only) on the headline
+    workload: octree build plus traversal per evaluation (HIP events), the
+    error against the device's FP64 direct sum, and the oracle's
+    single-thread restatement of the reference's pointer octree on the same
+    particles (a full tree build, then a row sample; the traversal time is
+    scaled to all rows)."""
+    out = {"N": w.N}
+    N = w.N
+    for theta in (0.5, 0.8):
+        d, keep = steps_amd.make_desc(steps_amd.TOPO_R3, bh_theta=theta, bh_particle_radius=w.particle_radius, **kw)
+        sim = steps_amd.Sim(d, N)
+        sim.set_state(w.x, w.M, w.soft, w.v)
+        _, ms, _ = time_path(sim, reps, 1)
+        F = sim.get_forces()
+        sim.close()
+        err = np.linalg.norm(F - F64, axis=1) / np.linalg.norm(F64, axis=1)
+        out[f"theta_{theta}"] = {"eval_ms": ms, "evaluations_per_s": 1e3 / ms,
+                                 "speedup_vs_direct_fp64": direct_ms / ms,
+                                 "rel_err_vs_direct_p50": float(np.median(err)),
+                                 "rel_err_vs_direct_p99": float(np.percentile(err, 99))}
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib  # the oracle: timed here as the CPU baseline only
+
+    p = oracle_lib.params(0, **kw)
+    t0 = time.perf_counter()
+    oracle_lib.forces_bh(p, 0.5, w.particle_radius, w.x, w.M, w.soft, 0, 0)
+    t_build = time.perf_counter() - t0
+    rows = 65536
+    t0 = time.perf_counter()
+    oracle_lib.forces_bh(p, 0.5, w.particle_radius, w.x, w.M, w.soft, 0, rows - 1)
+    t_rows = time.perf_counter() - t0
+    t_eval = t_build + max(t_rows - t_build, 0.0) * N / rows
+    out["cpu_baseline"] = {"value": 1.0 / t_eval, "unit": "evaluations/s", "cores": 1, "kind": "port",
+                           "seconds_per_evaluation": t_eval,
+                           "sample": f"theta 0.5: tree build over all {N} particles ({t_build:.2f} s), "
+                                     f"rows 0..{rows - 1} traversed ({t_rows - t_build:.2f} s), scaled to {N} rows"}
+    out["theta_0.5"]["speedup_vs_cpu_port"] = t_eval * 1e3 / out["theta_0.5"]["eval_ms"]
+    print(f"[bench] barnes-hut: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def host_path_leg(w, desc, calls: int) -> dict:
     """The host-buffer entry point (steps_force_eval, the reference routine's
     boundary): host positions, masses and softening lengths in, forces out,
@@ -399,6 +442,7 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the T3 / S1xR2 variant leg")
+    ap.add_argument("--no-bh", action="store_true", help="skip the Barnes-Hut octree leg")
     ap.add_argument("--no-host-path", action="store_true",
                     help="skip the host-buffer entry-point leg (profiling runs: its N = 16384 launches share kernel names)")
     ap.add_argument("--serving-requests", type=int, default=160,
@@ -501,6 +545,9 @@ def main() -> int:
             "roofline": {"bound": "valu", "achieved": ach_m, "peak": PEAK_FP32_VECTOR_TFLOPS,
                          "unit": "TFLOP/s", "frac": ach_m / PEAK_FP32_VECTOR_TFLOPS},
         }
+
+    if not args.no_bh:
+        out["barnes_hut"] = bh_leg(w, kw, F64, pair_ms, max(1, min(args.steps, 3)))
 
     if not args.no_variants:
         out["variants"] = variants_leg(max(1, min(args.steps, 3)))
