@@ -408,11 +408,20 @@ def bh_leg(w, kw, F64, direct_ms: float, reps: int) -> dict:
     t0 = time.perf_counter()
     oracle_lib.forces_bh(p, 0.5, w.particle_radius, w.x, w.M, w.soft, 0, rows - 1)
     t_rows = time.perf_counter() - t0
+    visited, accepted = oracle_lib.bh_last_counts()
     t_eval = t_build + max(t_rows - t_build, 0.0) * N / rows
-    out["cpu_baseline"] = {"value": 1.0 / t_eval, "unit": "evaluations/s", "cores": 1, "kind": "port",
+    # node visits and interactions per row at theta 0.5 (the same for every
+    # engine that makes the reference's decisions), from the sampled rows
+    g = out["theta_0.5"]
+    g["nodes_examined_per_row"] = visited / rows
+    g["interactions_per_row"] = accepted / rows
+    g["g_interactions_per_s"] = accepted / rows * N / (g["eval_ms"] * 1e-3) / 1e9
+    threads = int(oracle_lib.lib().oracle_max_threads())
+    out["cpu_baseline"] = {"value": 1.0 / t_eval, "unit": "evaluations/s", "cores": threads, "kind": "port",
                            "seconds_per_evaluation": t_eval,
-                           "sample": f"theta 0.5: tree build over all {N} particles ({t_build:.2f} s), "
-                                     f"rows 0..{rows - 1} traversed ({t_rows - t_build:.2f} s), scaled to {N} rows"}
+                           "sample": f"theta 0.5: pointer-octree build over all {N} particles, one thread "
+                                     f"({t_build:.2f} s), rows 0..{rows - 1} traversed on {threads} OpenMP threads "
+                                     f"({t_rows - t_build:.2f} s), scaled to {N} rows"}
     out["theta_0.5"]["speedup_vs_cpu_port"] = t_eval * 1e3 / out["theta_0.5"]["eval_ms"]
     print(f"[bench] barnes-hut: {out}", file=sys.stderr, flush=True)
     return out

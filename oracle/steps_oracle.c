@@ -1134,20 +1134,30 @@ static int orc_bh_insert(orc_bh_tree *t, int node, const double *x, const double
 
 /* forces.cc:374-398 */
 static void orc_bh_force(const orc_bh_tree *t, int node, const double *x, int i, const double *h, double theta,
-                         double pr, double M_min, double *f)
+                         double pr, double M_min, double *f, long long *cnt)
 {
     if (node < 0) return;
     const orc_bh_node *q = &t->n[node];
     if (q->mass == 0 || q->pidx == i) return;
     double dx = q->com[0] - x[3 * i], dy = q->com[1] - x[3 * i + 1], dz = q->com[2] - x[3 * i + 2];
     double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    cnt[0]++;
     if (q->pidx != -1 || q->size / dist < theta) {
         double beta = cbrt(q->mass / M_min) * pr + h[i];
         double w = q->mass * oracle_force_softening(dist, beta);
         f[0] += w * dx; f[1] += w * dy; f[2] += w * dz;
+        cnt[1]++;
     } else {
-        for (int j = 0; j < 8; ++j) orc_bh_force(t, q->child[j], x, i, h, theta, pr, M_min, f);
+        for (int j = 0; j < 8; ++j) orc_bh_force(t, q->child[j], x, i, h, theta, pr, M_min, f, cnt);
     }
+}
+
+static long long orc_bh_counts[2];
+
+void oracle_bh_last_counts(long long *visited, long long *accepted)
+{
+    *visited = orc_bh_counts[0];
+    *accepted = orc_bh_counts[1];
 }
 
 int oracle_forces_bh(const oracle_params *p, double theta, double particle_radius, int N, const double *x,
@@ -1169,17 +1179,23 @@ int oracle_forces_bh(const oracle_params *p, double theta, double particle_radiu
         return rc;
     }
     const double DE = p->H0 * p->H0 * p->Omega_lambda;
+    long long visited = 0, accepted = 0;
     int kk;
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : visited, accepted)
     for (kk = id_min; kk <= id_max; ++kk) {
         double f[3] = {0, 0, 0};
-        orc_bh_force(&t, 0, x, kk, h, theta, particle_radius, M_min, f);
+        long long cnt[2] = {0, 0};
+        orc_bh_force(&t, 0, x, kk, h, theta, particle_radius, M_min, f, cnt);
+        visited += cnt[0];
+        accepted += cnt[1];
         if (p->cosmology == 1) { /* forces.cc:474-491 */
             double c = p->comoving == 1 ? p->mass_in_unit_sphere : DE;
             for (int a = 0; a < 3; ++a) f[a] += c * x[3 * kk + a];
         }
         for (int a = 0; a < 3; ++a) F[3 * (kk - id_min) + a] = f[a];
     }
+    orc_bh_counts[0] = visited;
+    orc_bh_counts[1] = accepted;
     free(t.n);
     return 0;
 }

@@ -179,6 +179,9 @@ double oracle_initial_errmax(const oracle_params *p, oracle_step_state *s, int N
  * close to separate within 64 levels. */
 int oracle_forces_bh(const oracle_params *p, double theta, double particle_radius, int N, const double *x,
                      const double *M, const double *h, double *F, int id_min, int id_max);
+/* Nodes the last oracle_forces_bh call examined (massless and own-leaf nodes
+ * excluded) and took, summed over its rows. */
+void oracle_bh_last_counts(long long *visited, long long *accepted);
 
 #ifdef __cplusplus
 }

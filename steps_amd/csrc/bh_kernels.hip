@@ -22,11 +22,11 @@
 // The traversal makes the reference's decision per (row, node): a leaf, or a
 // node with side / dist < theta, is taken at its centre of mass with
 // softening length cbrt(mass / M_min) * ParticleRadi + h_i; any other node
-// is opened; the row's own leaf and massless nodes are skipped.  Each lane
-// follows its own row's decisions (stackless, by the skip links), so the set
-// of interactions is the reference's and only the rounding of the centres of
-// mass and of the sums differs.  Rows run in key order so that the lanes of a
-// wave walk similar paths.
+// is opened; the row's own leaf and massless nodes are skipped.  The lanes of
+// a wave (rows in key order, so neighbours in space) walk the pre-order list
+// together, stackless by the skip links, each lane with its own decisions
+// (bh_force_kernel), so the set of interactions is the reference's and only
+// the rounding of the centres of mass and of the sums differs.
 //
 // Particles closer than 2^-21 of the root side share one level-21 node whose
 // children are their leaves (the reference splits further, which changes the
@@ -246,25 +246,28 @@ __global__ __launch_bounds__(256) void bh_scan_add_kernel(uint32_t *__restrict__
     if (p < n) base[p] += bsum[p / kScanItems];
 }
 
-// The items: centre of mass (x, y, z) and mass; side and the mass part of
-// the softening length, cbrt(mass / M_min) * ParticleRadi; skip link and the
-// particle (input index) of a leaf or -1; level of an internal node (-1 for
-// a leaf).
-struct BhNodes {
-    double4 *cm;
-    double2 *aux;
-    int2 *link;
-    int8_t *level;
+// One item of the pre-order list, 64 bytes (one scalar load of 16 dwords):
+// centre of mass and mass; the cube's side (0 for a leaf); the mass part of
+// the softening length, cbrt(mass / M_min) * ParticleRadi; the skip link
+// (the item after the subtree) and the particle (input index) of a leaf, or
+// -1 for an internal node.
+struct __align__(64) BhNode {
+    double x, y, z, m;
+    double side, msoft;
+    int next, pidx;
+    int pad[2];
 };
+static_assert(sizeof(BhNode) == 64, "one 64-byte record per item");
 
 // Every item of the pre-order list: the internal nodes starting at p, then
-// p's leaf.
+// p's leaf.  level[] holds an internal node's level, -1 for a leaf.
 __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__ tab, int rows,
                                                       const uint32_t *__restrict__ base,
                                                       const double4 *__restrict__ srec,
                                                       const uint32_t *__restrict__ sidx, int64_t n,
                                                       const double *__restrict__ red, double pr,
-                                                      const int *__restrict__ info, BhNodes nd) {
+                                                      const int *__restrict__ info, BhNode *__restrict__ nodes,
+                                                      int8_t *__restrict__ level) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int total = info[0];
@@ -279,34 +282,49 @@ __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__
             if (tab[(size_t)k * n + j] >= l) j = min(j + ((int64_t)1 << k), n - 1);
         double side = root;
         for (int q = 0; q < l; ++q) side = side / 2;  // forces.cc:155, level by level
-        nd.link[item] = make_int2((j + 1 < n) ? (int)base[j + 1] : total, -1);
-        nd.aux[item] = make_double2(side, 0.0);
-        nd.level[item] = (int8_t)l;
+        BhNode &o = nodes[item];
+        o.side = side;
+        o.next = (j + 1 < n) ? (int)base[j + 1] : total;
+        o.pidx = -1;
+        level[item] = (int8_t)l;
     }
     const double4 r = srec[p];
-    nd.link[item] = make_int2((p + 1 < n) ? (int)base[p + 1] : total, (int)sidx[p]);
-    nd.cm[item] = r;
-    nd.aux[item] = make_double2(0.0, cbrt(r.w / mmin) * pr);
-    nd.level[item] = -1;
+    BhNode o;
+    o.x = r.x;
+    o.y = r.y;
+    o.z = r.z;
+    o.m = r.w;
+    o.side = 0.0;
+    o.msoft = cbrt(r.w / mmin) * pr;
+    o.next = (p + 1 < n) ? (int)base[p + 1] : total;
+    o.pidx = (int)sidx[p];
+    o.pad[0] = o.pad[1] = 0;
+    nodes[item] = o;
+    level[item] = -1;
 }
 
 // The internal nodes of one level: mass and centre of mass from their direct
 // children (deeper levels are done), and the mass part of the softening.
-__global__ __launch_bounds__(256) void bh_level_kernel(int level, const int *__restrict__ info,
-                                                       const double *__restrict__ red, double pr, BhNodes nd) {
+__global__ __launch_bounds__(256) void bh_level_kernel(int lev, const int *__restrict__ info,
+                                                       const double *__restrict__ red, double pr,
+                                                       BhNode *__restrict__ nodes, const int8_t *__restrict__ level) {
     const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (it >= info[0] || nd.level[it] != level) return;
-    const int end = nd.link[it].x;
+    if (it >= info[0] || level[it] != lev) return;
+    const int end = nodes[it].next;
     double sx = 0, sy = 0, sz = 0, sm = 0;
-    for (int c = (int)it + 1; c < end; c = nd.link[c].x) {
-        const double4 q = nd.cm[c];
-        sx += q.w * q.x;
-        sy += q.w * q.y;
-        sz += q.w * q.z;
-        sm += q.w;
+    for (int c = (int)it + 1; c < end; c = nodes[c].next) {
+        const BhNode &q = nodes[c];
+        sx += q.m * q.x;
+        sy += q.m * q.y;
+        sz += q.m * q.z;
+        sm += q.m;
     }
-    nd.cm[it] = make_double4(sx / sm, sy / sm, sz / sm, sm);
-    nd.aux[it].y = cbrt(sm / red[1]) * pr;
+    BhNode &o = nodes[it];
+    o.x = sx / sm;
+    o.y = sy / sm;
+    o.z = sz / sm;
+    o.m = sm;
+    o.msoft = cbrt(sm / red[1]) * pr;
 }
 
 // forces.cc:52-87 (force_softening): r^-3 outside beta, the cubic spline inside
@@ -319,43 +337,59 @@ __device__ __forceinline__ double bh_soft(double r, double beta) {
     return 32.0 / b6 * r3 + -38.4 / b5 * r2 + 32.0 / (3.0 * b3);
 }
 
-// One row per lane (rows in key order): the reference's traversal
-// (forces.cc:378-399) by skip links, then the external term (forces.cc:474-491).
+// One row per lane, rows in key order: the reference's traversal
+// (forces.cc:378-399), then the external term (forces.cc:474-491).
+//
+// The wave walks the pre-order list together: the item `it` is the same for
+// every lane (a scalar load of its 64-byte record), and each lane makes its
+// own decision there -- take it (a leaf, or side / dist < theta), skip it
+// (the row's own leaf, a massless node), or open it.  A lane that takes or
+// skips an item waits until the wave reaches that item's skip link; the wave
+// moves to the next item in pre-order when any lane opened the current one,
+// else to its skip link (every waiting lane waits for an item at or beyond
+// it, since it took an ancestor).  So each lane sees exactly the items of its
+// own traversal, in the same order: the interactions and their summation
+// order are those of a lane walking alone.
 __global__ __launch_bounds__(256) void bh_force_kernel(const double4 *__restrict__ srec,
                                                        const double *__restrict__ ssoft,
                                                        const uint32_t *__restrict__ sidx, int64_t n,
-                                                       const int *__restrict__ info, BhNodes nd, double theta,
-                                                       int ext_mode, double ext_coeff, int64_t id_min,
-                                                       int64_t id_max, double *__restrict__ F) {
+                                                       const int *__restrict__ info,
+                                                       const BhNode *__restrict__ nodes, double theta, int ext_mode,
+                                                       double ext_coeff, int64_t id_min, int64_t id_max,
+                                                       double *__restrict__ F) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int i = (int)sidx[s];
-    if (i < id_min || i > id_max) return;
-    const double4 me = srec[s];
-    const double hi = ssoft[s];
+    const int64_t sc = min(s, n - 1);
+    const int i = (int)sidx[sc];
+    const bool mine = s < n && i >= id_min && i <= id_max;
+    const double4 me = srec[sc];
+    const double hi = ssoft[sc];
     double fx = 0, fy = 0, fz = 0;
     const int total = info[0];
+    int wait = mine ? 0 : INT32_MAX;  // the next item this lane takes part in
     int it = 0;
     while (it < total) {
-        const int2 ln = nd.link[it];
-        const double4 c = nd.cm[it];
-        if (ln.y == i || c.w == 0.0) {  // the row's own leaf; a massless node
-            it = ln.x;
-            continue;
+        const BhNode q = nodes[it];
+        bool open = false;
+        if (wait <= it) {
+            if (q.pidx == i || q.m == 0.0) {  // the row's own leaf; a massless node
+                wait = q.next;
+            } else {
+                const double dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+                const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+                if (q.pidx >= 0 || q.side / dist < theta) {
+                    const double w = q.m * bh_soft(dist, q.msoft + hi);
+                    fx += w * dx;
+                    fy += w * dy;
+                    fz += w * dz;
+                    wait = q.next;
+                } else {
+                    open = true;  // its first child follows in pre-order
+                }
+            }
         }
-        const double dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
-        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-        const double2 a = nd.aux[it];
-        if (ln.y >= 0 || a.x / dist < theta) {
-            const double w = c.w * bh_soft(dist, a.y + hi);
-            fx += w * dx;
-            fy += w * dy;
-            fz += w * dz;
-            it = ln.x;
-        } else {
-            ++it;  // opened: its first child follows in pre-order
-        }
+        it = __ballot(open) ? it + 1 : q.next;
     }
+    if (!mine) return;
     if (ext_mode == 1) {
         fx += ext_coeff * me.x;
         fy += ext_coeff * me.y;
@@ -376,7 +410,8 @@ struct BhView {
     uint32_t *sidx;
     int8_t *tab;
     int *info;
-    BhNodes nd;
+    BhNode *nodes;
+    int8_t *level;
     void *sort;
 };
 
@@ -394,16 +429,14 @@ template <class Take> void bh_pieces(int64_t n, Take &&take) {
     take(nn * 4);                             // sidx
     take(nn * (size_t)table_rows(n));         // tab
     take(16);                                 // info
-    take(cap * 32);                           // cm
-    take(cap * 16);                           // aux
-    take(cap * 8);                            // link
+    take(cap * sizeof(BhNode));               // nodes
     take(cap);                                // level
     take(sort_u64_scratch_bytes(n));          // sort
 }
 
 BhView bh_view(void *scratch, int64_t n) {
     char *base = (char *)scratch;
-    char *ptr[16];
+    char *ptr[14];
     int k = 0;
     size_t off = 0;
     bh_pieces(n, [&](size_t b) {
@@ -422,11 +455,9 @@ BhView bh_view(void *scratch, int64_t n) {
     v.sidx = (uint32_t *)ptr[8];
     v.tab = (int8_t *)ptr[9];
     v.info = (int *)ptr[10];
-    v.nd.cm = (double4 *)ptr[11];
-    v.nd.aux = (double2 *)ptr[12];
-    v.nd.link = (int2 *)ptr[13];
-    v.nd.level = (int8_t *)ptr[14];
-    v.sort = ptr[15];
+    v.nodes = (BhNode *)ptr[11];
+    v.level = (int8_t *)ptr[12];
+    v.sort = ptr[13];
     return v;
 }
 
@@ -469,14 +500,15 @@ hipError_t launch_bh_forces(const EvalPlan &p, const DeviceParticles &dp, void *
     hipLaunchKernelGGL(bh_scan_top_kernel, dim3(1), dim3(1024), 0, st, v.bsum, sb, v.info);
     hipLaunchKernelGGL(bh_scan_add_kernel, dim3(ng), dim3(256), 0, st, v.base, n, v.bsum);
     hipLaunchKernelGGL(bh_fill_kernel, dim3(ng), dim3(256), 0, st, v.tab, rows, v.base, v.srec, v.sidx, n, red,
-                       p.bh_particle_radius, v.info, v.nd);
+                       p.bh_particle_radius, v.info, v.nodes, v.level);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const unsigned gi = (unsigned)((bh_item_capacity(n) + 255) / 256);
     for (int l = kLevels; l >= 0; --l)
-        hipLaunchKernelGGL(bh_level_kernel, dim3(gi), dim3(256), 0, st, l, v.info, red, p.bh_particle_radius, v.nd);
+        hipLaunchKernelGGL(bh_level_kernel, dim3(gi), dim3(256), 0, st, l, v.info, red, p.bh_particle_radius,
+                           v.nodes, v.level);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_tree_end && (e = hipEventRecord(ev_tree_end, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(bh_force_kernel, dim3(ng), dim3(256), 0, st, v.srec, v.ssoft, v.sidx, n, v.info, v.nd,
+    hipLaunchKernelGGL(bh_force_kernel, dim3(ng), dim3(256), 0, st, v.srec, v.ssoft, v.sidx, n, v.info, v.nodes,
                        p.bh_theta, p.ext_mode, p.ext_coeff, p.id_min, p.id_min + p.nrows - 1, F);
     return hipGetLastError();
 }

@@ -78,6 +78,8 @@ def lib() -> C.CDLL:
         getattr(L, name).restype = C.c_int
         getattr(L, name).argtypes = [C.POINTER(OracleParams), C.c_int, _DP, _DP, _DP, _DP, _IP, C.c_int]
     L.oracle_max_threads.restype = C.c_int
+    L.oracle_bh_last_counts.restype = None
+    L.oracle_bh_last_counts.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.oracle_forces_bh.restype = C.c_int
     L.oracle_forces_bh.argtypes = [C.POINTER(OracleParams), C.c_double, C.c_double, C.c_int, _DP, _DP, _DP, _DP,
                                    C.c_int, C.c_int]
@@ -204,6 +206,13 @@ def forces_bh(p: OracleParams, theta: float, particle_radius: float, x, M, h, id
     rc = lib().oracle_forces_bh(C.byref(p), theta, particle_radius, N, dp(x), dp(M), dp(h), dp(F), id_min, id_max)
     assert rc == 0, rc
     return F[: 3 * n].reshape(n, 3)
+
+
+def bh_last_counts() -> tuple[int, int]:
+    """(nodes examined, nodes taken) summed over the rows of the last forces_bh call."""
+    v, a = C.c_longlong(), C.c_longlong()
+    lib().oracle_bh_last_counts(C.byref(v), C.byref(a))
+    return v.value, a.value
 
 
 def forces_rows(p: OracleParams, x, M, h, rows, kind: str = "oracle") -> np.ndarray:
