@@ -70,7 +70,9 @@ typedef struct steps_force_desc {
     int32_t s1r2_lookup;         /* 1: S1R2 Ewald table (default build);
                                     0: direct z-image sum (-DPERIODIC_Z_NOLOOKUP)          */
     int32_t interp_order;        /* S1R2 EWALD_INTERPOLATION_ORDER: 0 NGP, 2 CIC, 4 TSC.
-                                    T3 is always tricubic Catmull-Rom (main.cc:501)       */
+                                    T3 direct sum: always tricubic Catmull-Rom (main.cc:501);
+                                    T3 Barnes-Hut: the CPU build's order (ewald_space.cc:452-
+                                    523): 2 trilinear, 4 Catmull-Rom, 6 / 8 Lagrange     */
     const double *t3_table;      /* T3_EWALD_FORCE_TABLE [ng^3*3], host memory            */
     int32_t t3_ngrid;            /* N_EWALD_FORCE_GRID                                    */
     const double *s1r2_table;    /* S1R2_EWALD_FORCE_TABLE [nrho*nz*2], host memory       */
@@ -80,7 +82,8 @@ typedef struct steps_force_desc {
     int32_t precision;           /* enum steps_precision                                  */
     int32_t device;              /* HIP device ordinal                                    */
     double bh_theta;             /* > 0: Barnes-Hut octree forces with this opening angle
-                                    (USE_BH, main.cc:303; R3 and FP64 only); 0: direct sum */
+                                    (USE_BH, main.cc:303; every topology and IS_PERIODIC
+                                    mode of forces.cc:373-1217; FP64 only); 0: direct sum */
     double bh_particle_radius;   /* ParticleRadi: a node's softening length is
                                     cbrt(mass / M_min) * ParticleRadi + h_i (forces.cc:386) */
 } steps_force_desc;

@@ -1132,23 +1132,58 @@ static int orc_bh_insert(orc_bh_tree *t, int node, const double *x, const double
     return 0;
 }
 
-/* forces.cc:374-398 */
-static void orc_bh_force(const orc_bh_tree *t, int node, const double *x, int i, const double *h, double theta,
-                         double pr, double M_min, double *f, long long *cnt)
+/* The pair term of one traversal: the topology's image choice and table
+ * correction (forces.cc:373-398 R3; :584-652 T3 Ewald and nearest image;
+ * :903-981 S1R2 image sum, Ewald table and nearest z-image). */
+typedef struct orc_bh_ctx {
+    const oracle_params *p;
+    const orc_bh_tree *t;
+    const double *x, *h;
+    double theta, pr, M_min;
+    int mode;            /* 0 R3, 1 T3 nearest, 2 T3 Ewald, 3 S1R2 nearest, 4 S1R2 table, 5 S1R2 images */
+    double zi_image;     /* mode 5: the row's z plus m L */
+    double cut;          /* mode 5: ewald_cut * L */
+    long long cnt[2];
+} orc_bh_ctx;
+
+static void orc_bh_force(orc_bh_ctx *c, int node, int i, double *f)
 {
     if (node < 0) return;
-    const orc_bh_node *q = &t->n[node];
+    const orc_bh_node *q = &c->t->n[node];
     if (q->mass == 0 || q->pidx == i) return;
-    double dx = q->com[0] - x[3 * i], dy = q->com[1] - x[3 * i + 1], dz = q->com[2] - x[3 * i + 2];
+    const double *x = c->x;
+    const double L = c->p->L;
+    double dx = q->com[0] - x[3 * i], dy = q->com[1] - x[3 * i + 1];
+    double dz = q->com[2] - (c->mode == 5 ? c->zi_image : x[3 * i + 2]);
+    if (c->mode == 1 || c->mode == 2) { /* forces.cc:594-599 */
+        if (fabs(dx) > 0.5 * L) dx = dx - L * dx / fabs(dx);
+        if (fabs(dy) > 0.5 * L) dy = dy - L * dy / fabs(dy);
+    }
+    if (c->mode >= 1 && c->mode <= 4) /* forces.cc:600-601, 941-942 */
+        if (fabs(dz) > 0.5 * L) dz = dz - L * dz / fabs(dz);
     double dist = sqrt(dx * dx + dy * dy + dz * dz);
-    cnt[0]++;
-    if (q->pidx != -1 || q->size / dist < theta) {
-        double beta = cbrt(q->mass / M_min) * pr + h[i];
-        double w = q->mass * oracle_force_softening(dist, beta);
-        f[0] += w * dx; f[1] += w * dy; f[2] += w * dz;
-        cnt[1]++;
+    c->cnt[0]++;
+    if (q->pidx != -1 || q->size / dist < c->theta) {
+        if (c->mode == 5 && fabs(dz) > c->cut) return; /* forces.cc:915 */
+        double beta = cbrt(q->mass / c->M_min) * c->pr + c->h[i];
+        double w = oracle_force_softening(dist, beta);
+        double D[3] = {0, 0, 0};
+        if (c->mode == 2)
+            oracle_t3_interpolate(c->p->t3_ngrid, L, c->p->t3_table, dx, dy, dz, D, c->p->interp_order);
+        else if (c->mode == 4)
+            oracle_s1r2_interpolate(c->p->s1r2_table, c->p->s1r2_nrho, c->p->s1r2_nz, ORC_RADIAL_EXTENT * c->p->Rsim,
+                                    L, dx, dy, dz, c->p->interp_order, D);
+        if (c->mode == 2 || c->mode == 4) { /* forces.cc:607-610, 948-951 */
+            f[0] += q->mass * (w * dx - D[0]);
+            f[1] += q->mass * (w * dy - D[1]);
+            f[2] += q->mass * (w * dz - D[2]);
+        } else {
+            w *= q->mass;
+            f[0] += w * dx; f[1] += w * dy; f[2] += w * dz;
+        }
+        c->cnt[1]++;
     } else {
-        for (int j = 0; j < 8; ++j) orc_bh_force(t, q->child[j], x, i, h, theta, pr, M_min, f, cnt);
+        for (int j = 0; j < 8; ++j) orc_bh_force(c, q->child[j], i, f);
     }
 }
 
@@ -1164,33 +1199,75 @@ int oracle_forces_bh(const oracle_params *p, double theta, double particle_radiu
                      const double *M, const double *h, double *F, int id_min, int id_max)
 {
     if (!p || N <= 0 || id_min < 0 || id_max >= N || id_max < id_min - 1) return -1;
-    if (p->topology != ORC_R3) return -4;
+    int mode;
+    if (p->topology == ORC_R3) mode = 0;
+    else if (p->topology == ORC_T3) mode = p->is_periodic >= 2 ? 2 : 1;
+    else if (p->topology == ORC_S1R2) mode = p->is_periodic >= 2 ? (p->s1r2_lookup ? 4 : 5) : 3;
+    else return -4;
+    if (mode == 2 && !p->t3_table) return -3;
+    if (mode == 4 && !p->s1r2_table) return -3;
+    if ((mode == 3 || mode == 5) && p->cosmology == 1 && p->comoving == 1 && !p->radial_table) return -3;
     double rmax = 0, M_min = M[0];
     for (int i = 0; i < N; ++i) {
-        double r = sqrt(pow(x[3 * i], 2) + pow(x[3 * i + 1], 2) + pow(x[3 * i + 2], 2));
+        double r = (p->topology == ORC_S1R2) ? sqrt(pow(x[3 * i], 2) + pow(x[3 * i + 1], 2)) /* forces.cc:1018 */
+                                             : sqrt(pow(x[3 * i], 2) + pow(x[3 * i + 1], 2) + pow(x[3 * i + 2], 2));
         if (r > rmax) rmax = r;
-        if (M_min > M[i]) M_min = M[i];
+        if (M_min > M[i]) M_min = M[i]; /* utils.cc:62-67 */
+    }
+    /* the root cube: forces.cc:437-438 (R3), :690 (T3), :1047-1058 (S1R2) */
+    double c0[3] = {0, 0, 0}, side;
+    if (p->topology == ORC_R3) {
+        side = 2.00002 * rmax;
+    } else if (p->topology == ORC_T3) {
+        side = p->L;
+        c0[0] = c0[1] = c0[2] = 0.50 * p->L;
+    } else {
+        side = (2 * rmax < p->L) ? p->L : 2.0 * rmax;
+        c0[2] = 0.50 * side;
     }
     orc_bh_tree t = {NULL, 0, 0};
-    int rc = orc_bh_new(&t, 0.0, 0.0, 0.0, 2.00002 * rmax) < 0 ? -1 : 0; /* forces.cc:437-438 */
+    int rc = orc_bh_new(&t, c0[0], c0[1], c0[2], side) < 0 ? -1 : 0;
     for (int i = 0; i < N && !rc; ++i) rc = orc_bh_insert(&t, 0, x, M, i, 0);
     if (rc) {
         free(t.n);
         return rc;
     }
     const double DE = p->H0 * p->H0 * p->Omega_lambda;
+    const int ewald_max = p->is_periodic + 1;          /* main.cc:1270 */
+    const double cut = ((double)ewald_max - 0.4) * p->L; /* main.cc:1271, forces.cc:1100 */
     long long visited = 0, accepted = 0;
     int kk;
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : visited, accepted)
     for (kk = id_min; kk <= id_max; ++kk) {
+        orc_bh_ctx c = {p, &t, x, h, theta, particle_radius, M_min, mode, 0.0, cut, {0, 0}};
         double f[3] = {0, 0, 0};
-        long long cnt[2] = {0, 0};
-        orc_bh_force(&t, 0, x, kk, h, theta, particle_radius, M_min, f, cnt);
-        visited += cnt[0];
-        accepted += cnt[1];
-        if (p->cosmology == 1) { /* forces.cc:474-491 */
-            double c = p->comoving == 1 ? p->mass_in_unit_sphere : DE;
-            for (int a = 0; a < 3; ++a) f[a] += c * x[3 * kk + a];
+        if (mode == 5) { /* forces.cc:1108-1118: one traversal per z image, summed in image order */
+            for (int m = -ewald_max; m < ewald_max + 1; m++) {
+                double g[3] = {0, 0, 0};
+                c.zi_image = x[3 * kk + 2] + ((double)m) * p->L;
+                orc_bh_force(&c, 0, kk, g);
+                for (int a = 0; a < 3; ++a) f[a] += g[a];
+            }
+        } else {
+            orc_bh_force(&c, 0, kk, f);
+        }
+        visited += c.cnt[0];
+        accepted += c.cnt[1];
+        if (p->topology == ORC_R3 && p->cosmology == 1) { /* forces.cc:474-491 */
+            double k = p->comoving == 1 ? p->mass_in_unit_sphere : DE;
+            for (int a = 0; a < 3; ++a) f[a] += k * x[3 * kk + a];
+        } else if (p->topology == ORC_S1R2 && p->cosmology == 1) { /* forces.cc:1127-1150, 1170-1190 */
+            double k = DE;
+            if (p->comoving == 1) {
+                k = p->mass_in_unit_sphere;
+                if (mode != 4) {
+                    double rxy = sqrt(pow(x[3 * kk], 2) + pow(x[3 * kk + 1], 2));
+                    k *= oracle_cylindrical_force_correction_order(rxy, p->Rsim, p->radial_table,
+                                                                   p->radial_table_size, 1);
+                }
+            }
+            f[0] += k * x[3 * kk];
+            f[1] += k * x[3 * kk + 1];
         }
         for (int a = 0; a < 3; ++a) F[3 * (kk - id_min) + a] = f[a];
     }

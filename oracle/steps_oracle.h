@@ -167,16 +167,24 @@ double oracle_initial_errmax(const oracle_params *p, oracle_step_state *s, int N
                              double *x, const double *v, const double *F,
                              const double *M);
 
-/* Barnes-Hut octree forces, free boundary (USE_BH build; forces.cc:97-187,
- * 321-506): the reference's pointer octree built by inserting particles
- * 0..N-1 into a root cube centred on the origin of side 2.00002 max|x_i|,
- * then per row a recursive traversal (children 0..7) that takes a leaf, or a
- * node with nodesize / dist < theta, at its centre of mass with softening
- * cbrt(mass / M_min) * particle_radius + h_i, and opens the others; the own
- * leaf and empty nodes are skipped.  Then the external term as for the
- * direct sum (the radial BH correction table is not restated).  Rows
- * id_min..id_max, F overwritten.  Returns 0, or -5 when two particles are too
- * close to separate within 64 levels. */
+/* Barnes-Hut octree forces (USE_BH build; forces.cc:97-187, 373-506 R3,
+ * 582-772 T3, 881-1217 S1R2): the reference's pointer octree built by
+ * inserting particles 0..N-1 into the topology's root cube (R3: side
+ * 2.00002 max|x_i| at the origin; T3: the box; S1R2: side L, or twice the
+ * largest cylindrical radius if that is not below L, on the axis at z =
+ * side/2), then per row a recursive traversal (children 0..7) that takes a
+ * leaf, or a node with nodesize / dist < theta, at its centre of mass with
+ * softening cbrt(mass / M_min) * particle_radius + h_i, and opens the others;
+ * the own leaf and empty nodes are skipped.  Pair terms: R3 plain; T3
+ * nearest image, plus for IS_PERIODIC >= 2 the table correction of the CPU
+ * build (oracle_t3_interpolate, order p->interp_order); S1R2 nearest
+ * z-image, plus the table correction for the lookup build; the S1R2 image
+ * sum (s1r2_lookup = 0) walks once per z image with the (ewald_max - 0.4) L
+ * cut.  External terms as forces.cc:474-491 (R3) and 1127-1190 (S1R2, x and
+ * y only, times the CPU cylindrical correction of order 1 unless the table
+ * is in use); the radial BH correction table is not restated.  Rows
+ * id_min..id_max, F overwritten.  Returns 0, -3 for a missing table, or -5
+ * when two particles are too close to separate within 64 levels. */
 int oracle_forces_bh(const oracle_params *p, double theta, double particle_radius, int N, const double *x,
                      const double *M, const double *h, double *F, int id_min, int id_max);
 /* Nodes the last oracle_forces_bh call examined (massless and own-leaf nodes

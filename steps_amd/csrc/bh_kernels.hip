@@ -56,8 +56,9 @@ int table_rows(int64_t n) {
 
 int64_t scan_blocks(int64_t n) { return std::max<int64_t>(1, (n + kScanItems - 1) / kScanItems); }
 
-// max |x_i| and min m_i, in two steps (per-block partials, then one block)
-__global__ __launch_bounds__(256) void bh_reduce_partial(const double4 *__restrict__ rec, int64_t n,
+// max |x_i| (R3) or max sqrt(x_i^2 + y_i^2) (S1R2), and min m_i, in two steps
+// (per-block partials, then one block that also places the root cube)
+__global__ __launch_bounds__(256) void bh_reduce_partial(const double4 *__restrict__ rec, int64_t n, bool xy,
                                                          double *__restrict__ part) {
     __shared__ double rr[256], mm[256];
     double r = 0.0, m = INFINITY;
@@ -65,8 +66,8 @@ __global__ __launch_bounds__(256) void bh_reduce_partial(const double4 *__restri
         const double4 p = rec[i];
         {
 #pragma clang fp contract(off)
-            // forces.cc:410-418: sqrt(x^2 + y^2 + z^2), no fused operations
-            r = fmax(r, sqrt(p.x * p.x + p.y * p.y + p.z * p.z));
+            // forces.cc:410-418 and 1016-1023: the radius without fused operations
+            r = fmax(r, sqrt(xy ? p.x * p.x + p.y * p.y : p.x * p.x + p.y * p.y + p.z * p.z));
         }
         m = fmin(m, p.w);
     }
@@ -87,8 +88,13 @@ __global__ __launch_bounds__(256) void bh_reduce_partial(const double4 *__restri
     }
 }
 
-__global__ __launch_bounds__(256) void bh_reduce_final(const double *__restrict__ part, int nb,
-                                                       double *__restrict__ out) {
+// out: root side, M_min (utils.cc:62-67), root centre (x, y, z).  The root
+// cube of each topology: R3 side 2.00002 max|x| at the origin (forces.cc:
+// 437-438); T3 the box, side L at L/2 (forces.cc:690); S1R2 side L, or twice
+// the largest cylindrical radius when that is not below L, centred on the
+// axis at z = side/2 (forces.cc:1047-1058).
+__global__ __launch_bounds__(256) void bh_reduce_final(const double *__restrict__ part, int nb, int topology,
+                                                       double L, double *__restrict__ out) {
     __shared__ double rr[256], mm[256];
     const int t = threadIdx.x;
     double r = 0.0, m = INFINITY;
@@ -107,8 +113,21 @@ __global__ __launch_bounds__(256) void bh_reduce_final(const double *__restrict_
         __syncthreads();
     }
     if (t == 0) {
-        out[0] = 2.00002 * rr[0];  // root side (forces.cc:437-438)
-        out[1] = mm[0];            // M_min (utils.cc:62-67)
+        double side, cx = 0.0, cy = 0.0, cz = 0.0;
+        if (topology == STEPS_TOPO_T3) {
+            side = L;
+            cx = cy = cz = 0.50 * L;
+        } else if (topology == STEPS_TOPO_S1R2) {
+            side = (2 * rr[0] < L) ? L : 2.0 * rr[0];
+            cz = 0.50 * side;
+        } else {
+            side = 2.00002 * rr[0];
+        }
+        out[0] = side;
+        out[1] = mm[0];
+        out[2] = cx;
+        out[3] = cy;
+        out[4] = cz;
     }
 }
 
@@ -122,7 +141,7 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(const double4 *__restrict_
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double4 p = rec[i];
-    double cx = 0.0, cy = 0.0, cz = 0.0, side = red[0];
+    double cx = red[2], cy = red[3], cz = red[4], side = red[0];
     uint64_t key = 0;
     for (int l = 0; l < kLevels; ++l) {
         const bool bx = p.x > cx, by = p.y > cy, bz = p.z > cz;
@@ -308,23 +327,25 @@ __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__
 __global__ __launch_bounds__(256) void bh_level_kernel(int lev, const int *__restrict__ info,
                                                        const double *__restrict__ red, double pr,
                                                        BhNode *__restrict__ nodes, const int8_t *__restrict__ level) {
-    const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (it >= info[0] || level[it] != lev) return;
-    const int end = nodes[it].next;
-    double sx = 0, sy = 0, sz = 0, sm = 0;
-    for (int c = (int)it + 1; c < end; c = nodes[c].next) {
-        const BhNode &q = nodes[c];
-        sx += q.m * q.x;
-        sy += q.m * q.y;
-        sz += q.m * q.z;
-        sm += q.m;
+    const int total = info[0];
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < total; it += gridDim.x * blockDim.x) {
+        if (level[it] != lev) continue;
+        const int end = nodes[it].next;
+        double sx = 0, sy = 0, sz = 0, sm = 0;
+        for (int c = it + 1; c < end; c = nodes[c].next) {
+            const BhNode &q = nodes[c];
+            sx += q.m * q.x;
+            sy += q.m * q.y;
+            sz += q.m * q.z;
+            sm += q.m;
+        }
+        BhNode &o = nodes[it];
+        o.x = sx / sm;
+        o.y = sy / sm;
+        o.z = sz / sm;
+        o.m = sm;
+        o.msoft = cbrt(sm / red[1]) * pr;
     }
-    BhNode &o = nodes[it];
-    o.x = sx / sm;
-    o.y = sy / sm;
-    o.z = sz / sm;
-    o.m = sm;
-    o.msoft = cbrt(sm / red[1]) * pr;
 }
 
 // forces.cc:52-87 (force_softening): r^-3 outside beta, the cubic spline inside
@@ -337,8 +358,206 @@ __device__ __forceinline__ double bh_soft(double r, double beta) {
     return 32.0 / b6 * r3 + -38.4 / b5 * r2 + 32.0 / (3.0 * b3);
 }
 
-// One row per lane, rows in key order: the reference's traversal
-// (forces.cc:378-399), then the external term (forces.cc:474-491).
+// y = 1/r and r from r^2: the v_rsq_f64 seed y0 (relative error |d| <= 5.2e-8
+// on gfx950) refined to second order, y0 (1 + e/2 + 3e^2/8) with
+// e = 1 - r^2 y0^2, relative error below 1e-21 before rounding, so y and r
+// are within 2 ulp of 1/sqrt(r^2) and sqrt(r^2).  The reference's tests
+// side / r < theta and r >= beta become side * y < theta and r >= beta; 1/r^3
+// on the r >= beta branch is y^3.  r^2 = 0 (coincident particles) gives NaN
+// for both: the opening test fails, so such a node is opened as the
+// reference's side / 0 = infinity opens it, and a leaf falls to the softened
+// branch, which takes r = 0 there.
+__device__ __forceinline__ void bh_rsq(double r2, double &y, double &r) {
+    const double y0 = __builtin_amdgcn_rsq(r2);
+    const double e = fma(-r2, y0 * y0, 1.0);
+    y = fma(y0, e * fma(e, 0.375, 0.5), y0);
+    r = r2 * y;
+}
+
+// Pair terms of the traversal, one per topology mode (forces.cc:373-398 R3;
+// :584-652 T3 Ewald and nearest image; :903-981 S1R2 image sum, Ewald table
+// and nearest z-image).
+enum BhMode : int { BH_R3 = 0, BH_T3_NEAR = 1, BH_T3_EWALD = 2, BH_S1R2_NEAR = 3, BH_S1R2_TABLE = 4, BH_S1R2_IMAGES = 5 };
+
+struct BhTables {
+    const double *t3;  // T3 table, device layout (t3_device_layout)
+    int ng, lo;
+    const double *s1r2;  // S1R2 table, device layout (s1r2_device_layout)
+    int nrho, nz;
+    double rho_max;
+    const double *radial;  // cylindrical radial table
+    int radial_size;
+    double Rsim;
+};
+
+__device__ __forceinline__ int bh_imodp(int i, int n) {
+    const int r = i % n;
+    return (r < 0) ? r + n : r;
+}
+
+// ewald_space.cc:398-449: the left cell centre and the offset from it; 1-D
+// Lagrange weights of an even stencil (Catmull-Rom for 4).
+__device__ __forceinline__ int bh_t3_axis(double r, double L, int ng, double &f) {
+    const double gs = L / (double)ng;
+    const double u = (r + L * 0.5) / gs - 0.5;
+    const double uf = floor(u);
+    f = u - uf;
+    return bh_imodp((int)uf, ng);
+}
+
+template <int ORD> __device__ __forceinline__ void bh_t3_weights(double f, double *w) {
+    if (ORD == 4) {  // ewald_space.cc:440-449
+        const double t2 = f * f, t3 = f * t2;
+        w[0] = -0.5 * t3 + t2 - 0.5 * f;
+        w[1] = 1.5 * t3 - 2.5 * t2 + 1.0;
+        w[2] = -1.5 * t3 + 2.0 * t2 + 0.5 * f;
+        w[3] = 0.5 * t3 - 0.5 * t2;
+    } else {  // ewald_space.cc:409-438
+        constexpr int start = -(ORD / 2 - 1);
+#pragma unroll
+        for (int j = 0; j < ORD; ++j) {
+            w[j] = 1.0;
+#pragma unroll
+            for (int k = 0; k < ORD; ++k)
+                if (k != j) w[j] *= (f - (double)(start + k)) / (double)(j - k);
+        }
+    }
+}
+
+// ewald_space.cc:452-523: the T3 correction of the CPU build, Lagrange order
+// ORD (Catmull-Rom for 4) on the cell-centred grid, summed x, then y, then z
+// node by node.  The table is the device layout: x and y nodes lo..Ng-1, a
+// node i < lo read at its mirror Ng-1-i with that axis' component negated;
+// z nodes at layout index iz + 1.
+template <int ORD>
+__device__ __forceinline__ void bh_t3_correction(const BhTables &tb, double L, double dx, double dy, double dz,
+                                                 double &Dx, double &Dy, double &Dz) {
+    const int ng = tb.ng, lo = tb.lo;
+    double fx, fy, fz, wx[ORD], wy[ORD], wz[ORD];
+    const int ix0 = bh_t3_axis(dx, L, ng, fx), iy0 = bh_t3_axis(dy, L, ng, fy), iz0 = bh_t3_axis(dz, L, ng, fz);
+    bh_t3_weights<ORD>(fx, wx);
+    bh_t3_weights<ORD>(fy, wy);
+    bh_t3_weights<ORD>(fz, wz);
+    constexpr int off = -(ORD / 2 - 1);
+    const size_t NY = (size_t)(ng - lo), NZ = (size_t)(ng + 3);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < ORD; ++i) {
+        const int jx = bh_imodp(ix0 + off + i, ng);
+        const size_t mx = (size_t)(jx >= lo ? jx - lo : ng - 1 - lo - jx);
+        const double sx = jx >= lo ? 1.0 : -1.0;
+        for (int j = 0; j < ORD; ++j) {
+            const int jy = bh_imodp(iy0 + off + j, ng);
+            const size_t my = (size_t)(jy >= lo ? jy - lo : ng - 1 - lo - jy);
+            const double sy = jy >= lo ? 1.0 : -1.0;
+            const double wxy = wx[i] * wy[j];
+            const double *row = tb.t3 + (mx * NY + my) * NZ * 3u;
+            for (int k = 0; k < ORD; ++k) {
+                const int iz = bh_imodp(iz0 + off + k, ng);
+                const double w = wxy * wz[k];
+                const double *v = row + (size_t)(iz + 1) * 3u;
+                s0 += w * (sx * v[0]);
+                s1 += w * (sy * v[1]);
+                s2 += w * v[2];
+            }
+        }
+    }
+    Dx = s0;
+    Dy = s1;
+    Dz = s2;
+}
+
+// ewald_space.cc:803-1050: the S1R2 correction, NGP / CIC / TSC (ORD 0 / 2 /
+// 4) in (rho, z), z periodic, rho clamped, then rotated to Cartesian.  The
+// table is the device layout: node (ir, iz) at ((ir + 1) (Nz + 3) + iz + 1) * 2.
+template <int ORD>
+__device__ __forceinline__ void bh_s1r2_correction(const BhTables &tb, double Lz, double dx, double dy, double dz,
+                                                   double &Dx, double &Dy, double &Dz) {
+    const int Nrho = tb.nrho, Nz = tb.nz;
+    const double rho_max = tb.rho_max;
+    const double rho = sqrt(dx * dx + dy * dy);
+    const double half = 0.5 * Lz;
+    const double drho = rho_max / (double)max(1, Nrho - 1);
+    const double dzg = Lz / (double)Nz;
+    const size_t NZ3 = (size_t)(Nz + 3);
+    auto node = [&](int ir, int iz, int c) { return tb.s1r2[(((size_t)(ir + 1)) * NZ3 + (size_t)(iz + 1)) * 2u + c]; };
+    double Dr = 0.0, Dzv = 0.0;
+    if (ORD == 0) {
+        const double rc = fmin(fmax(rho, 0.0), rho_max);
+        const double ur = (drho > 0.0) ? (rc / drho) : 0.0;
+        const double uz = (dz + half) / dzg - 0.5;
+        int ir = (int)floor(ur + 0.5);
+        const int iz = bh_imodp((int)floor(uz + 0.5), Nz);
+        ir = min(max(ir, 0), Nrho - 1);
+        Dr = node(ir, iz, 0);
+        Dzv = node(ir, iz, 1);
+    } else if (ORD == 2) {
+        const double rc = fmax(0.0, fmin(rho, rho_max));
+        const double ur = (drho > 0.0) ? rc / drho : 0.0;
+        int ir0 = (int)floor(ur);
+        double fr = ur - (double)ir0;
+        if (ir0 < 0) {
+            ir0 = 0;
+            fr = 0.0;
+        }
+        if (ir0 > Nrho - 2) {
+            ir0 = max(0, Nrho - 2);
+            fr = 1.0;
+        }
+        const int ir1 = ir0 + 1;
+        const double uz = (dz + half) / dzg - 0.5;
+        int iz0 = (int)floor(uz);
+        const double fz = uz - (double)iz0;
+        iz0 = bh_imodp(iz0, Nz);
+        const int iz1 = bh_imodp(iz0 + 1, Nz);
+        double w00 = 1.0 - fr;
+        w00 *= 1.0 - fz;
+        double w10 = fr;
+        w10 *= 1.0 - fz;
+        double w01 = 1.0 - fr;
+        w01 *= fz;
+        const double w11 = fr * fz;
+        Dr = w00 * node(ir0, iz0, 0) + w10 * node(ir1, iz0, 0) + w01 * node(ir0, iz1, 0) + w11 * node(ir1, iz1, 0);
+        Dzv = w00 * node(ir0, iz0, 1) + w10 * node(ir1, iz0, 1) + w01 * node(ir0, iz1, 1) + w11 * node(ir1, iz1, 1);
+    } else {
+        const double rc = fmin(fmax(rho, 0.0), rho_max);
+        const double ur = (drho > 0.0) ? (rc / drho) : 0.0;
+        const double uz = (dz + half) / dzg - 0.5;
+        const int jr = (int)floor(ur + 0.5), jz = (int)floor(uz + 0.5);
+        const double sr = ur - (double)jr, sz = uz - (double)jz;
+        const double wr[3] = {0.5 * (0.5 - sr) * (0.5 - sr), 0.75 - sr * sr, 0.5 * (0.5 + sr) * (0.5 + sr)};
+        const double wz[3] = {0.5 * (0.5 - sz) * (0.5 - sz), 0.75 - sz * sz, 0.5 * (0.5 + sz) * (0.5 + sz)};
+        const int irs[3] = {max(jr - 1, 0), min(max(jr, 0), Nrho - 1), min(jr + 1, Nrho - 1)};
+        for (int row = 0; row < 3; ++row) {
+            const int iz = bh_imodp(jz - 1 + row, Nz);
+            const double w0 = wr[0] * wz[row], w1 = wr[1] * wz[row], w2 = wr[2] * wz[row];
+            Dr += w0 * node(irs[0], iz, 0) + w1 * node(irs[1], iz, 0) + w2 * node(irs[2], iz, 0);
+            Dzv += w0 * node(irs[0], iz, 1) + w1 * node(irs[1], iz, 1) + w2 * node(irs[2], iz, 1);
+        }
+    }
+    const double ex = (rho > 0.0) ? dx / rho : 0.0;
+    const double ey = (rho > 0.0) ? dy / rho : 0.0;
+    Dx = Dr * ex;
+    Dy = Dr * ey;
+    Dz = Dzv;
+}
+
+// forces.cc:276-289 with order 1: linear interpolation of the radial table,
+// nodes taken at R / size * i (the reference's convention).
+__device__ __forceinline__ double bh_cyl_correction(double r, double R, const double *T, int size) {
+    const double step = R / (double)size;
+    const int i = (int)floor(r / R * (size - 1));
+    double c = T[size - 1];
+    if (i < size - 1) {
+        const double x1 = step * i, y1 = T[i], x2 = step * (i + 1), y2 = T[i + 1];
+        const double A = (y2 - y1) / (x2 - x1);  // utils.cc:24-37
+        const double B = y1 - A * x1;
+        c = A * r + B;
+    }
+    return c;
+}
+
+// One walk of the pre-order list for the lanes of a wave, each lane with its
+// own row (xi, yi, zi, hi, row index i) and decisions, accumulating into f.
 //
 // The wave walks the pre-order list together: the item `it` is the same for
 // every lane (a scalar load of its 64-byte record), and each lane makes its
@@ -350,21 +569,10 @@ __device__ __forceinline__ double bh_soft(double r, double beta) {
 // it, since it took an ancestor).  So each lane sees exactly the items of its
 // own traversal, in the same order: the interactions and their summation
 // order are those of a lane walking alone.
-__global__ __launch_bounds__(256) void bh_force_kernel(const double4 *__restrict__ srec,
-                                                       const double *__restrict__ ssoft,
-                                                       const uint32_t *__restrict__ sidx, int64_t n,
-                                                       const int *__restrict__ info,
-                                                       const BhNode *__restrict__ nodes, double theta, int ext_mode,
-                                                       double ext_coeff, int64_t id_min, int64_t id_max,
-                                                       double *__restrict__ F) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t sc = min(s, n - 1);
-    const int i = (int)sidx[sc];
-    const bool mine = s < n && i >= id_min && i <= id_max;
-    const double4 me = srec[sc];
-    const double hi = ssoft[sc];
-    double fx = 0, fy = 0, fz = 0;
-    const int total = info[0];
+template <int MODE, int ORD>
+__device__ __forceinline__ void bh_walk(const BhNode *__restrict__ nodes, int total, bool mine, int i, double xi,
+                                        double yi, double zi, double hi, double theta, double L, double cut,
+                                        const BhTables &tb, double *f) {
     int wait = mine ? 0 : INT32_MAX;  // the next item this lane takes part in
     int it = 0;
     while (it < total) {
@@ -374,14 +582,37 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const double4 *__restrict
             if (q.pidx == i || q.m == 0.0) {  // the row's own leaf; a massless node
                 wait = q.next;
             } else {
-                const double dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-                const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-                if (q.pidx >= 0 || q.side / dist < theta) {
-                    const double w = q.m * bh_soft(dist, q.msoft + hi);
-                    fx += w * dx;
-                    fy += w * dy;
-                    fz += w * dz;
+                double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                if (MODE == BH_T3_NEAR || MODE == BH_T3_EWALD) {  // forces.cc:594-599
+                    if (fabs(dx) > 0.5 * L) dx -= copysign(L, dx);
+                    if (fabs(dy) > 0.5 * L) dy -= copysign(L, dy);
+                }
+                if (MODE >= BH_T3_NEAR && MODE <= BH_S1R2_TABLE)  // forces.cc:600-601, 941-942
+                    if (fabs(dz) > 0.5 * L) dz -= copysign(L, dz);
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                double y, dist;
+                bh_rsq(r2, y, dist);
+                if (q.pidx >= 0 || q.side * y < theta) {
                     wait = q.next;
+                    if (MODE != BH_S1R2_IMAGES || fabs(dz) <= cut) {  // forces.cc:915
+                        const double beta = q.msoft + hi;
+                        const double w = dist >= beta ? (y * y) * y : bh_soft(r2 > 0.0 ? dist : 0.0, beta);
+                        if constexpr (MODE == BH_T3_EWALD || MODE == BH_S1R2_TABLE) {  // forces.cc:606-610, 947-951
+                            double Dx, Dy, Dz;
+                            if constexpr (MODE == BH_T3_EWALD)
+                                bh_t3_correction<ORD>(tb, L, dx, dy, dz, Dx, Dy, Dz);
+                            else
+                                bh_s1r2_correction<ORD>(tb, L, dx, dy, dz, Dx, Dy, Dz);
+                            f[0] += q.m * (w * dx - Dx);
+                            f[1] += q.m * (w * dy - Dy);
+                            f[2] += q.m * (w * dz - Dz);
+                        } else {
+                            const double wm = q.m * w;
+                            f[0] += wm * dx;
+                            f[1] += wm * dy;
+                            f[2] += wm * dz;
+                        }
+                    }
                 } else {
                     open = true;  // its first child follows in pre-order
                 }
@@ -389,20 +620,62 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const double4 *__restrict
         }
         it = __ballot(open) ? it + 1 : q.next;
     }
+}
+
+// One row per lane, rows in key order: the traversal of the row's topology
+// mode, then its external term (forces.cc:474-491 R3; none for T3;
+// forces.cc:1127-1150, 1170-1190 S1R2 in the x and y components only, times
+// the cylindrical correction unless the Ewald table is in use).  The S1R2
+// image sum walks the tree once per z image m = -(P+1) .. P+1 with the row
+// moved by m L and no wrap, and adds the walks in that order (forces.cc:
+// 1108-1118).
+template <int MODE, int ORD>
+__global__ __launch_bounds__(256) void bh_force_kernel(const double4 *__restrict__ srec,
+                                                       const double *__restrict__ ssoft,
+                                                       const uint32_t *__restrict__ sidx, int64_t n,
+                                                       const int *__restrict__ info,
+                                                       const BhNode *__restrict__ nodes, double theta, double L,
+                                                       int images, BhTables tb, int ext_mode, double ext_coeff,
+                                                       int64_t id_min, int64_t id_max, double *__restrict__ F) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t sc = min(s, n - 1);
+    const int i = (int)sidx[sc];
+    const bool mine = s < n && i >= id_min && i <= id_max;
+    const double4 me = srec[sc];
+    const double hi = ssoft[sc];
+    double f[3] = {0.0, 0.0, 0.0};
+    const int total = info[0];
+    if (MODE == BH_S1R2_IMAGES) {
+        const double cut = ((double)images - 0.4) * L;  // main.cc:1270-1271, forces.cc:1100
+        for (int m = -images; m < images + 1; ++m) {
+            double g[3] = {0.0, 0.0, 0.0};
+            bh_walk<MODE, ORD>(nodes, total, mine, i, me.x, me.y, me.z + (double)m * L, hi, theta, L, cut, tb, g);
+            f[0] += g[0];
+            f[1] += g[1];
+            f[2] += g[2];
+        }
+    } else {
+        bh_walk<MODE, ORD>(nodes, total, mine, i, me.x, me.y, me.z, hi, theta, L, 0.0, tb, f);
+    }
     if (!mine) return;
     if (ext_mode == 1) {
-        fx += ext_coeff * me.x;
-        fy += ext_coeff * me.y;
-        fz += ext_coeff * me.z;
+        f[0] += ext_coeff * me.x;
+        f[1] += ext_coeff * me.y;
+        f[2] += ext_coeff * me.z;
+    } else if (ext_mode == 2 || ext_mode == 3) {
+        double k = ext_coeff;
+        if (ext_mode == 3) k *= bh_cyl_correction(sqrt(me.x * me.x + me.y * me.y), tb.Rsim, tb.radial, tb.radial_size);
+        f[0] += k * me.x;
+        f[1] += k * me.y;
     }
     double *o = F + 3 * (i - id_min);
-    o[0] = fx;
-    o[1] = fy;
-    o[2] = fz;
+    o[0] = f[0];
+    o[1] = f[1];
+    o[2] = f[2];
 }
 
 struct BhView {
-    double *red;  // kRedBlocks partials (max |x|, min m), then root side and M_min
+    double *red;  // kRedBlocks partials (max radius, min m), then root side, M_min, root centre
     uint32_t *khi, *klo, *base, *bsum;
     double4 *srec;
     double *ssoft;
@@ -418,7 +691,7 @@ struct BhView {
 // the pieces of the scratch buffer, 256-byte aligned, in this order
 template <class Take> void bh_pieces(int64_t n, Take &&take) {
     const size_t nn = (size_t)std::max<int64_t>(n, 1), cap = (size_t)bh_item_capacity(n);
-    take((size_t)(2 * kRedBlocks + 2) * 8);  // red
+    take((size_t)(2 * kRedBlocks + 5) * 8);  // red
     take(nn * 4);                             // khi
     take(nn * 4);                             // klo
     take(nn * 4);                             // base
@@ -484,9 +757,12 @@ hipError_t launch_bh_forces(const EvalPlan &p, const DeviceParticles &dp, void *
     const int nb = (int)std::min<int64_t>(kRedBlocks, (n + 255) / 256);
     const int rows = table_rows(n);
     hipError_t e;
-    double *red = v.red + 2 * kRedBlocks;  // root side, M_min
-    hipLaunchKernelGGL(bh_reduce_partial, dim3(nb), dim3(256), 0, st, dp.jrec, n, v.red);
-    hipLaunchKernelGGL(bh_reduce_final, dim3(1), dim3(256), 0, st, v.red, nb, red);
+    double *red = v.red + 2 * kRedBlocks;  // root side, M_min, root centre
+    const int topology = (p.variant == PV_R3)                                     ? STEPS_TOPO_R3
+                         : (p.variant == PV_T3_NEAREST || p.variant == PV_T3_EWALD) ? STEPS_TOPO_T3
+                                                                                    : STEPS_TOPO_S1R2;
+    hipLaunchKernelGGL(bh_reduce_partial, dim3(nb), dim3(256), 0, st, dp.jrec, n, topology == STEPS_TOPO_S1R2, v.red);
+    hipLaunchKernelGGL(bh_reduce_final, dim3(1), dim3(256), 0, st, v.red, nb, topology, p.L, red);
     hipLaunchKernelGGL(bh_keys_kernel, dim3(ng), dim3(256), 0, st, dp.jrec, n, red, v.khi, v.klo);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const uint32_t *perm = nullptr;
@@ -502,14 +778,39 @@ hipError_t launch_bh_forces(const EvalPlan &p, const DeviceParticles &dp, void *
     hipLaunchKernelGGL(bh_fill_kernel, dim3(ng), dim3(256), 0, st, v.tab, rows, v.base, v.srec, v.sidx, n, red,
                        p.bh_particle_radius, v.info, v.nodes, v.level);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const unsigned gi = (unsigned)((bh_item_capacity(n) + 255) / 256);
+    // the items of one level are few next to the grid: a grid-stride loop over
+    // the items the tree has (info[0], known only on the device)
+    const unsigned gi = (unsigned)std::min<int64_t>(4096, (bh_item_capacity(n) + 255) / 256);
     for (int l = kLevels; l >= 0; --l)
         hipLaunchKernelGGL(bh_level_kernel, dim3(gi), dim3(256), 0, st, l, v.info, red, p.bh_particle_radius,
                            v.nodes, v.level);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_tree_end && (e = hipEventRecord(ev_tree_end, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(bh_force_kernel, dim3(ng), dim3(256), 0, st, v.srec, v.ssoft, v.sidx, n, v.info, v.nodes,
-                       p.bh_theta, p.ext_mode, p.ext_coeff, p.id_min, p.id_min + p.nrows - 1, F);
+    BhTables tb{p.t3, p.t3_ng, p.t3_lo, p.s1r2, p.s1r2_nrho, p.s1r2_nz, kRadialExtent * p.Rsim, p.radial,
+                p.radial_size, p.Rsim};
+    const int64_t id_max = p.id_min + p.nrows - 1;
+#define STEPS_BH_LAUNCH(MODE, ORD)                                                                                  \
+    hipLaunchKernelGGL((bh_force_kernel<MODE, ORD>), dim3(ng), dim3(256), 0, st, v.srec, v.ssoft, v.sidx, n, v.info, \
+                       v.nodes, p.bh_theta, p.L, p.images, tb, p.ext_mode, p.ext_coeff, p.id_min, id_max, F)
+    switch (p.variant) {
+    case PV_R3: STEPS_BH_LAUNCH(BH_R3, 0); break;
+    case PV_T3_NEAREST: STEPS_BH_LAUNCH(BH_T3_NEAR, 0); break;
+    case PV_T3_EWALD:
+        switch (p.bh_order) {
+        case 4: STEPS_BH_LAUNCH(BH_T3_EWALD, 4); break;
+        case 6: STEPS_BH_LAUNCH(BH_T3_EWALD, 6); break;
+        case 8: STEPS_BH_LAUNCH(BH_T3_EWALD, 8); break;
+        default: STEPS_BH_LAUNCH(BH_T3_EWALD, 2); break;
+        }
+        break;
+    case PV_S1R2_NEAREST: STEPS_BH_LAUNCH(BH_S1R2_NEAR, 0); break;
+    case PV_S1R2_NGP: STEPS_BH_LAUNCH(BH_S1R2_TABLE, 0); break;
+    case PV_S1R2_CIC: STEPS_BH_LAUNCH(BH_S1R2_TABLE, 2); break;
+    case PV_S1R2_TSC: STEPS_BH_LAUNCH(BH_S1R2_TABLE, 4); break;
+    case PV_S1R2_IMAGES: STEPS_BH_LAUNCH(BH_S1R2_IMAGES, 0); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef STEPS_BH_LAUNCH
     return hipGetLastError();
 }
 

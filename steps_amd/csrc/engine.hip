@@ -168,18 +168,6 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
     switch (d.topology) {
     case STEPS_TOPO_R3:
         p.variant = PV_R3;
-        if (d.bh_theta != 0.0) {
-            // USE_BH (main.cc:302-303): the octree instead of the direct sum
-            if (!(d.bh_theta > 0.0) || !std::isfinite(d.bh_theta))
-                return fail(STEPS_ERR_ARG, "Barnes-Hut opening angle must be positive and finite");
-            if (!(d.bh_particle_radius >= 0.0) || !std::isfinite(d.bh_particle_radius))
-                return fail(STEPS_ERR_ARG, "Barnes-Hut particle radius must be non-negative and finite");
-            if (d.precision != STEPS_PREC_FP64) return fail(STEPS_ERR_ARG, "Barnes-Hut forces are FP64 only");
-            if (bh_item_capacity(N) > INT32_MAX) return fail(STEPS_ERR_ARG, "N too large for the octree");
-            p.variant = PV_R3_BH;
-            p.bh_theta = d.bh_theta;
-            p.bh_particle_radius = d.bh_particle_radius;
-        }
         if (d.cosmology == 1) {
             p.ext_mode = 1;
             p.ext_coeff = (d.comoving == 1) ? d.mass_in_unit_sphere : DE;
@@ -238,8 +226,23 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
     default:
         return fail(STEPS_ERR_ARG, "unknown topology");
     }
-    if (d.bh_theta != 0.0 && p.variant != PV_R3_BH)
-        return fail(STEPS_ERR_ARG, "Barnes-Hut forces are implemented for the free (R3) boundary only");
+    if (d.bh_theta != 0.0) {
+        // USE_BH (main.cc:302-303): the octree instead of the direct sum
+        if (!(d.bh_theta > 0.0) || !std::isfinite(d.bh_theta))
+            return fail(STEPS_ERR_ARG, "Barnes-Hut opening angle must be positive and finite");
+        if (!(d.bh_particle_radius >= 0.0) || !std::isfinite(d.bh_particle_radius))
+            return fail(STEPS_ERR_ARG, "Barnes-Hut particle radius must be non-negative and finite");
+        if (d.precision != STEPS_PREC_FP64) return fail(STEPS_ERR_ARG, "Barnes-Hut forces are FP64 only");
+        if (bh_item_capacity(N) > INT32_MAX) return fail(STEPS_ERR_ARG, "N too large for the octree");
+        p.bh = true;
+        p.bh_theta = d.bh_theta;
+        p.bh_particle_radius = d.bh_particle_radius;
+        // the CPU build's T3 table interpolation (ewald_space.cc:452-466): even
+        // orders up to 8, anything else falls back to 2
+        p.bh_order = d.interp_order;
+        if (p.bh_order < 2 || p.bh_order % 2 != 0) p.bh_order = 2;
+        if (p.bh_order > 8) p.bh_order = 8;
+    }
     if (p.precision == STEPS_PREC_MIXED && !mixed_supported(p.variant))
         return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3, T3 and S1R2 (nearest z-image, CIC or TSC correction)");
 
@@ -269,7 +272,7 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
 }
 
 size_t scratch_bytes(const EvalPlan &p) {
-    if (p.variant == PV_R3_BH) return bh_scratch_bytes(p.N);
+    if (p.bh) return bh_scratch_bytes(p.N);
     return ((partial_bytes(p) + 255) & ~(size_t)255) + tile_extra_bytes(p);
 }
 

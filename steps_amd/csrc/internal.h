@@ -28,7 +28,6 @@ enum PairVariant : int {
     PV_S1R2_CIC = 5,      //   ... order 2
     PV_S1R2_TSC = 6,      //   ... order 4
     PV_S1R2_IMAGES = 7,   // forces_cuda.cu:711-759   (PERIODIC_Z_NOLOOKUP, IS_PERIODIC >= 2)
-    PV_R3_BH = 8,         // forces.cc:378-506        (USE_BH, free boundary; CPU-only in the reference)
 };
 
 constexpr bool is_s1r2_lookup(int v) { return v == PV_S1R2_NGP || v == PV_S1R2_CIC || v == PV_S1R2_TSC; }
@@ -80,7 +79,13 @@ struct EvalPlan {
     const double *s1r2 = nullptr; int s1r2_nrho = 0, s1r2_nz = 0;
     const double *radial = nullptr; int radial_size = 0;
     int images = 0;        // S1R2 image-sum: ewald_max
-    double bh_theta = 0, bh_particle_radius = 0;  // Barnes-Hut opening angle, ParticleRadi
+    // Barnes-Hut octree instead of the direct sum (USE_BH; forces.cc:373-506,
+    // 582-772, 881-1217, CPU-only in the reference): the variant above then
+    // names the pair term of each accepted node, and bh_order the T3 table
+    // interpolation of the CPU build (Lagrange order 2, 6, 8; 4 Catmull-Rom)
+    bool bh = false;
+    double bh_theta = 0, bh_particle_radius = 0;  // opening angle, ParticleRadi
+    int bh_order = 2;
 };
 
 // Scratch of one evaluation: partial sums [splits][3][nrows] (FP64), followed

@@ -181,6 +181,64 @@ __global__ __launch_bounds__(1024) void exclusive_scan_kernel(uint32_t *__restri
     }
 }
 
+// The same scan over many workgroups, for large count arrays: each
+// workgroup scans kScanChunk consecutive counts in place and writes its total
+// (scan_chunk_kernel), one workgroup scans the totals (exclusive_scan_kernel),
+// and every count gets its chunk's offset added (scan_add_kernel).  Integer
+// sums: the result is the single-workgroup scan's exactly.
+constexpr int kScanChunk = 1024;
+constexpr int64_t kScanOneGroupMax = 16384;  // up to this many counts: one workgroup, one launch
+
+__global__ __launch_bounds__(256) void scan_chunk_kernel(uint32_t *__restrict__ a, int64_t total,
+                                                         uint32_t *__restrict__ bsum) {
+    __shared__ uint32_t s[256];
+    constexpr int kPer = kScanChunk / 256;
+    const int t = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * kScanChunk + (int64_t)t * kPer;
+    uint32_t c[kPer], sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        c[q] = (i0 + q < total) ? a[i0 + q] : 0u;
+        sum += c[q];
+    }
+    s[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = (t >= off) ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[t] - sum;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        if (i0 + q < total) a[i0 + q] = run;
+        run += c[q];
+    }
+    if (t == 255) bsum[blockIdx.x] = s[255];
+}
+
+__global__ __launch_bounds__(256) void scan_add_kernel(uint32_t *__restrict__ a, int64_t total,
+                                                       const uint32_t *__restrict__ bsum) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) a[i] += bsum[i / kScanChunk];
+}
+
+int64_t scan_chunks(int64_t total) { return (total + kScanChunk - 1) / kScanChunk; }
+
+// Exclusive scan of total counts in place; bsum holds scan_chunks(total) words.
+hipError_t launch_counts_scan(uint32_t *counts, int64_t total, uint32_t *bsum, hipStream_t st) {
+    if (total <= kScanOneGroupMax) {
+        hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, counts, total);
+        return hipGetLastError();
+    }
+    const int64_t nc = scan_chunks(total);
+    hipLaunchKernelGGL(scan_chunk_kernel, dim3((unsigned)nc), dim3(256), 0, st, counts, total, bsum);
+    hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, bsum, nc);
+    hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, counts, total, bsum);
+    return hipGetLastError();
+}
+
 // Stable scatter by one 8-bit digit: within a block, elements are placed in
 // index order (rounds of 256, waves in order, lanes in order); the rank of a
 // lane among the lanes of its wave holding the same digit comes from eight
@@ -259,13 +317,12 @@ __global__ __launch_bounds__(256) void gather_u32_kernel(const uint32_t *__restr
 
 // One stable LSD pass over 8 bits of key[cur] (moving val[cur] along).
 hipError_t radix_pass(uint32_t *key[2], uint32_t *val[2], int &cur, int64_t n, int shift, uint32_t *counts,
-                      hipStream_t st) {
+                      uint32_t *bsum, hipStream_t st) {
     const int nblk = sort_blocks(n);
     hipLaunchKernelGGL(radix_hist_kernel, dim3(nblk), dim3(kSortThreads), 0, st, key[cur], n, shift, nblk, counts);
     hipError_t e;
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, counts, (int64_t)256 * nblk);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_counts_scan(counts, (int64_t)256 * nblk, bsum, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(radix_scatter_kernel, dim3(nblk), dim3(kSortThreads), 0, st, key[cur], val[cur], n, shift,
                        counts, nblk, key[1 - cur], val[1 - cur]);
     cur = 1 - cur;
@@ -276,7 +333,8 @@ hipError_t radix_pass(uint32_t *key[2], uint32_t *val[2], int &cur, int64_t n, i
 
 size_t sort_u64_scratch_bytes(int64_t n) {
     const size_t nn = (size_t)std::max<int64_t>(n, 1);
-    return 4 * a256(nn * 4) + a256((size_t)256 * sort_blocks(n) * 4) + 256;
+    return 4 * a256(nn * 4) + a256((size_t)256 * sort_blocks(n) * 4) +
+           a256((size_t)scan_chunks((int64_t)256 * sort_blocks(n)) * 4) + 256;
 }
 
 hipError_t launch_sort_u64(const uint32_t *hi, const uint32_t *lo, int64_t n, void *scratch, const uint32_t **perm,
@@ -290,19 +348,20 @@ hipError_t launch_sort_u64(const uint32_t *hi, const uint32_t *lo, int64_t n, vo
     val[0] = (uint32_t *)(p + 2 * a256(nn * 4));
     val[1] = (uint32_t *)(p + 3 * a256(nn * 4));
     uint32_t *counts = (uint32_t *)(p + 4 * a256(nn * 4));
+    uint32_t *bsum = (uint32_t *)(p + 4 * a256(nn * 4) + a256((size_t)256 * sort_blocks(n) * 4));
     const unsigned ng = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(iota_copy_kernel, dim3(ng), dim3(256), 0, st, lo, n, key[0], val[0]);
     hipError_t e;
     if ((e = hipGetLastError()) != hipSuccess) return e;
     int cur = 0;
     for (int shift = 0; shift < 32; shift += 8)
-        if ((e = radix_pass(key, val, cur, n, shift, counts, st)) != hipSuccess) return e;
+        if ((e = radix_pass(key, val, cur, n, shift, counts, bsum, st)) != hipSuccess) return e;
     // the high halves, in the order the low-half passes left: a stable sort
     // by them completes the sort by (hi, lo)
     hipLaunchKernelGGL(gather_u32_kernel, dim3(ng), dim3(256), 0, st, hi, val[cur], n, key[cur]);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     for (int shift = 0; shift < 32; shift += 8)
-        if ((e = radix_pass(key, val, cur, n, shift, counts, st)) != hipSuccess) return e;
+        if ((e = radix_pass(key, val, cur, n, shift, counts, bsum, st)) != hipSuccess) return e;
     *perm = val[cur];
     return hipSuccess;
 }
@@ -310,7 +369,7 @@ hipError_t launch_sort_u64(const uint32_t *hi, const uint32_t *lo, int64_t n, vo
 size_t order_scratch_bytes(int64_t n) {
     const size_t nn = (size_t)std::max<int64_t>(n, 1);
     return 4 * a256(nn * 4) + a256((size_t)256 * sort_blocks(n) * 4) + a256((size_t)kBboxBlocks * 6 * 8) +
-           a256(6 * 8) + 256;
+           a256(6 * 8) + a256((size_t)scan_chunks((int64_t)256 * sort_blocks(n)) * 4) + 256;
 }
 
 const uint32_t *spatial_order_keys(const void *scratch, int64_t n) {
@@ -343,6 +402,8 @@ hipError_t launch_spatial_order(const double4 *jrec, const double *soft, int64_t
     double *part = (double *)p;
     p += a256((size_t)kBboxBlocks * 6 * 8);
     double *box = (double *)p;
+    p += a256(6 * 8);
+    uint32_t *bsum = (uint32_t *)p;
 
     hipError_t e;
     const unsigned ng = (unsigned)((n + 255) / 256);
@@ -362,8 +423,7 @@ hipError_t launch_spatial_order(const double4 *jrec, const double *soft, int64_t
         hipLaunchKernelGGL(radix_hist_kernel, dim3(nblk), dim3(kSortThreads), 0, st, key[cur], n, shift, nblk,
                            counts);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, counts, (int64_t)256 * nblk);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = launch_counts_scan(counts, (int64_t)256 * nblk, bsum, st)) != hipSuccess) return e;
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(nblk), dim3(kSortThreads), 0, st, key[cur], val[cur], n,
                            shift, counts, nblk, key[1 - cur], val[1 - cur]);
         if ((e = hipGetLastError()) != hipSuccess) return e;

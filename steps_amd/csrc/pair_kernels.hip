@@ -2324,7 +2324,7 @@ size_t tile_extra_bytes(const EvalPlan &p) {
 
 hipError_t launch_forces_prepare(const EvalPlan &p, const DeviceParticles &dp, double *partials,
                                  hipStream_t st) {
-    if (p.variant == PV_R3_BH) return hipErrorInvalidValue;  // launch_forces runs the octree whole
+    if (p.bh) return hipErrorInvalidValue;  // launch_forces runs the octree whole
     if (!has_pairs(p)) return hipSuccess;
     const ScratchView v = scratch_view(p, partials);
     const unsigned nt = (unsigned)((p.N + kTile - 1) / kTile);
@@ -2399,7 +2399,7 @@ hipError_t launch_forces_finish(const EvalPlan &p, const DeviceParticles &dp, do
 hipError_t launch_forces(const EvalPlan &p, const DeviceParticles &dp, double *partials, double *F, hipStream_t st,
                          hipEvent_t ev_begin, hipEvent_t ev_pair_end, hipEvent_t ev_end) {
     hipError_t e;
-    if (p.variant == PV_R3_BH) {  // tree build and traversal between the first two events
+    if (p.bh) {  // tree build and traversal between the first two events
         if (ev_begin && (e = hipEventRecord(ev_begin, st)) != hipSuccess) return e;
         if ((e = launch_bh_forces(p, dp, partials, F, st, nullptr)) != hipSuccess) return e;
         if (ev_pair_end && (e = hipEventRecord(ev_pair_end, st)) != hipSuccess) return e;

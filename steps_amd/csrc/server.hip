@@ -638,7 +638,7 @@ int steps_server_add_sim(steps_server *S, const steps_force_desc *d, int64_t N, 
     EvalPlan p;
     rc = make_plan(*d, N, 0, N - 1, p);  // validates the descriptor and its tables
     if (rc) return rc;
-    if (p.variant == PV_R3_BH)
+    if (p.bh)
         return fail(STEPS_ERR_ARG, "the server batches direct-sum simulations; run Barnes-Hut ones through steps_sim");
     if ((rc = check_finite(x, 3 * N))) return rc;
     SimEntry e;

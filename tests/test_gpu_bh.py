@@ -1,6 +1,6 @@
 """Barnes-Hut octree forces on the device (the reference's USE_BH engine,
-forces.cc:97-187, 378-506, CPU-only there) against the oracle's restatement
-of the reference's pointer octree.  The device builds the same tree (cubes
+forces.cc:97-187, 373-506 R3, 582-772 T3, 881-1217 S1R2, CPU-only there)
+against the oracle's restatement of the reference's pointer octree.  The device builds the same tree (cubes
 holding two or more particles, children in octant order) and makes the same
 accept / open decision per (row, node), so the forces agree to the rounding
 of the centres of mass and of the sums: per-row relative error <= 1e-12,
@@ -94,7 +94,7 @@ def test_bh_refusals(steps):
     bad = [steps.make_desc(steps.TOPO_R3, bh_theta=-0.5),
            steps.make_desc(steps.TOPO_R3, bh_theta=0.5, bh_particle_radius=-1.0),
            steps.make_desc(steps.TOPO_R3, bh_theta=0.5, precision=steps.PREC_MIXED),
-           steps.make_desc(steps.TOPO_T3, is_periodic=1, L=10.0, bh_theta=0.5)]
+           steps.make_desc(steps.TOPO_T3, is_periodic=1, L=10.0, bh_theta=float("nan"))]
     for d, keep in bad:
         with pytest.raises(_abi.StepsError) as ei:
             steps.force_eval(d, w.x, w.M, w.soft)
@@ -105,3 +105,50 @@ def test_bh_refusals(steps):
         srv.add_sim(d, w.x, w.M, w.soft)
     assert ei.value.code == _abi.STEPS_ERR_ARG
     srv.close()
+
+
+# ---- periodic topologies (forces.cc:582-772 T3, 881-1217 S1R2) ------------------
+
+def _periodic_case(steps, oracle, mode):
+    """(workload, device descriptor + keepalive, oracle params) of one periodic octree mode."""
+    if mode.startswith("t3"):
+        ip = 1 if mode == "t3_nearest" else 2
+        w = wl.t3_lattice(n_side=16, seed=21, is_periodic=ip)
+        order = {"t3_nearest": 2, "t3_ewald_o2": 2, "t3_ewald_o4": 4, "t3_ewald_o6": 6}[mode]
+        T = steps.t3_table(2, w.L) if ip >= 2 else None
+        d = steps.make_desc(steps.TOPO_T3, is_periodic=ip, L=w.L, t3_table=T, interp_order=order, bh_theta=0.5,
+                            bh_particle_radius=w.particle_radius)
+        p = oracle.params(1, is_periodic=ip, L=w.L, t3_table=T, interp_order=order)
+        return w, d, p
+    w = wl.s1r2_cylinder(n_omega=16, n_phi=16, n_z=16, seed=22, is_periodic=1 if mode == "s1r2_nearest" else 2)
+    kw = dict(L=w.L, Rsim=w.Rsim, cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere)
+    bh = dict(bh_theta=0.5, bh_particle_radius=w.particle_radius)
+    if mode == "s1r2_nearest":
+        R = steps.radial_table(w.Rsim, 0.5 * w.L, 1000, 1000)
+        d = steps.make_desc(steps.TOPO_S1R2, is_periodic=1, radial_table=R, **kw, **bh)
+        p = oracle.params(2, is_periodic=1, radial_table=R, **kw)
+    elif mode == "s1r2_images":
+        R = steps.radial_table(w.Rsim, w.L * (2 + 1 - 0.4), 1000, 1000)
+        d = steps.make_desc(steps.TOPO_S1R2, is_periodic=2, s1r2_lookup=0, radial_table=R, **kw, **bh)
+        p = oracle.params(2, is_periodic=2, s1r2_lookup=0, cut_inclusive=1, radial_table=R, **kw)
+    else:
+        order = {"s1r2_ngp": 0, "s1r2_cic": 2, "s1r2_tsc": 4}[mode]
+        T, shape = steps.s1r2_table(2, w.L, w.Rsim)
+        d = steps.make_desc(steps.TOPO_S1R2, is_periodic=2, s1r2_table=T, s1r2_shape=shape, interp_order=order,
+                            **kw, **bh)
+        p = oracle.params(2, is_periodic=2, s1r2_table=T, s1r2_shape=shape, interp_order=order, **kw)
+    return w, d, p
+
+
+@pytest.mark.parametrize("mode", ["t3_nearest", "t3_ewald_o2", "t3_ewald_o4", "t3_ewald_o6", "s1r2_nearest",
+                                  "s1r2_ngp", "s1r2_cic", "s1r2_tsc", "s1r2_images"])
+def test_bh_periodic_matches_oracle(steps, oracle, mode):
+    print(f"case bh periodic {mode}", flush=True)
+    w, (d, keep), p = _periodic_case(steps, oracle, mode)
+    Fo = oracle.forces_bh(p, 0.5, w.particle_radius, w.x, w.M, w.soft)
+    F = steps.force_eval(d, w.x, w.M, w.soft)
+    e = rel_err(F, Fo)
+    report(f"bh {mode} N={w.N}", e)
+    assert e.max() <= 1e-12
+    lo, hi = 100, 900
+    assert np.array_equal(steps.force_eval(d, w.x, w.M, w.soft, id_min=lo, id_max=hi), F[lo:hi + 1])

@@ -52,3 +52,61 @@ def test_bh_error_against_direct_sum(oracle):
         errs.append(np.median(e))
     assert errs[0] > 3 * errs[1] > 9 * errs[2] > 27 * errs[3]
     assert errs[3] < 1e-3
+
+
+# ---- periodic topologies (forces.cc:582-772 T3, 881-1217 S1R2) ------------------
+# With an opening angle so small that no internal node is taken, every other
+# particle's leaf is visited: the octree's pair term then has to be the direct
+# sum's (nearest image, the same table corrections, the same image cut), which
+# the oracle computes by a separate code path (pair_row).
+
+def _rel(F, R):
+    return np.linalg.norm(F - R, axis=1) / np.linalg.norm(R, axis=1)
+
+
+@pytest.mark.parametrize("is_periodic,order", [(1, 4), (2, 2), (2, 4)])
+def test_bh_t3_tiny_theta_is_the_direct_sum(oracle, is_periodic, order):
+    w = wl.t3_lattice(n_side=10, seed=3, is_periodic=is_periodic)
+    T = oracle.t3_table(2, w.L) if is_periodic >= 2 else None
+    p = oracle.params(1, is_periodic=is_periodic, L=w.L, t3_table=T, interp_order=order)
+    Fd = oracle.forces(p, w.x, w.M, w.soft)
+    Fb = oracle.forces_bh(p, 1e-12, w.particle_radius, w.x, w.M, w.soft)
+    e = _rel(Fb, Fd)
+    assert np.median(e) < 1e-13 and e.max() < 1e-10
+
+
+@pytest.mark.parametrize("mode", ["nearest", "ngp", "cic", "tsc", "images"])
+def test_bh_s1r2_tiny_theta_is_the_direct_sum(oracle, mode):
+    c = wl.s1r2_cylinder(n_omega=8, n_phi=8, n_z=8, seed=4, is_periodic=1 if mode == "nearest" else 2)
+    kw = dict(L=c.L, Rsim=c.Rsim, cosmology=1, comoving=1, mass_in_unit_sphere=c.mass_in_unit_sphere)
+    if mode == "nearest":
+        p = oracle.params(2, is_periodic=1, radial_table=oracle.radial_table(c.Rsim, 0.5 * c.L, 1000, 1000), **kw)
+    elif mode == "images":
+        # the CPU image sum keeps |dz| <= (ewald_max - 0.4) L (forces.cc:915), as the direct CPU sum does
+        p = oracle.params(2, is_periodic=2, s1r2_lookup=0, cut_inclusive=1,
+                          radial_table=oracle.radial_table(c.Rsim, c.L * (2 + 1 - 0.4), 1000, 1000), **kw)
+    else:
+        T, shape = oracle.s1r2_table(2, c.L, c.Rsim)
+        p = oracle.params(2, is_periodic=2, s1r2_table=T, s1r2_shape=shape,
+                          interp_order={"ngp": 0, "cic": 2, "tsc": 4}[mode], **kw)
+    Fd = oracle.forces(p, c.x, c.M, c.soft)
+    if mode in ("ngp", "cic", "tsc"):
+        # the direct sum includes the self pair, whose table term is -m_i D(0);
+        # the octree skips the row's own leaf.  D(0) is the node at (0, +dz/2)
+        # for NGP and vanishes (antisymmetric neighbours) for CIC and TSC
+        D0 = oracle.s1r2_interp(T, shape, 2.25 * c.Rsim, c.L, np.zeros(3), p.interp_order)
+        assert (mode == "ngp") == (abs(D0[2]) > 1e-3 * np.abs(T).max())
+        Fd = Fd + c.M[:, None] * D0[None, :]
+    Fb = oracle.forces_bh(p, 1e-12, c.particle_radius, c.x, c.M, c.soft)
+    e = _rel(Fb, Fd)
+    assert np.median(e) < 1e-12 and e.max() < 1e-9, (np.median(e), e.max())
+
+
+def test_bh_periodic_error_shrinks_with_theta(oracle):
+    w = wl.t3_lattice(n_side=12, seed=5, is_periodic=2)
+    T = oracle.t3_table(2, w.L)
+    p = oracle.params(1, is_periodic=2, L=w.L, t3_table=T, interp_order=2)
+    Fd = oracle.forces(p, w.x, w.M, w.soft)
+    errs = [np.median(_rel(oracle.forces_bh(p, th, w.particle_radius, w.x, w.M, w.soft), Fd))
+            for th in (0.8, 0.4, 0.2)]
+    assert errs[0] > 2 * errs[1] > 4 * errs[2]
