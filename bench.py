@@ -423,7 +423,42 @@ def bh_leg(w, kw, F64, direct_ms: float, reps: int) -> dict:
                                      f"({t_build:.2f} s), rows 0..{rows - 1} traversed on {threads} OpenMP threads "
                                      f"({t_rows - t_build:.2f} s), scaled to {N} rows"}
     out["theta_0.5"]["speedup_vs_cpu_port"] = t_eval * 1e3 / out["theta_0.5"]["eval_ms"]
+    out["periodic"] = bh_periodic(reps)
     print(f"[bench] barnes-hut: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def bh_periodic(reps: int) -> dict:
+    """The octree on the periodic measurement configurations at theta 0.5
+    (forces.cc:582-772 T3, 881-1217 S1R2): T3 Ewald with the Catmull-Rom
+    table interpolation (the CPU build's order 4, the direct GPU kernel's
+    interpolation) on the 64^3 lattice, and S1xR2 with the CIC table on the
+    N = 524288 cylinder; evaluation time and error against the FP64 direct
+    sum of the same configuration."""
+    out = {}
+    t3 = wl.t3_lattice()
+    T = steps_amd.t3_table(2, t3.L)
+    c = wl.s1r2_cylinder()
+    Ts, shape = steps_amd.s1r2_table(2, c.L, c.Rsim)
+    kw = dict(L=c.L, Rsim=c.Rsim, cosmology=1, comoving=1, mass_in_unit_sphere=c.mass_in_unit_sphere)
+    cases = [("t3_ewald_order4", t3, steps_amd.TOPO_T3, dict(is_periodic=2, L=t3.L, t3_table=T, interp_order=4)),
+             ("s1r2_cic", c, steps_amd.TOPO_S1R2, dict(is_periodic=2, s1r2_table=Ts, s1r2_shape=shape,
+                                                        interp_order=2, **kw))]
+    for name, w, topo, dkw in cases:
+        res = {"N": w.N}
+        for tag, extra in (("direct", {}), ("bh", dict(bh_theta=0.5, bh_particle_radius=w.particle_radius))):
+            d, keep = steps_amd.make_desc(topo, **dkw, **extra)
+            sim = steps_amd.Sim(d, w.N)
+            sim.set_state(w.x, w.M, w.soft, w.v)
+            _, ms, _ = time_path(sim, reps if tag == "bh" else 1, 1)
+            res[tag + "_ms"] = ms
+            res[tag + "_F"] = sim.get_forces()
+            sim.close()
+        err = np.linalg.norm(res.pop("bh_F") - res["direct_F"], axis=1) / np.linalg.norm(res.pop("direct_F"), axis=1)
+        res.update({"speedup_vs_direct_fp64": res["direct_ms"] / res["bh_ms"],
+                    "rel_err_vs_direct_p50": float(np.median(err)),
+                    "rel_err_vs_direct_p99": float(np.percentile(err, 99))})
+        out[name] = res
     return out
 
 

@@ -2,11 +2,14 @@
 // with the force evaluations and the kick-drift-kick steps on the device.
 //
 //   steps_run <parameter file> [--topology r3|t3|s1r2] [--precision fp64|mixed]
-//             [--interp-order 0|2|4] [--no-lookup] [--max-steps K] [--device D]
+//             [--interp-order 0|2|4|6|8] [--no-lookup] [--bh THETA]
+//             [--max-steps K] [--device D]
 //
-// The topology, the S1xR2 interpolation order and the table / no-table mode
-// are build flags in the reference (PERIODIC, PERIODIC_Z,
-// EWALD_INTERPOLATION_ORDER, PERIODIC_Z_NOLOOKUP); here they are options.
+// The topology, the interpolation order, the table / no-table mode and the
+// Barnes-Hut octree with its opening angle are build flags in the reference
+// (PERIODIC, PERIODIC_Z, EWALD_INTERPOLATION_ORDER, PERIODIC_Z_NOLOOKUP,
+// USE_BH=THETA); here they are options.  The octree's radial correction
+// table (RADIAL_BH_FORCE_CORRECTION, main.cc:950-1019) is not built.
 // The run follows main.cc:1100-1864 and step.cc:100-312:
 //   * initial conditions (ASCII or GADGET), H0-independent units, velocity
 //     rescaling (main.cc:1144-1181); equal GADGET masses for T3, the mass
@@ -69,6 +72,7 @@ struct Opts {
     int precision = STEPS_PREC_FP64;
     int interp_order = 2;  // EWALD_INTERPOLATION_ORDER of the reference's Makefile
     int lookup = 1;
+    double bh_theta = 0.0;
     long max_steps = -1;
     int device = 0;
 };
@@ -93,6 +97,9 @@ bool parse_opts(int argc, char **argv, Opts &o) {
             o.interp_order = std::atoi(val().c_str());
         } else if (a == "--no-lookup") {
             o.lookup = 0;
+        } else if (a == "--bh") {
+            o.bh_theta = std::atof(val().c_str());
+            if (!(o.bh_theta > 0.0)) return false;
         } else if (a == "--max-steps") {
             o.max_steps = std::atol(val().c_str());
         } else if (a == "--device") {
@@ -124,7 +131,7 @@ int main(int argc, char **argv) {
     Opts o;
     if (!parse_opts(argc, argv, o))
         return die("usage: steps_run <parameter file> [--topology r3|t3|s1r2] [--precision fp64|mixed] "
-                   "[--interp-order 0|2|4] [--no-lookup] [--max-steps K] [--device D]", 2);
+                   "[--interp-order 0|2|4|6|8] [--no-lookup] [--bh THETA] [--max-steps K] [--device D]", 2);
     steps_run_params P;
     if (steps_read_params(o.param.c_str(), o.topology, &P)) return die(steps_host_last_error());
     if (P.IC_FORMAT == 2) return die("IC_FORMAT 2 (HDF5) needs the HDF5 library, which this build does not link");
@@ -223,6 +230,8 @@ int main(int argc, char **argv) {
     d.interp_order = o.interp_order;
     d.precision = o.precision;
     d.device = o.device;
+    d.bh_theta = o.bh_theta;             // USE_BH (main.cc:302-303)
+    d.bh_particle_radius = P.ParticleRadi;
     std::vector<double> t3, s1r2, radial;
     if (o.topology == STEPS_TOPO_T3 && P.IS_PERIODIC >= 2) {
         int32_t ng = 0;

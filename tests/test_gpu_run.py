@@ -74,3 +74,46 @@ def test_steps_run_r3_comoving_matches_oracle(steps, oracle, tmp_path):
     vs = v.reshape(-1, 3) * math.sqrt(st.a) * UNIT_V
     assert np.allclose(snap[:, 3:6], vs, rtol=1e-8, atol=1e-10 * np.abs(vs).max())
     assert np.allclose(snap[:, 6], w.M, rtol=1e-15)
+
+
+def _write_run(tmp_path, w, a0, a_max, acc, tag):
+    ic = tmp_path / f"ic_{tag}.dat"
+    with open(ic, "w") as f:
+        for i in range(w.N):
+            row = list(w.x[i]) + list(w.v[i] * math.sqrt(a0) * UNIT_V) + [w.M[i]]
+            f.write("\t".join("%.17g" % c for c in row) + "\n")
+    out = tmp_path / f"out_{tag}"
+    out.mkdir()
+    par = tmp_path / f"run_{tag}.param"
+    par.write_text("\n".join([
+        "Omega_b\t0.0", f"Omega_lambda\t{wl.OMEGA_L!r}", f"Omega_m\t{wl.OMEGA_M!r}", "Omega_r\t0.0",
+        f"HubbleConstant\t{wl.H0 * UNIT_V!r}", f"a_start\t{a0!r}", f"a_max\t{a_max!r}",
+        "COSMOLOGY\t1", "IS_PERIODIC\t0", "COMOVING_INTEGRATION\t1", "L_BOX\t100.0", "R_SIM\t1000.0",
+        f"IC_FILE\t{ic}", "IC_FORMAT\t0", f"OUT_DIR\t{out}", "OUTPUT_TIME_VARIABLE\t0", "OUTPUT_FORMAT\t0",
+        f"ACC_PARAM\t{acc!r}", "STEP_MIN\t0.00002", "STEP_MAX\t0.03125", f"PARTICLE_RADII\t{w.particle_radius!r}",
+        "FIRST_T_OUT\t13.0", "H_OUT\t13.0", ""]))
+    return par, out
+
+
+def test_steps_run_barnes_hut(tmp_path):
+    """--bh THETA (the reference's USE_BH=THETA build): with an opening angle
+    too small to take any internal node the octree run is the direct-sum run
+    (same step lengths, same final snapshot to rounding); at theta = 0.5 it
+    runs the same schedule with forces within the octree's error."""
+    print("case steps_run R3 comoving N=2048, direct vs octree", flush=True)
+    w = wl.r3_compactified(N=2048, seed=23)
+    exe = ROOT / "steps_amd" / "lib" / "steps_run"
+    snaps = {}
+    for tag, extra in (("direct", []), ("bh_tiny", ["--bh", "1e-9"]), ("bh_half", ["--bh", "0.5"])):
+        par, out = _write_run(tmp_path, w, 1.0 / 32, 0.0318, 0.0125, tag)
+        r = subprocess.run([str(exe), str(par), "--max-steps", "30", *extra], capture_output=True, text=True,
+                           timeout=120)
+        print(tag, r.stdout[-300:], r.stderr[-300:], flush=True)
+        assert r.returncode == 0
+        snaps[tag] = (np.loadtxt(out / "Logfile.dat"), np.loadtxt(out / "t1300.dat"))
+    (log_d, s_d), (log_t, s_t), (log_h, s_h) = snaps["direct"], snaps["bh_tiny"], snaps["bh_half"]
+    assert 2 <= log_d.shape[0] < 30
+    assert log_t.shape == log_d.shape and np.allclose(log_t[:, 5], log_d[:, 5], rtol=1e-9)
+    assert np.allclose(s_t[:, :3], s_d[:, :3], rtol=0, atol=1e-9 * np.abs(s_d[:, :3]).max())
+    assert np.allclose(s_t[:, 3:6], s_d[:, 3:6], rtol=1e-6, atol=1e-9 * np.abs(s_d[:, 3:6]).max())
+    assert log_h.shape[0] >= 2 and np.allclose(s_h[:, :3], s_d[:, :3], rtol=0, atol=1e-4 * np.abs(s_d[:, :3]).max())
