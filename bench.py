@@ -375,7 +375,7 @@ def c0_kdk_leg(nsteps: int = 10) -> dict:
     return out
 
 
-def bh_leg(w, kw, F64, direct_ms: float, reps: int) -> dict:
+def bh_leg(w, kw, F64, direct_ms, reps: int, cpu: bool = True, periodic: bool = True) -> dict:
     """Barnes-Hut octree forces (the reference's USE_BH engine, forces.cc:
     378-506, which the reference runs on the CPU only) on the headline
     workload: octree build plus traversal per evaluation (HIP events), the
@@ -392,11 +392,14 @@ def bh_leg(w, kw, F64, direct_ms: float, reps: int) -> dict:
         _, ms, _ = time_path(sim, reps, 1)
         F = sim.get_forces()
         sim.close()
-        err = np.linalg.norm(F - F64, axis=1) / np.linalg.norm(F64, axis=1)
-        out[f"theta_{theta}"] = {"eval_ms": ms, "evaluations_per_s": 1e3 / ms,
-                                 "speedup_vs_direct_fp64": direct_ms / ms,
-                                 "rel_err_vs_direct_p50": float(np.median(err)),
-                                 "rel_err_vs_direct_p99": float(np.percentile(err, 99))}
+        out[f"theta_{theta}"] = {"eval_ms": ms, "evaluations_per_s": 1e3 / ms}
+        if F64 is not None:
+            err = np.linalg.norm(F - F64, axis=1) / np.linalg.norm(F64, axis=1)
+            out[f"theta_{theta}"].update({"speedup_vs_direct_fp64": direct_ms / ms,
+                                          "rel_err_vs_direct_p50": float(np.median(err)),
+                                          "rel_err_vs_direct_p99": float(np.percentile(err, 99))})
+    if not cpu:
+        return out
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib  # the oracle: timed here as the CPU baseline only
 
@@ -423,7 +426,8 @@ def bh_leg(w, kw, F64, direct_ms: float, reps: int) -> dict:
                                      f"({t_build:.2f} s), rows 0..{rows - 1} traversed on {threads} OpenMP threads "
                                      f"({t_rows - t_build:.2f} s), scaled to {N} rows"}
     out["theta_0.5"]["speedup_vs_cpu_port"] = t_eval * 1e3 / out["theta_0.5"]["eval_ms"]
-    out["periodic"] = bh_periodic(reps)
+    if periodic:
+        out["periodic"] = bh_periodic(reps)
     print(f"[bench] barnes-hut: {out}", file=sys.stderr, flush=True)
     return out
 
@@ -491,7 +495,7 @@ def main() -> int:
                     help="skip the host-buffer entry-point leg (profiling runs: its N = 16384 launches share kernel names)")
     ap.add_argument("--serving-requests", type=int, default=160,
                     help="length of the seeded Poisson request stream of the serving leg")
-    ap.add_argument("--only", choices=["variants", "serving"],
+    ap.add_argument("--only", choices=["variants", "serving", "barnes-hut"],
                     help="run only that leg and print its JSON (profiling runs); skips the headline line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
@@ -509,6 +513,12 @@ def main() -> int:
         return 0
     if args.only == "serving":
         print(json.dumps({"serving": serving_leg(args.serving_requests)}), flush=True)
+        return 0
+    if args.only == "barnes-hut":  # the octree kernels alone (profiling runs): no direct sum, no CPU leg
+        w = wl.r3_compactified(**WORKLOADS[args.workload])
+        kw = dict(cosmology=w.cosmology, comoving=w.comoving, mass_in_unit_sphere=w.mass_in_unit_sphere)
+        print(json.dumps({"barnes_hut": bh_leg(w, kw, None, None, max(1, args.steps), cpu=False,
+                                               periodic=False)}), flush=True)
         return 0
 
     gen = WORKLOADS[args.workload]

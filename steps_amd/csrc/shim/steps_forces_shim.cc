@@ -20,7 +20,10 @@
 // once per step.  Not kept: per-call device allocation and copies of the
 // tables (buffers and tables stay resident), and the n_GPU row split (one
 // device per process: STEPS_DEVICE, or rank modulo the device count).
-// STEPS_FORCE_PRECISION=mixed selects the mixed-precision R^3 path.
+// STEPS_FORCE_PRECISION=mixed selects the mixed-precision R^3 path.  Built
+// with -DUSE_BH (the reference's Barnes-Hut build, which links forces.o) the
+// same symbols evaluate the octree with the opening angle THETA and the
+// node softening of ParticleRadi, globals main.cc:61-64 defines.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -51,6 +54,9 @@ extern REAL beta;
 extern REAL ParticleRadi;
 extern REAL rho_part;
 extern REAL M_min;
+#ifdef USE_BH
+extern REAL THETA;  // main.cc:64, set from USE_BH at main.cc:303
+#endif
 #if defined(PERIODIC)
 extern REAL *T3_EWALD_FORCE_TABLE;
 extern int N_EWALD_FORCE_GRID;
@@ -97,6 +103,10 @@ steps_force_desc descriptor(int topology) {
     d.interp_order = EWALD_INTERPOLATION_ORDER;
     d.precision = pick_precision();
     d.device = pick_device();
+#ifdef USE_BH
+    d.bh_theta = THETA;
+    d.bh_particle_radius = ParticleRadi;
+#endif
 #if defined(PERIODIC)
     d.t3_table = T3_EWALD_FORCE_TABLE;
     d.t3_ngrid = N_EWALD_FORCE_GRID;

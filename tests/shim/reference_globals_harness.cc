@@ -21,6 +21,9 @@ REAL beta = 0;
 REAL ParticleRadi = 0;
 REAL rho_part = 0;
 REAL M_min = 0;
+#ifdef USE_BH
+REAL THETA = 0;
+#endif
 #if defined(PERIODIC)
 REAL *T3_EWALD_FORCE_TABLE = nullptr;
 int N_EWALD_FORCE_GRID = 0;
@@ -58,6 +61,12 @@ void harness_set_tables(REAL *t3, int ng, REAL *s1r2, int nrho, int nz, REAL *ra
     (void)t3; (void)ng; (void)s1r2; (void)nrho; (void)nz; (void)radial; (void)nradial;
 }
 void harness_set_softening(double particle_radi, double m_min) { ParticleRadi = particle_radi; M_min = m_min; }
+void harness_set_theta(double theta) {
+#ifdef USE_BH
+    THETA = theta;
+#endif
+    (void)theta;
+}
 int harness_variant() {
 #if defined(PERIODIC)
     return 1;

@@ -128,3 +128,42 @@ def test_shim_t3_and_s1r2(steps, oracle):
     Fo = oracle.forces(oracle.params(2, **kw), c.x, c.M, c.soft)
     report("shim S1R2", rel_err(F, Fo))
     assert np.median(rel_err(F, Fo)) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_shim_barnes_hut_builds(steps, oracle):
+    """-DUSE_BH: the reference's forces() / forces_periodic() with the global
+    THETA evaluate the octree; bitwise the C ABI's octree evaluation."""
+    w = wl.r3_compactified(N=4096, seed=78)
+    lib = shim("r3_bh")
+    lib.harness_set_theta.argtypes = [C.c_double]
+    M, h, x = w.M.copy(), w.soft.copy(), w.x.reshape(-1).copy()
+    lib.harness_set_particles(w.N, p(M), p(h))
+    lib.harness_set_scalars(0, 1, 1, 0.0, 0.0, w.mass_in_unit_sphere, wl.H0, wl.OMEGA_L)
+    lib.harness_set_softening(w.particle_radius, float(w.M.min()))
+    lib.harness_set_theta(0.5)
+    F = np.zeros(3 * w.N)
+    assert lib.harness_call(p(x), p(F), 0, w.N - 1) == 0
+    d, keep = steps.make_desc(steps.TOPO_R3, cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere,
+                              H0=wl.H0, Omega_lambda=wl.OMEGA_L, bh_theta=0.5, bh_particle_radius=w.particle_radius)
+    assert np.array_equal(F.reshape(-1, 3), steps.force_eval(d, w.x, w.M, w.soft))
+    Fo = oracle.forces_bh(oracle.params(0, cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere),
+                          0.5, w.particle_radius, w.x, w.M, w.soft)
+    assert rel_err(F, Fo).max() <= 1e-12
+    # T3 (-DPERIODIC -DUSE_BH): the CPU build's trilinear table interpolation
+    t = wl.t3_lattice(n_side=10, seed=3)
+    T = oracle.t3_table(2, t.L)
+    lib = shim("t3_bh")
+    lib.harness_set_theta.argtypes = [C.c_double]
+    M, h, x = t.M.copy(), t.soft.copy(), t.x.reshape(-1).copy()
+    lib.harness_set_particles(t.N, p(M), p(h))
+    lib.harness_set_scalars(2, 1, 1, t.L, 0.0, 0.0, 0.0, 0.0)
+    lib.harness_set_tables(p(T), 63, None, 0, 0, None, 0)
+    lib.harness_set_softening(t.particle_radius, float(t.M.min()))
+    lib.harness_set_theta(0.5)
+    F = np.zeros(3 * t.N)
+    assert lib.harness_call(p(x), p(F), 0, t.N - 1) == 0
+    Fo = oracle.forces_bh(oracle.params(1, is_periodic=2, L=t.L, t3_table=T, interp_order=2), 0.5, t.particle_radius,
+                          t.x, t.M, t.soft)
+    report("shim T3 octree", rel_err(F, Fo))
+    assert rel_err(F, Fo).max() <= 1e-12
