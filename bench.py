@@ -234,6 +234,12 @@ def serving_leg(nreq: int) -> dict:
     print(f"[bench] serving: sustainable {rate:.1f} req/s {sus}", file=sys.stderr, flush=True)
     sus["offered_req_per_s"] = rate
     sus["arrivals_s"] = sus_reqs[-1].t_arrival
+    # how much any schedule could gain: the requests' kernel time alone,
+    # against the sequential wall clock
+    dev = sv.device_time_alone(sims, reqs)
+    dev["sequential_wall_ms"] = seq["wall_s"] * 1e3
+    dev["speedup_bound_if_kernels_do_not_overlap"] = seq["wall_s"] * 1e3 / dev["kernel_ms_sum"]
+    print(f"[bench] serving: kernel time alone {dev}", file=sys.stderr, flush=True)
     return {"workload": "64 sims R3/S1R2, N log-uniform [8192, 524288], Poisson 200 req/s seed 1004, "
                         "KDK steps, FP64 / mixed 50/50",
             "requests": nreq, "arrivals_s": reqs[-1].t_arrival, "pair_interactions": pairs,
@@ -242,7 +248,8 @@ def serving_leg(nreq: int) -> dict:
             "speedup_vs_sequential": srv["requests_per_s"] / seq["requests_per_s"],
             "results_identical": bool(same),
             "latency_vs_stream_length": by_len,
-            "sustainable_load": sus}
+            "sustainable_load": sus,
+            "device_time": dev}
 
 
 def variants_leg(reps: int) -> dict:
@@ -371,6 +378,15 @@ def c0_kdk_leg(nsteps: int = 10) -> dict:
         sim.close()
         out[name] = {"force_ms": t_force * 1e3, "ms_per_step": dt * 1e3 / nsteps, "steps_per_s": nsteps / dt,
                      "g_pairs_per_s": float(w.N) ** 2 * nsteps / dt / 1e9}
+    # the mixed path's error against FP64 on the pair sum (external term off)
+    F = {}
+    for prec in (steps_amd.PREC_FP64, steps_amd.PREC_MIXED):
+        d, keep = steps_amd.make_desc(steps_amd.TOPO_R3, precision=prec)
+        F[prec] = steps_amd.force_eval(d, w.x, w.M, w.soft)
+    e = (np.linalg.norm(F[steps_amd.PREC_MIXED] - F[steps_amd.PREC_FP64], axis=1) /
+         np.linalg.norm(F[steps_amd.PREC_FP64], axis=1))
+    out["mixed"].update({"rel_err_pairsum_vs_fp64_p50": float(np.median(e)),
+                         "rel_err_pairsum_vs_fp64_p99": float(np.percentile(e, 99))})
     print(f"[bench] c0 KDK: {out}", file=sys.stderr, flush=True)
     return out
 

@@ -299,10 +299,17 @@ int sim_create(const steps_force_desc &d, int64_t N, bool both_precisions, steps
     *out = nullptr;
     if (N <= 0) return fail(STEPS_ERR_ARG, "N must be positive");
     EvalPlan p;
-    steps_force_desc dd = d;
-    if (both_precisions && desc_mixed_capable(d)) dd.precision = STEPS_PREC_MIXED;
-    int rc = make_plan(dd, N, 0, N - 1, p);
+    int rc = make_plan(d, N, 0, N - 1, p);
     if (rc) return rc;
+    size_t scratch = scratch_bytes(p);
+    if (both_precisions && desc_mixed_capable(d)) {
+        // the scratch serves either precision (neither plan's need bounds the other's)
+        steps_force_desc dd = d;
+        dd.precision = (d.precision == STEPS_PREC_MIXED) ? STEPS_PREC_FP64 : STEPS_PREC_MIXED;
+        EvalPlan q;
+        if ((rc = make_plan(dd, N, 0, N - 1, q))) return rc;
+        scratch = std::max(scratch, scratch_bytes(q));
+    }
     if ((rc = set_device(d.device))) return rc;
     steps_sim *s = new (std::nothrow) steps_sim();
     if (!s) return fail(STEPS_ERR_NOMEM, "host allocation");
@@ -317,7 +324,7 @@ int sim_create(const steps_force_desc &d, int64_t N, bool both_precisions, steps
     };
     if ((rc = s->jrec.reserve((size_t)N * sizeof(double4), "alloc jrec")) ||
         (rc = s->jsoft.reserve(nb, "alloc soft")) || (rc = s->v.reserve(3 * nb, "alloc v")) ||
-        (rc = s->F.reserve(3 * nb, "alloc F")) || (rc = s->part.reserve(scratch_bytes(p), "alloc partials")) ||
+        (rc = s->F.reserve(3 * nb, "alloc F")) || (rc = s->part.reserve(scratch, "alloc partials")) ||
         (rc = s->stage.reserve(std::max(3 * nb, (size_t)1024), "alloc stage")) ||
         (rc = s->blockmax.reserve((size_t)(kdk_reduce_blocks(N) + 1) * sizeof(double), "alloc blockmax")) ||
         (rc = s->dparams.reserve(sizeof(KdkParams), "alloc step params")))

@@ -2083,15 +2083,17 @@ bool is_mixed(const EvalPlan &p) { return p.precision == STEPS_PREC_MIXED && mix
 
 // The mixed kernels read spatially ordered sources (Morton order; their FP32
 // offsets need compact tiles) except the S1R2 TSC one, whose table window
-// needs tiles compact in xy (input order), the R3 and nearest-image FP64 kernels from
-// N = 65536 on (their far-tile test gains 7% for R3 at N = 2^21; below, the
-// ~16 launches of the ordering cost more than the pair kernel saves), the T3
-// Ewald kernel sources in z pencils.
+// needs tiles compact in xy (input order), and the R3 one below N = 65536; the R3 and
+// nearest-image FP64 kernels from N = 65536 on (their far-tile test gains 7%
+// for R3 at N = 2^21); the T3 Ewald kernel sources in z pencils.  Below
+// N = 65536 the ~16 launches of the ordering cost more than the pair kernels
+// save: the mixed R3 force at N = 16384 took 0.32-0.34 ms with it against
+// 0.24-0.25 ms for FP64.
 constexpr int64_t kR3SortMinN = 65536;
 bool sorted_sources(const EvalPlan &p) {
     const bool far_tiles = p.variant == PV_R3 || p.variant == PV_T3_NEAREST || p.variant == PV_S1R2_NEAREST;
-    return (is_mixed(p) && p.variant != PV_S1R2_TSC) || (far_tiles && p.N >= kR3SortMinN) ||
-           p.variant == PV_T3_EWALD;
+    const bool mixed_sorted = is_mixed(p) && p.variant != PV_S1R2_TSC && !(p.variant == PV_R3 && p.N < kR3SortMinN);
+    return mixed_sorted || (far_tiles && p.N >= kR3SortMinN) || p.variant == PV_T3_EWALD;
 }
 bool pencil_order(const EvalPlan &p) { return p.variant == PV_T3_EWALD; }
 // The mixed S1R2 CIC / TSC kernels take their FP32 source offsets relative to

@@ -82,14 +82,20 @@ struct Slot {
 size_t footprint(const steps_force_desc &d, int64_t N) {
     EvalPlan p;
     steps_force_desc dd = d;
-    if (desc_mixed_capable(d)) dd.precision = STEPS_PREC_MIXED;
     // tables only have to look present for the size computation
     static const double dummy = 0;
     if (!dd.t3_table) dd.t3_table = &dummy;
     if (!dd.s1r2_table) dd.s1r2_table = &dummy;
     if (!dd.radial_table) dd.radial_table = &dummy;
     size_t b = (size_t)N * (32 + 8 + 24 + 24 + 24) + 4096;
-    if (make_plan(dd, N, 0, N - 1, p) == STEPS_OK) b += scratch_bytes(p);
+    // the scratch of either precision (sim_create sizes it for both)
+    size_t scratch = 0;
+    for (int prec : {STEPS_PREC_FP64, STEPS_PREC_MIXED}) {
+        if (prec == STEPS_PREC_MIXED && !desc_mixed_capable(d)) continue;
+        dd.precision = prec;
+        if (make_plan(dd, N, 0, N - 1, p) == STEPS_OK) scratch = std::max(scratch, scratch_bytes(p));
+    }
+    b += scratch;
     b += (size_t)(kdk_reduce_blocks(N) + 1) * 8;
     if (d.topology == STEPS_TOPO_T3 && d.is_periodic >= 2 && d.t3_ngrid > 0)
         b += t3_device_entries(d.t3_ngrid, 0) * sizeof(double);  // unfolded layout: the larger one

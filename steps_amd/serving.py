@@ -131,6 +131,31 @@ def run_server(sims, reqs, streams: int = 4, use_graphs: bool = True, budget: in
     return out, finals
 
 
+def device_time_alone(sims, reqs) -> dict:
+    """Kernel time of the stream's requests one at a time: each (simulation,
+    precision) force evaluation timed once with HIP events on its own stream
+    (a step's kick/drift kernels add a few microseconds), summed over the
+    requests.  The large simulations' kernels fill the device, so no schedule
+    serves the stream in much less than this."""
+    need = sorted({(r.sim, r.precision) for r in reqs})
+    ms = {}
+    for si in sorted({k[0] for k in need}):
+        w, d, _ = sims[si]
+        s = abi.Sim(d, w.N)
+        s.set_state(w.x, w.M, w.soft, w.v)
+        for (sj, prec) in need:
+            if sj != si:
+                continue
+            s.set_precision(prec)
+            s.forces()
+            s.synchronize()
+            s.forces()
+            ms[(si, prec)] = s.last_kernel_ms()[1]
+        s.close()
+    total = sum(ms[(r.sim, r.precision)] for r in reqs)
+    return {"kernel_ms_sum": total, "kernel_ms_by_largest_sims": sorted(ms.values())[-8:]}
+
+
 def run_sequential(sims, reqs):
     """The same stream served one request at a time: resident simulations,
     one launch sequence per request, no graphs, no concurrency."""
