@@ -268,6 +268,12 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
     splits = std::max<int64_t>(1, (N + jchunk - 1) / jchunk);
     p.splits = (int)splits;
     p.jchunk = jchunk;
+    if (t3_slab_plan(p)) {
+        // one split per XCD; the packed (server) launches, which keep the
+        // generic kernel, split the sources into eight equal chunks instead
+        p.jchunk = ((N + 7) / 8 + kTile - 1) / kTile * kTile;
+        p.splits = 8;
+    }
     return STEPS_OK;
 }
 

@@ -88,6 +88,10 @@ struct EvalPlan {
     int bh_order = 2;
 };
 
+// The T3 Ewald evaluation deals its sources out to the eight XCDs by table
+// slab (pair_kernels.hip, body_t3_slab): make_plan then sets eight splits.
+bool t3_slab_plan(const EvalPlan &p);
+
 // Scratch of one evaluation: partial sums [splits][3][nrows] (FP64), followed
 // (256-byte aligned) by the source-tile table and, for STEPS_PREC_MIXED, the
 // FP32 tile-recentred source records and an FP32 copy of the S1R2 table.

@@ -1581,6 +1581,80 @@ __device__ __forceinline__ void body_t3_win(const PairArgs &a, const int bx, con
 }
 
 // ---------------------------------------------------------------------------
+// T3 Ewald pair kernel for tables larger than an XCD's L2 (the folded 127^3
+// table of IS_PERIODIC = 3 is 12.8 MB against 4 MB of L2 per XCD), with rows
+// and sources in z pencils.  The source columns (table cell columns in x)
+// are dealt out to the eight XCDs by the table x nodes their pairs read: for
+// the row block's first x column c0, XCD k takes the source columns whose
+// displacement column (c - c0 + Ng/2) mod Ng lies in [k Ng/8, (k+1) Ng/8), so
+// the pairs an XCD evaluates read one slab of the table (~20 x nodes of 127:
+// ~4 MB), which its L2 can hold, instead of all of it.  Workgroups are
+// dispatched to the XCDs round robin, so workgroup id = 8 * row block + k
+// runs on XCD k; its sums are split k of the row block's partial sums (eight
+// splits, reduced in fixed order).  Every source column falls in exactly one
+// slab, so each pair is evaluated once; the pair arithmetic is pair_f64's.
+// ---------------------------------------------------------------------------
+constexpr int kXcds = 8;
+
+// first index of the sorted keys at or above v (scalar binary search)
+__device__ __forceinline__ int64_t keys_lower_bound(const uint32_t *__restrict__ keys, int64_t n, uint32_t v) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void body_t3_slab(const PairArgs &a, const int id) {
+    const int bx = id / kXcds, slab = id % kXcds;
+    const double4 *__restrict__ irec = a.irec;
+    const double *__restrict__ isoft = a.isoft;
+    const double4 *__restrict__ jrec = a.jrec;
+    const double *__restrict__ soft = a.soft;
+    const double4 *__restrict__ tiles = a.tiles;
+    const double *__restrict__ table = a.table;
+    const uint32_t *__restrict__ keys = a.keys;
+    KScalars k = a.k;
+    k.c15 = in_vgpr(a.k.c15);
+    const int tid = threadIdx.x;
+    const int64_t row_base = (int64_t)bx * kBlock;
+    // rows are the sorted particles (id_min = 0, nrows = N)
+    const int64_t r = min(row_base + tid, k.nrows - 1);
+    const double4 p = irec[r];
+    const double xi = p.x, yi = p.y, zi = p.z, hi = isoft[r];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int Ng = k.ng;
+    const int c0 = (int)(keys[row_base] >> 20);  // x column of the block's first row
+    const int lo_k = slab * Ng / kXcds, hi_k = (slab + 1) * Ng / kXcds;
+    const int cs = imodp(c0 - Ng / 2 + lo_k, Ng), ncol = hi_k - lo_k;
+    for (int part = 0; part < 2; ++part) {
+        // source columns [ca, cb): the slab's columns, in two runs where they wrap
+        const int ca = part == 0 ? cs : 0;
+        const int cb = part == 0 ? min(Ng, cs + ncol) : cs + ncol - Ng;
+        if (cb <= ca) continue;
+        const int64_t j0 = keys_lower_bound(keys, k.N, (uint32_t)ca << 20);
+        const int64_t j1 = (cb >= Ng) ? k.N : keys_lower_bound(keys, k.N, (uint32_t)cb << 20);
+        for (int64_t t0 = (j0 / kTile) * kTile; t0 < j1; t0 += kTile) {
+            const double b = hi + tiles[t0 / kTile].w;
+            const double B = b * b;
+            const int u0 = (int)(max(j0, t0) - t0), u1 = (int)(min(j1, t0 + kTile) - t0);
+            const double4 *__restrict__ tj = jrec + t0;
+            for (int u = u0; u < u1; ++u)
+                pair_f64<PV_T3_EWALD>(k, table, soft, t0 + u, tj[u], xi, yi, zi, hi, B, fx, fy, fz);
+        }
+    }
+    const int64_t rr = row_base + tid;
+    if (rr < k.nrows) {
+        double *out = a.part + (size_t)slab * 3u * (size_t)k.nrows;
+        out[rr] = fx;
+        out[k.nrows + rr] = fy;
+        out[2 * k.nrows + rr] = fz;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Mixed-precision T3 Ewald pair kernel.  Each displacement is formed and
 // wrapped in FP64 (forces_cuda.cu:620-627) and rounded to FP32 once: the
 // sources are in z pencils, whose 64-source tiles span the box in z, so
@@ -1832,12 +1906,14 @@ __device__ __forceinline__ void body_t3_mixed(const PairArgs &a, const int bx, c
     }
 }
 
-enum BodyKind : int { BK_F64 = 0, BK_MIXED_R3 = 1, BK_MIXED_Z = 2, BK_T3_WIN = 3, BK_T3_MIXED = 4, BK_T3_WIN_MIXED = 5 };
+enum BodyKind : int { BK_F64 = 0, BK_MIXED_R3 = 1, BK_MIXED_Z = 2, BK_T3_WIN = 3, BK_T3_MIXED = 4, BK_T3_WIN_MIXED = 5,
+                      BK_T3_SLAB = 6 };
 
 template <int BK, int V, int NI, int UNR>
 __device__ __forceinline__ void run_body(const PairArgs &a, int bx, int by) {
     if (BK == BK_F64) body_f64<V, NI, UNR>(a, bx, by);
     else if (BK == BK_T3_WIN) body_t3_win(a, bx, by);
+    else if (BK == BK_T3_SLAB) body_t3_slab(a, bx);
     else if (BK == BK_T3_MIXED) body_t3_mixed<false>(a, bx, by);
     else if (BK == BK_T3_WIN_MIXED) body_t3_mixed<true>(a, bx, by);
     else if (BK == BK_MIXED_R3) body_mixed_r3<NI, UNR>(a, bx, by);
@@ -2039,7 +2115,7 @@ constexpr int unroll_of(int v, int precision) {
 template <int BK, int V>
 hipError_t launch_pair_t(const PairArgs *a, const PackSeg *segs, int nseg, unsigned gx, unsigned gy,
                          hipStream_t st) {
-    constexpr int prec = (BK == BK_F64 || BK == BK_T3_WIN) ? STEPS_PREC_FP64 : STEPS_PREC_MIXED;
+    constexpr int prec = (BK == BK_F64 || BK == BK_T3_WIN || BK == BK_T3_SLAB) ? STEPS_PREC_FP64 : STEPS_PREC_MIXED;
     constexpr int NI = rows_per_lane(V, prec);
     constexpr int UNR = unroll_of(V, prec);
     if (segs)
@@ -2120,9 +2196,12 @@ bool sorted_rows(const EvalPlan &p) {
     return (pencil_order(p) || nearest) && sorted_sources(p) && p.id_min == 0 && p.nrows == p.N;
 }
 // The LDS-window T3 kernel needs rows in pencils and room for a 5 x 5 window
-// of full z runs (Ng <= 119); packed launches keep the global kernel.
+// of full z runs (Ng <= 127); packed launches keep the global kernel.  On
+// finer tables a workgroup's 256 rows span many more table columns, its
+// windows rarely fit and it ran slower than the global kernel (127^3 table
+// on the 64^3 lattice: 5.67 against 5.87 G pairs/s), so it serves Ng <= 100.
 bool t3_window(const EvalPlan &p) {
-    return p.variant == PV_T3_EWALD && sorted_rows(p) && !is_mixed(p) &&
+    return p.variant == PV_T3_EWALD && sorted_rows(p) && !is_mixed(p) && p.t3_ng <= 100 &&
            (int64_t)t3_win_stride(3u * (unsigned)(p.t3_ng + 3) + 2u) * 25 <= kT3WinDoubles;
 }
 // The same for the mixed T3 kernel (16-byte nodes; Ng <= 127).
@@ -2130,6 +2209,18 @@ bool t3_window_mixed(const EvalPlan &p) {
     return p.variant == PV_T3_EWALD && sorted_rows(p) && is_mixed(p) &&
            (int64_t)(p.t3_ng + 3) * 25 <= kT3WinNodes32;
 }
+
+// The XCD-slab T3 kernel (body_t3_slab) takes the tables the window does not
+// serve, up to Ng = 160 (a slab of the folded table: Ng/8 + 4 x nodes of
+// (Ng - Ng/2) y nodes x (Ng + 3) z nodes x 24 bytes, 4.4 MB at Ng = 127), for
+// full evaluations of at least 4096 particles (eight non-empty splits).
+}  // namespace
+bool t3_slab_plan(const EvalPlan &p) {
+    return p.variant == PV_T3_EWALD && p.precision == STEPS_PREC_FP64 && p.t3_ng > 100 && p.t3_ng <= 160 &&
+           p.id_min == 0 && p.nrows == p.N && p.N >= 4096;
+}
+namespace {
+bool t3_slab(const EvalPlan &p) { return t3_slab_plan(p) && sorted_rows(p) && p.splits == kXcds; }
 
 size_t s1r2_f32_entries(const EvalPlan &p) {
     return (is_mixed(p) && is_s1r2_lookup(p.variant)) ? s1r2_device_nodes(p.s1r2_nrho, p.s1r2_nz) : 0;
@@ -2380,6 +2471,7 @@ hipError_t launch_forces_pairs(const EvalPlan &p, const DeviceParticles &dp, dou
     if (!has_pairs(p)) return hipSuccess;
     const PairArgs a = make_args(p, dp, scratch_view(p, partials));
     if (t3_window(p)) return launch_pair_t<BK_T3_WIN, PV_T3_EWALD>(&a, nullptr, 0, row_blocks(p), (unsigned)p.splits, st);
+    if (t3_slab(p)) return launch_pair_t<BK_T3_SLAB, PV_T3_EWALD>(&a, nullptr, 0, row_blocks(p) * kXcds, 1, st);
     if (t3_window_mixed(p))
         return launch_pair_t<BK_T3_WIN_MIXED, PV_T3_EWALD>(&a, nullptr, 0, row_blocks(p), (unsigned)p.splits, st);
     return launch_pair(p.variant, is_mixed(p) ? STEPS_PREC_MIXED : STEPS_PREC_FP64, &a, nullptr, 0, row_blocks(p),
