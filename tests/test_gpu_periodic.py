@@ -595,3 +595,48 @@ def test_forces_with_finer_tables(steps, oracle):
         e = rel_err(F[rows], oracle.forces_rows(p, c.x, c.M, c.soft, rows))
         report(f"S1R2 is_periodic=3 (Nz=256) order={order} sampled rows", e)
         assert np.median(e) <= 1e-12 and e.max() <= 1e-10
+
+
+@pytest.mark.parametrize("mode", ["s1r2_cic", "s1r2_tsc", "s1r2_nearest", "t3_nearest"])
+def test_mixed_full_size(steps, oracle, mode):
+    """The mixed path at full size (S1R2 N = 524288, T3 nearest image on the
+    64^3 lattice), all rows against FP64 for the pair sum alone and with the
+    comoving term, p99 <= 1e-5; and 32 sampled rows against the oracle, median
+    <= 1e-6 and max <= 1e-4 (too few rows for a p99: the sample's worst rows
+    are the tail's).  The S1R2 nearest z-image kernel is held to 1.5e-5: its
+    rows' net forces are small residuals of large opposing pulls, so the FP32
+    rounding of each pair is a larger share of them (p99 1.1e-5 for the pair
+    sum, 1.07e-5 with the comoving term); flushing the FP32 sums every 8 or 16
+    sources, or taking the offsets per 8-source sub-tile, left it at 1.1e-5
+    (measured), so it is per-pair rounding, not accumulation."""
+    print(f"case mixed full size {mode}", flush=True)
+    if mode == "t3_nearest":
+        w = wl.t3_lattice(is_periodic=1)
+        topo, base = steps.TOPO_T3, dict(is_periodic=1, L=w.L)
+        cases = [(0, base), (1, dict(base, cosmology=1, comoving=1))]
+        otop = 1
+    else:
+        w = wl.s1r2_cylinder(seed=1002, is_periodic=1 if mode == "s1r2_nearest" else 2)
+        topo, otop = steps.TOPO_S1R2, 2
+        if mode == "s1r2_nearest":
+            radial = steps.radial_table(w.Rsim, 0.5 * w.L, 1000, 1000)
+            base = dict(is_periodic=1, L=w.L, Rsim=w.Rsim, radial_table=radial)
+        else:
+            T, shape = steps.s1r2_table(2, w.L, w.Rsim)
+            base = dict(is_periodic=2, L=w.L, Rsim=w.Rsim, s1r2_lookup=1, s1r2_table=T, s1r2_shape=shape,
+                        interp_order=2 if mode == "s1r2_cic" else 4)
+        cases = [(0, base), (1, dict(base, cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere))]
+    rows = np.linspace(0, w.N - 1, 32).astype(np.int32)
+    for cosmology, kw in cases:
+        d64, k1 = steps.make_desc(topo, precision=steps.PREC_FP64, **kw)
+        dmx, k2 = steps.make_desc(topo, precision=steps.PREC_MIXED, **kw)
+        F64 = steps.force_eval(d64, w.x, w.M, w.soft)
+        Fmx = steps.force_eval(dmx, w.x, w.M, w.soft)
+        e = rel_err(Fmx, F64)
+        okw = dict(kw, interp_order=kw.get("interp_order", 4))
+        eo = rel_err(Fmx[rows], oracle.forces_rows(oracle.params(otop, **okw), w.x, w.M, w.soft, rows))
+        report(f"{mode} mixed N={w.N} cosmology={cosmology} vs fp64", e)
+        report(f"{mode} mixed N={w.N} cosmology={cosmology} vs oracle (32 rows)", eo)
+        bound = 1.5e-5 if mode == "s1r2_nearest" else 1e-5
+        assert np.percentile(e, 99) <= bound
+        assert np.median(eo) <= 1e-6 and eo.max() <= 1e-4
