@@ -54,6 +54,7 @@ struct KScalars {
     // FP32 copies for the mixed-precision S1R2 pair
     float Lf, halfLf, rho_maxf, inv_drhof, inv_dzgf, last_rf;
     float inv_gsf;  // FP32 copy of inv_gs for the mixed-precision T3 pair
+    float wrap_epsf;  // 1e-6 L: the band around |d| = L/2 where an FP32 displacement may wrap otherwise
 };
 
 // Everything one evaluation's pair kernel reads: rows (caller order),
@@ -1028,12 +1029,20 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
 
 // FP32 displacement with the nearest z-image (forces_cuda.cu:833-836) and r^2;
 // T3 nearest image: the nearest image on every axis (forces_cuda.cu:622-627).
+// bnd: a wrapped axis lies within 1e-6 L of L/2, where the FP32 displacement
+// (off by a few ulp of L) may take the other image than the FP64 one: such a
+// pair is left to the FP64 redo with the close pairs, so every pair takes the
+// reference's image.  (On the S1R2 N = 524288 cylinder about 4% of the rows
+// had one pair in that band, whose flipped image set the p99 error.)
 template <int V>
 __device__ __forceinline__ float mixed_r2_z(const KScalars &k, const float4 pj, float xr, float yr, float zr,
-                                            float &dx, float &dy, float &dz) {
+                                            float &dx, float &dy, float &dz, bool &bnd) {
     dx = pj.x - xr;
     dy = pj.y - yr;
     dz = pj.z - zr;
+    bnd = fabsf(fabsf(dz) - k.halfLf) < k.wrap_epsf;
+    if (V == PV_T3_NEAREST)
+        bnd = bnd || fabsf(fabsf(dx) - k.halfLf) < k.wrap_epsf || fabsf(fabsf(dy) - k.halfLf) < k.wrap_epsf;
     if (V == PV_T3_NEAREST) {
         if (dx > k.halfLf) dx -= k.Lf;
         else if (dx < -k.halfLf) dx += k.Lf;
@@ -1050,8 +1059,9 @@ __device__ __forceinline__ void mixed_pair_z(const KScalars &k, const float2 *__
                                              float xr, float yr, float zr, float B, bool &near, float &ax,
                                              float &ay, float &az, const S1R2Win32 sw = S1R2Win32{}) {
     float dx, dy, dz;
-    const float r2 = mixed_r2_z<V>(k, pj, xr, yr, zr, dx, dy, dz);
-    const bool nr = r2 < B;  // left to the FP64 redo (r2 may be 0 here)
+    bool bnd;
+    const float r2 = mixed_r2_z<V>(k, pj, xr, yr, zr, dx, dy, dz, bnd);
+    const bool nr = r2 < B || bnd;  // left to the FP64 redo (r2 may be 0 here)
     near = near || nr;
     const float y = __builtin_amdgcn_rsqf(r2);
     const float y3 = nr ? 0.0f : (y * y) * y;
@@ -1306,7 +1316,8 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
 #pragma unroll
                         for (int q = 0; q < NI; ++q) {
                             float dx, dy, dz;
-                            if (mixed_r2_z<V>(k, pr, xs[q], ys[q], zs[q], dx, dy, dz) < B[q]) {
+                            bool bnd;
+                            if (mixed_r2_z<V>(k, pr, xs[q], ys[q], zs[q], dx, dy, dz, bnd) < B[q] || bnd) {
                                 const double b = hi[q] + c.w;
                                 const int64_t jg = t0 + s0 + jj;
                                 pair_f64<V>(k, table, soft, jg, jrec[jg], xi[q], yi[q], zi[q], hi[q], b * b, fx[q],
@@ -1348,7 +1359,8 @@ __device__ __forceinline__ void body_mixed_z(const PairArgs &a, const int bx, co
 #pragma unroll
                 for (int q = 0; q < NI; ++q) {
                     float dx, dy, dz;
-                    if (mixed_r2_z<V>(k, pr, xr[q], yr[q], zr[q], dx, dy, dz) < B[q]) {
+                    bool bnd;
+                    if (mixed_r2_z<V>(k, pr, xr[q], yr[q], zr[q], dx, dy, dz, bnd) < B[q] || bnd) {
                         const double b = hi[q] + c.w;
                         pair_f64<V>(k, table, soft, jj, jrec[jj], xi[q], yi[q], zi[q], hi[q], b * b, fx[q],
                                     fy[q], fz[q]);
@@ -2290,6 +2302,7 @@ KScalars make_scalars(const EvalPlan &p) {
     k.last_r = (double)std::max(1, p.s1r2_nrho - 1);
     k.Lf = (float)p.L;
     k.halfLf = (float)(0.5 * p.L);
+    k.wrap_epsf = (float)(1e-6 * p.L);
     k.rho_maxf = (float)k.rho_max;
     k.inv_drhof = (float)k.inv_drho;
     k.inv_dzgf = (float)k.inv_dzg;
