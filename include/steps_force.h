@@ -86,6 +86,28 @@ typedef struct steps_force_desc {
                                     mode of forces.cc:373-1217; FP64 only); 0: direct sum */
     double bh_particle_radius;   /* ParticleRadi: a node's softening length is
                                     cbrt(mass / M_min) * ParticleRadi + h_i (forces.cc:386) */
+    int32_t bh_randomize;        /* nonzero: the randomised octree of the RANDOMIZE_BH build
+                                    (forces.cc:420-434, 662-685, 1026-1058, 1194-1211): the
+                                    octree is built in a randomly moved frame and the forces
+                                    are turned back.  R3: rotation by angle u3*pi about the
+                                    unit vector of (theta = u4*2pi, z = u5*2-1), root cube of
+                                    side 4.00004 max|x| centred at (u0..u2 - 0.5)*2 max|x|.
+                                    T3: shift by (u0..u2 - 0.5)*L, wrapped into [0, L).
+                                    S1R2: rotation about z by u0*2pi, root centred at
+                                    ((u1, u2) - 0.5)*2 max r_xy, z shifted by (u3 - 0.5)*L
+                                    and wrapped, root side L unless 4 max r_xy >= L (then
+                                    4 max r_xy).  Positions are not modified.              */
+    double bh_random[6];         /* u0..u5: the draws rand()/RAND_MAX of one evaluation, in
+                                    the order the reference draws them (R3 six, T3 three,
+                                    S1R2 four); steps_sim_set_bh_random changes them      */
+    const double *radial_bh_table; /* RADIAL_BH_FORCE_TABLE [size] (host memory) or NULL:
+                                    with the octree, R3 or S1R2, comoving cosmology, each
+                                    row's force loses x * c(r) / r, c linearly interpolated
+                                    between bin centres (forces.cc:189-267, 479-485,
+                                    1143-1147, 1180-1184; r = |x| in R3, the cylindrical
+                                    radius and x, y only in S1R2); see
+                                    steps_radial_bh_table_accumulate                       */
+    int32_t radial_bh_table_size;  /* RADIAL_BH_FORCE_TABLE_SIZE, at least 2               */
 } steps_force_desc;
 
 /* Fill *d with the defaults (R3, FP64, no cosmology, device 0). */
@@ -133,6 +155,10 @@ int steps_sim_destroy(steps_sim *s);
  * whose variant has a mixed path is sized for both, so switching allocates
  * nothing. */
 int steps_sim_set_precision(steps_sim *s, int32_t precision);
+/* The draws of the randomised octree for later evaluations (u[6], the
+ * descriptor's bh_random): the reference draws them afresh in every force
+ * routine call (forces.cc:422-430, 662-667, 1026-1034). */
+int steps_sim_set_bh_random(steps_sim *s, const double *u);
 /* x, v AoS [3N] (v may be NULL: zero), M and soft [N]; host memory. */
 int steps_sim_set_state(steps_sim *s, const double *x, const double *v, const double *M,
                         const double *soft);
@@ -182,6 +208,16 @@ int steps_s1r2_table(int32_t is_periodic, double L, double Rsim, double *table,
                      int32_t *nrho, int32_t *nz);
 /* finite-cylinder radial force table, utils.cc:162-228 */
 int steps_radial_table(double R, double Lz, int32_t size, int32_t accuracy, double *table);
+/* The radial octree correction table (main.cc:950-1088, utils.cc:96-156):
+ * one call per octree force evaluation of the glass adds each particle's
+ * radial force (F . x / |x| in R3; the x-y parts over the cylindrical
+ * radius in S1R2) to sum[b] and one to count[b], b = floor(r / (Rsim /
+ * size)) capped at size - 1; host memory, N >= 0 rows of x and F.  After the
+ * last call steps_radial_bh_table_finish turns the sums into the table:
+ * sum[b] / count[b], or 0 for an empty bin. */
+int steps_radial_bh_table_accumulate(int32_t topology, double Rsim, int32_t size, int64_t N, const double *x,
+                                     const double *F, double *sum, int32_t *count);
+int steps_radial_bh_table_finish(int32_t size, double *sum, const int32_t *count);
 
 /* ---- serving loop ----------------------------------------------------------
  * No counterpart in the reference (one simulation per MPI job there).  A

@@ -37,7 +37,12 @@ extern "C" {
 /* Every parameter read_param() reads, with its global's name.  Unread keys
  * keep the reference's start values: 0 (globals are zero-initialised),
  * RADIAL_FORCE_ACCURACY = RADIAL_FORCE_TABLE_SIZE = 1000 for the S1xR2 build
- * (read_paramfile.cc:116-117).  After the file is read, as there: with
+ * (read_paramfile.cc:116-117); the octree build's keys (USE_BH,
+ * read_paramfile.cc:130-141, 348-379) start at RADIAL_BH_FORCE_CORRECTION
+ * 0, GLASS_FILE_FOR_BH_FORCE_CORRECTION "None", RADIAL_BH_FORCE_TABLE_SIZE
+ * 64, RADIAL_BH_FORCE_TABLE_ITERATION 4 and are clamped as read (a
+ * correction flag other than 0 or 1 reads as 0; at least 5 bins and one
+ * iteration).  After the file is read, as there: with
  * COSMOLOGY = 1, STEP_MIN/STEP_MAX are converted from Gy to internal units,
  * and with H0_INDEPENDENT_UNITS = 1, L_BOX and PARTICLE_RADII from Mpc/h to
  * Mpc; with COSMOLOGY != 1, H0_INDEPENDENT_UNITS is forced to 0. */
@@ -69,6 +74,10 @@ typedef struct steps_run_params {
     double Rsim;                    /* R_SIM */
     int32_t RADIAL_FORCE_ACCURACY;  /* S1xR2 build only; at least 10 */
     int32_t RADIAL_FORCE_TABLE_SIZE;/* S1xR2 build only; at least 5 */
+    int32_t RADIAL_BH_FORCE_CORRECTION;
+    char GLASS_FILE_FOR_BH_FORCE_CORRECTION[STEPS_PATH_MAX];
+    int32_t RADIAL_BH_FORCE_TABLE_SIZE;
+    int32_t RADIAL_BH_FORCE_TABLE_ITERATION;
 } steps_run_params;
 
 const char *steps_host_last_error(void);
@@ -100,6 +109,12 @@ int steps_read_gadget_ic(const char *path, int64_t *N, double *x, double *v, dou
  * OUTPUT_TIME_VARIABLE; values are returned in file order.  With out == NULL
  * only the count is returned (>= 0). */
 int64_t steps_read_output_list(const char *path, double *out, int64_t cap);
+
+/* save_function_to_ascii_table (inputoutput.cc:1038-1061): the header line,
+ * then "%.6f\t%.15f" rows of (x_min + k deltax, values[k]) while the
+ * argument does not exceed x_max. */
+int steps_write_ascii_table(const char *path, double x_min, double x_max, double deltax, const double *values,
+                            int32_t n, const char *header);
 
 /* write_ascii_snapshot: x and M multiplied by h0_dimless (1, or H0/100 with
  * H0-independent units), velocities by sqrt(a) * UNIT_V (km/s sqrt(a));

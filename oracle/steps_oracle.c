@@ -1188,6 +1188,7 @@ static void orc_bh_force(orc_bh_ctx *c, int node, int i, double *f)
 }
 
 static long long orc_bh_counts[2];
+static double orc_bh_root[4];
 
 void oracle_bh_last_counts(long long *visited, long long *accepted)
 {
@@ -1195,10 +1196,119 @@ void oracle_bh_last_counts(long long *visited, long long *accepted)
     *accepted = orc_bh_counts[1];
 }
 
+void oracle_bh_last_root(double out[4])
+{
+    for (int k = 0; k < 4; ++k) out[k] = orc_bh_root[k];
+}
+
+/* utils.cc:24-37 */
+static double orc_lin(double X, double X1, double Y1, double X2, double Y2)
+{
+    double A = (Y2 - Y1) / (X2 - X1);
+    double B = Y1 - A * X1;
+    return A * X + B;
+}
+
+/* forces.cc:182-187 */
+static double orc_bin_center(double R, int b, int n) { return (((double)b) + 0.5) * (R / (double)n); }
+
+/* forces.cc:189-245, ORDER 1 */
+double oracle_radial_bh_correction(double r, double R, const double *t, int n)
+{
+    if (r < 0) return 0;
+    if (r > R) return orc_lin(r, orc_bin_center(R, n - 2, n), t[n - 2], orc_bin_center(R, n - 1, n), t[n - 1]);
+    int b1 = (int)floor(r / (R / (double)n));
+    if (b1 > n - 1) b1 = n - 1; /* r within rounding of R */
+    double c1 = orc_bin_center(R, b1, n);
+    if (c1 <= r) {
+        int b2;
+        if (b1 < n - 1) {
+            b2 = b1 + 1;
+        } else {
+            b1 -= 1;
+            c1 = orc_bin_center(R, b1, n);
+            b2 = b1 + 1;
+        }
+        return orc_lin(r, c1, t[b1], orc_bin_center(R, b2, n), t[b2]);
+    }
+    if (b1 > 1) return orc_lin(r, orc_bin_center(R, b1 - 1, n), t[b1 - 1], c1, t[b1]);
+    return orc_lin(r, 0.0, 0.0, c1, t[b1]);
+}
+
+/* utils.cc:96-156 */
+void oracle_radial_bh_table_accumulate(int topology, double Rsim, int size, int N, const double *x,
+                                       const double *F, double *table, int *count)
+{
+    double *loc = (double *)calloc((size_t)size, sizeof(double));
+    int *cnt = (int *)calloc((size_t)size, sizeof(int));
+    if (!loc || !cnt) {
+        free(loc);
+        free(cnt);
+        return;
+    }
+    for (int i = 0; i < N; i++) {
+        double r, Fr;
+        if (topology != ORC_S1R2) {
+            r = sqrt(pow(x[3 * i], 2) + pow(x[3 * i + 1], 2) + pow(x[3 * i + 2], 2));
+            Fr = (F[3 * i] * x[3 * i] + F[3 * i + 1] * x[3 * i + 1] + F[3 * i + 2] * x[3 * i + 2]) / r;
+        } else {
+            r = sqrt(pow(x[3 * i], 2) + pow(x[3 * i + 1], 2));
+            Fr = (F[3 * i] * x[3 * i] + F[3 * i + 1] * x[3 * i + 1]) / r;
+        }
+        int b = (int)floor(r / (Rsim / (double)size));
+        if (b >= size) b = size - 1;
+        loc[b] += Fr;
+        cnt[b]++;
+    }
+    for (int b = 0; b < size; b++) {
+        table[b] += loc[b];
+        count[b] += cnt[b];
+    }
+    free(loc);
+    free(cnt);
+}
+
+/* forces.cc:324-356 (Rodrigues' formula about the unit vector a) */
+static void orc_rotate3(double *v, const double *a, double c, double s)
+{
+    double X[3] = {v[0], v[1], v[2]};
+    double cr[3] = {a[1] * X[2] - a[2] * X[1], a[2] * X[0] - a[0] * X[2], a[0] * X[1] - a[1] * X[0]};
+    double dot = a[0] * X[0] + a[1] * X[1] + a[2] * X[2];
+    for (int k = 0; k < 3; ++k) v[k] = X[k] * c + cr[k] * s + a[k] * dot * (1.0 - c);
+}
+
+/* forces.cc:884-899 */
+static void orc_rotate2(double *v, double c, double s)
+{
+    double r0 = v[0] * c - v[1] * s, r1 = v[0] * s + v[1] * c;
+    v[0] = r0;
+    v[1] = r1;
+}
+
+/* forces.cc:670-682 */
+static double orc_shift_wrap(double v, double d, double L)
+{
+    v += d;
+    if (v < 0) v = v + L;
+    else if (v >= L) v = v - L;
+    return v;
+}
+
 int oracle_forces_bh(const oracle_params *p, double theta, double particle_radius, int N, const double *x,
                      const double *M, const double *h, double *F, int id_min, int id_max)
 {
+    return oracle_forces_bh_ex(p, theta, particle_radius, NULL, N, x, M, h, F, id_min, id_max);
+}
+
+int oracle_forces_bh_ex(const oracle_params *p, double theta, double particle_radius, const oracle_bh_opts *o,
+                        int N, const double *x_in, const double *M, const double *h, double *F, int id_min,
+                        int id_max)
+{
     if (!p || N <= 0 || id_min < 0 || id_max >= N || id_max < id_min - 1) return -1;
+    const int rnd = o && o->randomize;
+    const double *rbh = (o && o->radial_bh_table && p->topology != ORC_T3 && p->cosmology == 1 && p->comoving == 1)
+                            ? o->radial_bh_table
+                            : NULL;
     int mode;
     if (p->topology == ORC_R3) mode = 0;
     else if (p->topology == ORC_T3) mode = p->is_periodic >= 2 ? 2 : 1;
@@ -1209,14 +1319,48 @@ int oracle_forces_bh(const oracle_params *p, double theta, double particle_radiu
     if ((mode == 3 || mode == 5) && p->cosmology == 1 && p->comoving == 1 && !p->radial_table) return -3;
     double rmax = 0, M_min = M[0];
     for (int i = 0; i < N; ++i) {
-        double r = (p->topology == ORC_S1R2) ? sqrt(pow(x[3 * i], 2) + pow(x[3 * i + 1], 2)) /* forces.cc:1018 */
-                                             : sqrt(pow(x[3 * i], 2) + pow(x[3 * i + 1], 2) + pow(x[3 * i + 2], 2));
+        double r = (p->topology == ORC_S1R2) ? sqrt(pow(x_in[3 * i], 2) + pow(x_in[3 * i + 1], 2)) /* forces.cc:1018 */
+                                             : sqrt(pow(x_in[3 * i], 2) + pow(x_in[3 * i + 1], 2) + pow(x_in[3 * i + 2], 2));
         if (r > rmax) rmax = r;
         if (M_min > M[i]) M_min = M[i]; /* utils.cc:62-67 */
     }
-    /* the root cube: forces.cc:437-438 (R3), :690 (T3), :1047-1058 (S1R2) */
+    /* the positions in the octree's frame (a copy: the reference moves x in
+     * place and back) */
+    double *x = (double *)malloc(sizeof(double) * 3 * (size_t)N);
+    if (!x) return -1;
+    memcpy(x, x_in, sizeof(double) * 3 * (size_t)N);
+    double axis[3] = {0, 0, 0}, angle = 0;
     double c0[3] = {0, 0, 0}, side;
-    if (p->topology == ORC_R3) {
+    if (rnd && p->topology == ORC_R3) { /* forces.cc:420-434, 358-369 */
+        for (int k = 0; k < 3; k++) c0[k] = (o->u[k] - 0.5) * 2.0 * rmax;
+        side = 4.00004 * rmax;
+        angle = o->u[3] * ORC_PI;
+        double th = o->u[4] * 2.0 * M_PI, z = o->u[5] * 2.0 - 1.0, r = sqrt(1.0 - z * z);
+        axis[0] = r * cos(th);
+        axis[1] = r * sin(th);
+        axis[2] = z;
+        double c = cos(angle), s = sin(angle);
+        for (int i = 0; i < N; i++) orc_rotate3(x + 3 * i, axis, c, s);
+    } else if (rnd && p->topology == ORC_T3) { /* forces.cc:662-685 */
+        double sh[3];
+        for (int k = 0; k < 3; k++) sh[k] = (o->u[k] - 0.5) * p->L;
+        for (int i = 0; i < N; i++)
+            for (int k = 0; k < 3; k++) x[3 * i + k] = orc_shift_wrap(x[3 * i + k], sh[k], p->L);
+        side = p->L;
+        c0[0] = c0[1] = c0[2] = 0.50 * p->L;
+    } else if (rnd) { /* forces.cc:1026-1073 */
+        angle = o->u[0] * 2.0 * ORC_PI;
+        c0[0] = (o->u[1] - 0.5) * 2 * rmax;
+        c0[1] = (o->u[2] - 0.5) * 2 * rmax;
+        double shz = (o->u[3] - 0.5) * p->L;
+        double c = cos(angle), s = sin(angle);
+        for (int i = 0; i < N; i++) {
+            orc_rotate2(x + 3 * i, c, s);
+            x[3 * i + 2] = orc_shift_wrap(x[3 * i + 2], shz, p->L);
+        }
+        side = (4 * rmax < p->L) ? p->L : 4.0 * rmax;
+        c0[2] = 0.50 * side;
+    } else if (p->topology == ORC_R3) { /* the root cube: forces.cc:437-438 (R3), :690 (T3), :1047-1058 (S1R2) */
         side = 2.00002 * rmax;
     } else if (p->topology == ORC_T3) {
         side = p->L;
@@ -1225,13 +1369,19 @@ int oracle_forces_bh(const oracle_params *p, double theta, double particle_radiu
         side = (2 * rmax < p->L) ? p->L : 2.0 * rmax;
         c0[2] = 0.50 * side;
     }
+    orc_bh_root[0] = c0[0];
+    orc_bh_root[1] = c0[1];
+    orc_bh_root[2] = c0[2];
+    orc_bh_root[3] = side;
     orc_bh_tree t = {NULL, 0, 0};
     int rc = orc_bh_new(&t, c0[0], c0[1], c0[2], side) < 0 ? -1 : 0;
     for (int i = 0; i < N && !rc; ++i) rc = orc_bh_insert(&t, 0, x, M, i, 0);
     if (rc) {
         free(t.n);
+        free(x);
         return rc;
     }
+    const double cb = cos(-angle), sb = sin(-angle);
     const double DE = p->H0 * p->H0 * p->Omega_lambda;
     const int ewald_max = p->is_periodic + 1;          /* main.cc:1270 */
     const double cut = ((double)ewald_max - 0.4) * p->L; /* main.cc:1271, forces.cc:1100 */
@@ -1253,26 +1403,39 @@ int oracle_forces_bh(const oracle_params *p, double theta, double particle_radiu
         }
         visited += c.cnt[0];
         accepted += c.cnt[1];
+        const double *xi = x + 3 * kk;
         if (p->topology == ORC_R3 && p->cosmology == 1) { /* forces.cc:474-491 */
             double k = p->comoving == 1 ? p->mass_in_unit_sphere : DE;
-            for (int a = 0; a < 3; ++a) f[a] += k * x[3 * kk + a];
+            for (int a = 0; a < 3; ++a) f[a] += k * xi[a];
+            if (rbh) { /* forces.cc:479-485 */
+                double r = sqrt(pow(xi[0], 2) + pow(xi[1], 2) + pow(xi[2], 2));
+                double c = oracle_radial_bh_correction(r, p->Rsim, rbh, o->radial_bh_table_size);
+                for (int a = 0; a < 3; ++a) f[a] -= xi[a] * c / r;
+            }
         } else if (p->topology == ORC_S1R2 && p->cosmology == 1) { /* forces.cc:1127-1150, 1170-1190 */
             double k = DE;
+            double rxy = sqrt(pow(xi[0], 2) + pow(xi[1], 2));
             if (p->comoving == 1) {
                 k = p->mass_in_unit_sphere;
-                if (mode != 4) {
-                    double rxy = sqrt(pow(x[3 * kk], 2) + pow(x[3 * kk + 1], 2));
+                if (mode != 4)
                     k *= oracle_cylindrical_force_correction_order(rxy, p->Rsim, p->radial_table,
                                                                    p->radial_table_size, 1);
-                }
             }
-            f[0] += k * x[3 * kk];
-            f[1] += k * x[3 * kk + 1];
+            f[0] += k * xi[0];
+            f[1] += k * xi[1];
+            if (rbh) { /* forces.cc:1143-1147, 1180-1184 */
+                double c = oracle_radial_bh_correction(rxy, p->Rsim, rbh, o->radial_bh_table_size);
+                f[0] -= xi[0] * c / rxy;
+                f[1] -= xi[1] * c / rxy;
+            }
         }
+        if (rnd && p->topology == ORC_R3) orc_rotate3(f, axis, cb, sb); /* forces.cc:502-503 */
+        else if (rnd && p->topology == ORC_S1R2) orc_rotate2(f, cb, sb); /* forces.cc:1210 */
         for (int a = 0; a < 3; ++a) F[3 * (kk - id_min) + a] = f[a];
     }
     orc_bh_counts[0] = visited;
     orc_bh_counts[1] = accepted;
     free(t.n);
+    free(x);
     return 0;
 }

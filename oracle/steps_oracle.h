@@ -35,6 +35,16 @@
  *                          against the FP64 oracle within float bounds and
  *                          growing with N (tests/test_oracle_r3.py); it is a
  *                          comparison point, not a parity oracle
+ *   forces_bh(_ex)         tiny opening angle = direct sum in every topology
+ *                          and mode, with and without the random frame; a far
+ *                          cluster taken at its centre of mass; error falls
+ *                          with theta; identity draws = plain octree bit for
+ *                          bit; root cubes by the placement rules; averaging
+ *                          over draws shrinks the error (tests/test_oracle_bh*.py)
+ *   radial_bh correction   linear data between bin centres come back, the
+ *                          origin line below bin 1's centre; the table is the
+ *                          binned mean radial force; the corrected force is
+ *                          the plain one less x c(r) / r, and only comoving
  *   fast_erfcx             the published A&S 7.1.26 bound (1.5e-7 on erfc)
  *                          and its reflection for x < 0; its contribution to
  *                          the S1R2 table is bounded node by node (<= ~1e-10
@@ -190,6 +200,37 @@ int oracle_forces_bh(const oracle_params *p, double theta, double particle_radiu
 /* Nodes the last oracle_forces_bh call examined (massless and own-leaf nodes
  * excluded) and took, summed over its rows. */
 void oracle_bh_last_counts(long long *visited, long long *accepted);
+
+/* The octree options of other build flags.  randomize: the RANDOMIZE_BH
+ * build, with u[] the draws rand()/RAND_MAX of the call in the reference's
+ * order (forces.cc:422-430 R3: centre x, y, z, angle, axis theta, axis z;
+ * :662-667 T3: shift x, y, z; :1026-1034 S1R2: angle, centre x, y, z
+ * shift).  radial_bh_table: USE_RADIAL_BH_CORRECTION with
+ * RADIAL_BH_FORCE_TABLE (R3 and S1R2, comoving cosmology only). */
+typedef struct oracle_bh_opts {
+    int randomize;
+    double u[6];
+    const double *radial_bh_table;
+    int radial_bh_table_size;
+} oracle_bh_opts;
+/* oracle_forces_bh with those options (o may be NULL): the positions are
+ * moved into the random frame (R3 forces.cc:324-356, 420-434; T3 :662-685;
+ * S1R2 :884-899, 1026-1073) on a copy, the octree is built and walked
+ * there, the external term and the radial correction are added there, and
+ * the force is turned back (forces.cc:500-503, 1207-1210). */
+int oracle_forces_bh_ex(const oracle_params *p, double theta, double particle_radius, const oracle_bh_opts *o,
+                        int N, const double *x, const double *M, const double *h, double *F, int id_min,
+                        int id_max);
+/* The root cube of the last oracle_forces_bh(_ex) call: centre x, y, z, side. */
+void oracle_bh_last_root(double out[4]);
+/* get_BH_radial_force_correction with RADIAL_BH_FORCE_ORDER 1 (forces.cc:
+ * 182-245); r within rounding of R, whose bin index is size, is taken in the
+ * last bin (the reference reads one entry past the table there). */
+double oracle_radial_bh_correction(double r, double R, const double *table, int size);
+/* get_radial_bh_force_correction_table (utils.cc:96-156): adds one force
+ * evaluation's per-bin radial force sums to table[] and counts to count[]. */
+void oracle_radial_bh_table_accumulate(int topology, double Rsim, int size, int N, const double *x,
+                                       const double *F, double *table, int *count);
 
 #ifdef __cplusplus
 }

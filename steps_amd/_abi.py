@@ -29,10 +29,12 @@ PREC_FP64, PREC_MIXED = 0, 1
 # every symbol include/steps_force.h declares (checked by tests/test_abi.py)
 EXPORTED = [
     "steps_force_desc_init", "steps_device_count", "steps_last_error", "steps_version",
-    "steps_force_eval", "steps_force_eval_noalloc", "steps_sim_create", "steps_sim_destroy", "steps_sim_set_precision", "steps_sim_set_state",
+    "steps_force_eval", "steps_force_eval_noalloc", "steps_sim_create", "steps_sim_destroy", "steps_sim_set_precision",
+    "steps_sim_set_bh_random", "steps_sim_set_state",
     "steps_sim_set_positions", "steps_sim_get_state", "steps_sim_forces", "steps_sim_get_forces",
     "steps_sim_synchronize", "steps_sim_last_kernel_ms", "steps_sim_step", "steps_sim_errmax",
-    "steps_t3_table", "steps_s1r2_table", "steps_radial_table", "steps_softening_lengths",
+    "steps_t3_table", "steps_s1r2_table", "steps_radial_table", "steps_radial_bh_table_accumulate",
+    "steps_radial_bh_table_finish", "steps_softening_lengths",
     "steps_server_config_init", "steps_server_create", "steps_server_destroy", "steps_server_add_sim",
     "steps_server_remove_sim", "steps_server_sim_bytes", "steps_server_submit", "steps_server_wait", "steps_server_drain",
     "steps_server_get_state", "steps_server_get_stats", "steps_server_clock",
@@ -71,6 +73,10 @@ class ForceDesc(C.Structure):
         ("device", C.c_int32),
         ("bh_theta", C.c_double),
         ("bh_particle_radius", C.c_double),
+        ("bh_randomize", C.c_int32),
+        ("bh_random", C.c_double * 6),
+        ("radial_bh_table", C.POINTER(C.c_double)),
+        ("radial_bh_table_size", C.c_int32),
     ]
 
 
@@ -126,6 +132,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "steps_force_eval_noalloc": (C.c_int, [C.POINTER(ForceDesc), i64, _DP, _DP, _DP, _DP, i64, i64]),
         "steps_sim_create": (C.c_int, [C.POINTER(ForceDesc), i64, C.POINTER(vp)]),
         "steps_sim_set_precision": (C.c_int, [vp, C.c_int32]),
+        "steps_sim_set_bh_random": (C.c_int, [vp, _DP]),
         "steps_sim_destroy": (C.c_int, [vp]),
         "steps_sim_set_state": (C.c_int, [vp, _DP, _DP, _DP, _DP]),
         "steps_sim_set_positions": (C.c_int, [vp, _DP]),
@@ -139,6 +146,9 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "steps_t3_table": (C.c_int, [i32, C.c_double, _DP, C.POINTER(i32)]),
         "steps_s1r2_table": (C.c_int, [i32, C.c_double, C.c_double, _DP, C.POINTER(i32), C.POINTER(i32)]),
         "steps_radial_table": (C.c_int, [C.c_double, C.c_double, i32, i32, _DP]),
+        "steps_radial_bh_table_accumulate": (C.c_int, [i32, C.c_double, i32, i64, _DP, _DP, _DP,
+                                                       C.POINTER(C.c_int32)]),
+        "steps_radial_bh_table_finish": (C.c_int, [i32, _DP, C.POINTER(C.c_int32)]),
         "steps_softening_lengths": (C.c_int, [_DP, i64, C.c_double, _DP, _DP, _DP]),
         "steps_server_config_init": (None, [C.POINTER(ServerConfig)]),
         "steps_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(vp)]),
@@ -186,10 +196,13 @@ def device_count() -> int:
 def make_desc(topology=TOPO_R3, *, is_periodic=0, cosmology=0, comoving=0, L=0.0, Rsim=0.0,
               mass_in_unit_sphere=0.0, H0=0.0, Omega_lambda=0.0, s1r2_lookup=1, interp_order=2,
               t3_table=None, s1r2_table=None, s1r2_shape=(0, 0), radial_table=None,
-              precision=PREC_FP64, device=0, bh_theta=0.0, bh_particle_radius=0.0):
+              precision=PREC_FP64, device=0, bh_theta=0.0, bh_particle_radius=0.0, bh_random=None,
+              radial_bh_table=None):
     """Build a ForceDesc; returns (desc, keepalive) -- keep the second alive
     as long as the desc is used (it owns the table arrays).  bh_theta > 0
-    selects the Barnes-Hut octree (USE_BH) with that opening angle."""
+    selects the Barnes-Hut octree (USE_BH) with that opening angle;
+    bh_random (six draws in [0, 1]) randomises it (RANDOMIZE_BH) and
+    radial_bh_table adds its radial correction."""
     d = ForceDesc()
     load_library().steps_force_desc_init(C.byref(d))
     d.topology = topology
@@ -226,6 +239,15 @@ def make_desc(topology=TOPO_R3, *, is_periodic=0, cosmology=0, comoving=0, L=0.0
         keep.append(t)
         d.radial_table = _ptr(t)
         d.radial_table_size = t.size
+    if bh_random is not None:
+        d.bh_randomize = 1
+        for k, u in enumerate(list(bh_random) + [0.0] * (6 - len(bh_random))):
+            d.bh_random[k] = float(u)
+    if radial_bh_table is not None:
+        t = f64(radial_bh_table)
+        keep.append(t)
+        d.radial_bh_table = _ptr(t)
+        d.radial_bh_table_size = t.size
     return d, keep
 
 
@@ -271,6 +293,10 @@ class Sim:
         x, M, soft = f64(x).reshape(-1), f64(M), f64(soft)
         vv = None if v is None else f64(v).reshape(-1)
         _check(self._lib.steps_sim_set_state(self.h, _ptr(x), _ptr(vv), _ptr(M), _ptr(soft)))
+
+    def set_bh_random(self, u):
+        uu = f64(list(u) + [0.0] * (6 - len(u)))
+        _check(self._lib.steps_sim_set_bh_random(self.h, _ptr(uu)))
 
     def set_positions(self, x):
         x = f64(x).reshape(-1)
@@ -335,6 +361,20 @@ def s1r2_table(is_periodic: int, L: float, Rsim: float):
 def radial_table(R: float, Lz: float, size: int, accuracy: int) -> np.ndarray:
     T = np.empty(size)
     _check(load_library().steps_radial_table(R, Lz, size, accuracy, _ptr(T)))
+    return T
+
+
+def radial_bh_table(topology: int, Rsim: float, size: int, evaluations) -> np.ndarray:
+    """The radial octree correction table from (x, F) pairs of octree force
+    evaluations of a glass (steps_radial_bh_table_accumulate, _finish)."""
+    lib = load_library()
+    T = np.zeros(size)
+    n = np.zeros(size, dtype=np.int32)
+    for x, F in evaluations:
+        x, F = f64(x).reshape(-1), f64(F).reshape(-1)
+        _check(lib.steps_radial_bh_table_accumulate(topology, Rsim, size, x.size // 3, _ptr(x), _ptr(F), _ptr(T),
+                                                    n.ctypes.data_as(C.POINTER(C.c_int32))))
+    _check(lib.steps_radial_bh_table_finish(size, _ptr(T), n.ctypes.data_as(C.POINTER(C.c_int32))))
     return T
 
 

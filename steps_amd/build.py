@@ -26,7 +26,9 @@ LIBNAME = "libsteps_amd.so"
 # test harness that plays the role of StePS's globals (tests/shim)
 SHIM_VARIANTS = {"r3": [], "t3": ["-DPERIODIC"], "s1r2": ["-DPERIODIC_Z", "-DEWALD_INTERPOLATION_ORDER=4"],
                  "s1r2_nolookup": ["-DPERIODIC_Z", "-DPERIODIC_Z_NOLOOKUP"],
-                 "r3_bh": ["-DUSE_BH=0.5"], "t3_bh": ["-DPERIODIC", "-DUSE_BH=0.5"]}
+                 "r3_bh": ["-DUSE_BH=0.5"], "t3_bh": ["-DPERIODIC", "-DUSE_BH=0.5"],
+                 "r3_bh_rand": ["-DUSE_BH=0.5", "-DRANDOMIZE_BH=7"],
+                 "s1r2_bh_rand": ["-DPERIODIC_Z", "-DEWALD_INTERPOLATION_ORDER=2", "-DUSE_BH=0.5", "-DRANDOMIZE_BH=7"]}
 
 HIP_SOURCES = ["pair_kernels.hip", "engine.hip", "kdk_kernels.hip", "table_kernels.hip", "order_kernels.hip",
                "server.hip", "noalloc.hip", "bh_kernels.hip"]

@@ -1,6 +1,7 @@
 // bh_kernels.hip -- Barnes-Hut octree forces on the device: the reference's
-// USE_BH engine for the free (R3) boundary (forces.cc:97-187, 378-506), which
-// StePS runs on the CPU only (main.cc:249-252 refuses USE_BH with CUDA).
+// USE_BH engine (forces.cc:97-187, 373-506 R3, 582-772 T3, 881-1217 S1R2),
+// which StePS runs on the CPU only (main.cc:249-252 refuses USE_BH with
+// CUDA), with its RANDOMIZE_BH frames and radial correction table.
 //
 // The reference builds a pointer octree by inserting the particles one by
 // one into a root cube of side 2.00002 max|x_i| centred on the origin: a cube
@@ -37,6 +38,7 @@
 #include "internal.h"
 
 #include <algorithm>
+#include <cmath>
 
 namespace steps {
 namespace {
@@ -55,6 +57,67 @@ int table_rows(int64_t n) {
 }
 
 int64_t scan_blocks(int64_t n) { return std::max<int64_t>(1, (n + kScanItems - 1) / kScanItems); }
+
+// The frame the randomised octree (RANDOMIZE_BH) is built in, with the
+// reference's operations: R3 turns the positions by Rodrigues' formula about
+// a random axis (forces.cc:324-356, 422-434), T3 shifts them by a random
+// vector and wraps them into [0, L) (forces.cc:662-685), S1R2 turns them
+// about the z axis and shifts and wraps z (forces.cc:884-899, 1026-1073).
+// The root cube moves with it (bh_reduce_final) and each row's force is
+// turned back at the end (forces.cc:500-503, 1207-1210).
+struct BhFrame {
+    int kind = 0;                   // 0 none, 1 R3 rotation, 2 T3 shift, 3 S1R2 rotation and z shift
+    double c = 1.0, s = 0.0;        // cos, sin of the angle
+    double cb = 1.0, sb = 0.0;      // cos, sin of minus the angle: forces turned back
+    double ax = 0, ay = 0, az = 0;  // R3 axis
+    double sx = 0, sy = 0, sz = 0;  // T3 shift; S1R2 z shift in sz
+    double L = 0;
+    double cu0 = 0.5, cu1 = 0.5, cu2 = 0.5;  // draws placing the root centre (R3 x, y, z; S1R2 x, y)
+};
+
+// forces.cc:324-356: rotated = X cos + (Y x X) sin + Y (Y . X)(1 - cos)
+__device__ __forceinline__ void bh_rodrigues(double &x, double &y, double &z, double ax, double ay, double az,
+                                             double c, double s) {
+#pragma clang fp contract(off)
+    const double cx = ay * z - az * y, cy = az * x - ax * z, cz = ax * y - ay * x;
+    const double dot = ax * x + ay * y + az * z;
+    const double rx = x * c + cx * s + ax * dot * (1.0 - c);
+    const double ry = y * c + cy * s + ay * dot * (1.0 - c);
+    const double rz = z * c + cz * s + az * dot * (1.0 - c);
+    x = rx;
+    y = ry;
+    z = rz;
+}
+
+// forces.cc:884-899: the 2-D rotation in the x-y plane
+__device__ __forceinline__ void bh_rotate_xy(double &x, double &y, double c, double s) {
+#pragma clang fp contract(off)
+    const double rx = x * c - y * s, ry = x * s + y * c;
+    x = rx;
+    y = ry;
+}
+
+// forces.cc:670-682, 1061-1071: shift, then one periodic wrap
+__device__ __forceinline__ double bh_shift_wrap(double v, double d, double L) {
+    v += d;
+    if (v < 0) v = v + L;
+    else if (v >= L) v = v - L;
+    return v;
+}
+
+__device__ __forceinline__ double4 bh_to_frame(const BhFrame &fr, double4 p) {
+    if (fr.kind == 1) {
+        bh_rodrigues(p.x, p.y, p.z, fr.ax, fr.ay, fr.az, fr.c, fr.s);
+    } else if (fr.kind == 2) {
+        p.x = bh_shift_wrap(p.x, fr.sx, fr.L);
+        p.y = bh_shift_wrap(p.y, fr.sy, fr.L);
+        p.z = bh_shift_wrap(p.z, fr.sz, fr.L);
+    } else if (fr.kind == 3) {
+        bh_rotate_xy(p.x, p.y, fr.c, fr.s);
+        p.z = bh_shift_wrap(p.z, fr.sz, fr.L);
+    }
+    return p;
+}
 
 // max |x_i| (R3) or max sqrt(x_i^2 + y_i^2) (S1R2), and min m_i, in two steps
 // (per-block partials, then one block that also places the root cube)
@@ -92,9 +155,12 @@ __global__ __launch_bounds__(256) void bh_reduce_partial(const double4 *__restri
 // cube of each topology: R3 side 2.00002 max|x| at the origin (forces.cc:
 // 437-438); T3 the box, side L at L/2 (forces.cc:690); S1R2 side L, or twice
 // the largest cylindrical radius when that is not below L, centred on the
-// axis at z = side/2 (forces.cc:1047-1058).
+// axis at z = side/2 (forces.cc:1047-1058).  Randomised: R3 side 4.00004
+// max|x| centred at (u - 0.5) 2 max|x| per axis (forces.cc:424-428); S1R2
+// side L unless 4 max r_xy >= L (then 4 max r_xy), centred at
+// ((u1, u2) - 0.5) 2 max r_xy and z = side/2 (forces.cc:1030-1045, 1058).
 __global__ __launch_bounds__(256) void bh_reduce_final(const double *__restrict__ part, int nb, int topology,
-                                                       double L, double *__restrict__ out) {
+                                                       double L, BhFrame fr, double *__restrict__ out) {
     __shared__ double rr[256], mm[256];
     const int t = threadIdx.x;
     double r = 0.0, m = INFINITY;
@@ -114,14 +180,26 @@ __global__ __launch_bounds__(256) void bh_reduce_final(const double *__restrict_
     }
     if (t == 0) {
         double side, cx = 0.0, cy = 0.0, cz = 0.0;
+        const double R = rr[0];
         if (topology == STEPS_TOPO_T3) {
             side = L;
             cx = cy = cz = 0.50 * L;
         } else if (topology == STEPS_TOPO_S1R2) {
-            side = (2 * rr[0] < L) ? L : 2.0 * rr[0];
+            if (fr.kind == 3) {
+                side = (4 * R < L) ? L : 4.0 * R;
+                cx = (fr.cu0 - 0.5) * 2 * R;
+                cy = (fr.cu1 - 0.5) * 2 * R;
+            } else {
+                side = (2 * R < L) ? L : 2.0 * R;
+            }
             cz = 0.50 * side;
+        } else if (fr.kind == 1) {
+            side = 4.00004 * R;
+            cx = (fr.cu0 - 0.5) * 2.0 * R;
+            cy = (fr.cu1 - 0.5) * 2.0 * R;
+            cz = (fr.cu2 - 0.5) * 2.0 * R;
         } else {
-            side = 2.00002 * rr[0];
+            side = 2.00002 * R;
         }
         out[0] = side;
         out[1] = mm[0];
@@ -135,12 +213,12 @@ __global__ __launch_bounds__(256) void bh_reduce_final(const double *__restrict_
 // level the reference's test x > cx (bit 0), y > cy (bit 1), z > cz (bit 2)
 // against the cube centre, then the child's centre c +- side/4 and side/2
 // (forces.cc:121-128, 147-155), in the same floating-point operations.
-__global__ __launch_bounds__(256) void bh_keys_kernel(const double4 *__restrict__ rec, int64_t n,
+__global__ __launch_bounds__(256) void bh_keys_kernel(const double4 *__restrict__ rec, int64_t n, BhFrame fr,
                                                       const double *__restrict__ red, uint32_t *__restrict__ khi,
                                                       uint32_t *__restrict__ klo) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double4 p = rec[i];
+    const double4 p = bh_to_frame(fr, rec[i]);
     double cx = red[2], cy = red[3], cz = red[4], side = red[0];
     uint64_t key = 0;
     for (int l = 0; l < kLevels; ++l) {
@@ -156,7 +234,8 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(const double4 *__restrict_
     klo[i] = (uint32_t)key;
 }
 
-__global__ __launch_bounds__(256) void bh_gather_kernel(const double4 *__restrict__ rec,
+// sorted records, in the octree's frame
+__global__ __launch_bounds__(256) void bh_gather_kernel(const double4 *__restrict__ rec, BhFrame fr,
                                                         const double *__restrict__ soft,
                                                         const uint32_t *__restrict__ khi,
                                                         const uint32_t *__restrict__ klo,
@@ -166,7 +245,7 @@ __global__ __launch_bounds__(256) void bh_gather_kernel(const double4 *__restric
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = perm[i];
-    srec[i] = rec[s];
+    srec[i] = bh_to_frame(fr, rec[s]);
     ssoft[i] = soft[s];
     skey[i] = ((uint64_t)khi[s] << 32) | klo[s];
     sidx[i] = s;
@@ -388,6 +467,8 @@ struct BhTables {
     const double *radial;  // cylindrical radial table
     int radial_size;
     double Rsim;
+    const double *rbh;  // radial octree correction table, or null
+    int rbh_size;
 };
 
 __device__ __forceinline__ int bh_imodp(int i, int n) {
@@ -556,6 +637,41 @@ __device__ __forceinline__ double bh_cyl_correction(double r, double R, const do
     return c;
 }
 
+// utils.cc:24-37
+__device__ __forceinline__ double bh_lin(double X, double X1, double Y1, double X2, double Y2) {
+#pragma clang fp contract(off)
+    const double A = (Y2 - Y1) / (X2 - X1);
+    const double B = Y1 - A * X1;
+    return A * X + B;
+}
+
+// forces.cc:182-245 with RADIAL_BH_FORCE_ORDER 1 (forces.cc:24): the table's
+// values sit at the bin centres (b + 0.5) R / n and are interpolated
+// linearly: beyond R along the last two bins; at or above the centre of bin
+// b towards bin b + 1 (the last two bins in the last bin); below the centre
+// of bin b >= 2 towards bin b - 1; below the centre of bin 0 or 1 along the
+// line from the origin to that centre.  r within rounding of R, whose bin
+// index is n, is taken in the last bin (the reference reads entry n there);
+// a NaN radius takes no correction (and indexes nothing).
+__device__ __forceinline__ double bh_radial_bh(double r, double R, const double *t, int n) {
+#pragma clang fp contract(off)
+    if (!(r >= 0.0)) return 0.0;
+    const double w = R / (double)n;
+    if (r > R) return bh_lin(r, ((double)(n - 2) + 0.5) * w, t[n - 2], ((double)(n - 1) + 0.5) * w, t[n - 1]);
+    int b1 = (int)floor(r / w);
+    if (b1 > n - 1) b1 = n - 1;
+    double c1 = ((double)b1 + 0.5) * w;
+    if (c1 <= r) {
+        if (b1 >= n - 1) {
+            b1 -= 1;
+            c1 = ((double)b1 + 0.5) * w;
+        }
+        return bh_lin(r, c1, t[b1], ((double)(b1 + 1) + 0.5) * w, t[b1 + 1]);
+    }
+    if (b1 > 1) return bh_lin(r, ((double)(b1 - 1) + 0.5) * w, t[b1 - 1], c1, t[b1]);
+    return bh_lin(r, 0.0, 0.0, c1, t[b1]);
+}
+
 // One walk of the pre-order list for the lanes of a wave, each lane with its
 // own row (xi, yi, zi, hi, row index i) and decisions, accumulating into f.
 //
@@ -625,7 +741,10 @@ __device__ __forceinline__ void bh_walk(const BhNode *__restrict__ nodes, int to
 // One row per lane, rows in key order: the traversal of the row's topology
 // mode, then its external term (forces.cc:474-491 R3; none for T3;
 // forces.cc:1127-1150, 1170-1190 S1R2 in the x and y components only, times
-// the cylindrical correction unless the Ewald table is in use).  The S1R2
+// the cylindrical correction unless the Ewald table is in use), less the
+// radial octree correction x c(r) / r when its table is given (forces.cc:
+// 479-485, 1143-1147, 1180-1184), all in the octree's frame; a randomised
+// rotation is then undone (forces.cc:500-503, 1207-1210).  The S1R2
 // image sum walks the tree once per z image m = -(P+1) .. P+1 with the row
 // moved by m L and no wrap, and adds the walks in that order (forces.cc:
 // 1108-1118).
@@ -635,8 +754,9 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const double4 *__restrict
                                                        const uint32_t *__restrict__ sidx, int64_t n,
                                                        const int *__restrict__ info,
                                                        const BhNode *__restrict__ nodes, double theta, double L,
-                                                       int images, BhTables tb, int ext_mode, double ext_coeff,
-                                                       int64_t id_min, int64_t id_max, double *__restrict__ F) {
+                                                       int images, BhTables tb, BhFrame fr, int ext_mode,
+                                                       double ext_coeff, int64_t id_min, int64_t id_max,
+                                                       double *__restrict__ F) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t sc = min(s, n - 1);
     const int i = (int)sidx[sc];
@@ -668,6 +788,23 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const double4 *__restrict
         f[0] += k * me.x;
         f[1] += k * me.y;
     }
+    if (tb.rbh) {
+#pragma clang fp contract(off)
+        if (ext_mode == 1) {
+            const double r = sqrt(me.x * me.x + me.y * me.y + me.z * me.z);
+            const double c = bh_radial_bh(r, tb.Rsim, tb.rbh, tb.rbh_size);
+            f[0] -= me.x * c / r;
+            f[1] -= me.y * c / r;
+            f[2] -= me.z * c / r;
+        } else {
+            const double r = sqrt(me.x * me.x + me.y * me.y);
+            const double c = bh_radial_bh(r, tb.Rsim, tb.rbh, tb.rbh_size);
+            f[0] -= me.x * c / r;
+            f[1] -= me.y * c / r;
+        }
+    }
+    if (fr.kind == 1) bh_rodrigues(f[0], f[1], f[2], fr.ax, fr.ay, fr.az, fr.cb, fr.sb);
+    else if (fr.kind == 3) bh_rotate_xy(f[0], f[1], fr.cb, fr.sb);
     double *o = F + 3 * (i - id_min);
     o[0] = f[0];
     o[1] = f[1];
@@ -734,6 +871,52 @@ BhView bh_view(void *scratch, int64_t n) {
     return v;
 }
 
+// The randomised frame from the draws u0..u5, with the reference's host
+// arithmetic (forces.cc:358-369, 422-429, 664-667, 1027-1033): R3 angle
+// u3 pi about the unit vector of (theta = u4 2 pi, z = u5 2 - 1); T3 shift
+// (u - 0.5) L; S1R2 angle u0 2 pi and z shift (u3 - 0.5) L.  The forces are
+// turned back with cos and sin of minus the angle (forces.cc:502-503, 1210).
+BhFrame bh_frame(const EvalPlan &p, int topology) {
+    BhFrame fr;
+    if (!p.bh_randomize) return fr;
+    const double pi = 3.14159265358979323846264338327950288419716939937510;
+    const double *u = p.bh_u;
+    fr.L = p.L;
+    if (topology == STEPS_TOPO_R3) {
+        const double angle = u[3] * pi;
+        const double theta = u[4] * 2.0 * M_PI;
+        const double z = u[5] * 2.0 - 1.0;
+        const double r = std::sqrt(1.0 - z * z);
+        fr.kind = 1;
+        fr.ax = r * std::cos(theta);
+        fr.ay = r * std::sin(theta);
+        fr.az = z;
+        fr.c = std::cos(angle);
+        fr.s = std::sin(angle);
+        fr.cb = std::cos(-angle);
+        fr.sb = std::sin(-angle);
+        fr.cu0 = u[0];
+        fr.cu1 = u[1];
+        fr.cu2 = u[2];
+    } else if (topology == STEPS_TOPO_T3) {
+        fr.kind = 2;
+        fr.sx = (u[0] - 0.5) * p.L;
+        fr.sy = (u[1] - 0.5) * p.L;
+        fr.sz = (u[2] - 0.5) * p.L;
+    } else {
+        const double angle = u[0] * 2.0 * pi;
+        fr.kind = 3;
+        fr.c = std::cos(angle);
+        fr.s = std::sin(angle);
+        fr.cb = std::cos(-angle);
+        fr.sb = std::sin(-angle);
+        fr.cu0 = u[1];
+        fr.cu1 = u[2];
+        fr.sz = (u[3] - 0.5) * p.L;
+    }
+    return fr;
+}
+
 }  // namespace
 
 int64_t bh_item_capacity(int64_t n) {
@@ -761,13 +944,14 @@ hipError_t launch_bh_forces(const EvalPlan &p, const DeviceParticles &dp, void *
     const int topology = (p.variant == PV_R3)                                     ? STEPS_TOPO_R3
                          : (p.variant == PV_T3_NEAREST || p.variant == PV_T3_EWALD) ? STEPS_TOPO_T3
                                                                                     : STEPS_TOPO_S1R2;
+    const BhFrame fr = bh_frame(p, topology);
     hipLaunchKernelGGL(bh_reduce_partial, dim3(nb), dim3(256), 0, st, dp.jrec, n, topology == STEPS_TOPO_S1R2, v.red);
-    hipLaunchKernelGGL(bh_reduce_final, dim3(1), dim3(256), 0, st, v.red, nb, topology, p.L, red);
-    hipLaunchKernelGGL(bh_keys_kernel, dim3(ng), dim3(256), 0, st, dp.jrec, n, red, v.khi, v.klo);
+    hipLaunchKernelGGL(bh_reduce_final, dim3(1), dim3(256), 0, st, v.red, nb, topology, p.L, fr, red);
+    hipLaunchKernelGGL(bh_keys_kernel, dim3(ng), dim3(256), 0, st, dp.jrec, n, fr, red, v.khi, v.klo);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const uint32_t *perm = nullptr;
     if ((e = launch_sort_u64(v.khi, v.klo, n, v.sort, &perm, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(bh_gather_kernel, dim3(ng), dim3(256), 0, st, dp.jrec, dp.soft, v.khi, v.klo, perm, n, v.srec,
+    hipLaunchKernelGGL(bh_gather_kernel, dim3(ng), dim3(256), 0, st, dp.jrec, fr, dp.soft, v.khi, v.klo, perm, n, v.srec,
                        v.ssoft, v.skey, v.sidx);
     hipLaunchKernelGGL(bh_lcp_kernel, dim3(ng), dim3(256), 0, st, v.skey, n, v.tab);
     for (int k = 1; k < rows; ++k) hipLaunchKernelGGL(bh_table_kernel, dim3(ng), dim3(256), 0, st, v.tab, n, k);
@@ -787,11 +971,11 @@ hipError_t launch_bh_forces(const EvalPlan &p, const DeviceParticles &dp, void *
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_tree_end && (e = hipEventRecord(ev_tree_end, st)) != hipSuccess) return e;
     BhTables tb{p.t3, p.t3_ng, p.t3_lo, p.s1r2, p.s1r2_nrho, p.s1r2_nz, kRadialExtent * p.Rsim, p.radial,
-                p.radial_size, p.Rsim};
+                p.radial_size, p.Rsim, p.bh_radial_size > 0 ? p.bh_radial : nullptr, p.bh_radial_size};
     const int64_t id_max = p.id_min + p.nrows - 1;
 #define STEPS_BH_LAUNCH(MODE, ORD)                                                                                  \
     hipLaunchKernelGGL((bh_force_kernel<MODE, ORD>), dim3(ng), dim3(256), 0, st, v.srec, v.ssoft, v.sidx, n, v.info, \
-                       v.nodes, p.bh_theta, p.L, p.images, tb, p.ext_mode, p.ext_coeff, p.id_min, id_max, F)
+                       v.nodes, p.bh_theta, p.L, p.images, tb, fr, p.ext_mode, p.ext_coeff, p.id_min, id_max, F)
     switch (p.variant) {
     case PV_R3: STEPS_BH_LAUNCH(BH_R3, 0); break;
     case PV_T3_NEAREST: STEPS_BH_LAUNCH(BH_T3_NEAR, 0); break;

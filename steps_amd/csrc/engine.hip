@@ -242,6 +242,27 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
         p.bh_order = d.interp_order;
         if (p.bh_order < 2 || p.bh_order % 2 != 0) p.bh_order = 2;
         if (p.bh_order > 8) p.bh_order = 8;
+        if (d.bh_randomize) {
+            // RANDOMIZE_BH: draws rand()/RAND_MAX keep the moved root cube
+            // around the particles
+            p.bh_randomize = true;
+            for (int k = 0; k < 6; ++k) {
+                if (!(d.bh_random[k] >= 0.0 && d.bh_random[k] <= 1.0))
+                    return fail(STEPS_ERR_ARG, "randomised octree draws must lie in [0, 1]");
+                p.bh_u[k] = d.bh_random[k];
+            }
+        }
+        if (d.radial_bh_table) {
+            // USE_RADIAL_BH_CORRECTION (main.cc:791-1091): R3 and S1R2 only,
+            // applied with the comoving external term
+            if (d.topology == STEPS_TOPO_T3)
+                return fail(STEPS_ERR_ARG, "the radial octree correction exists for R3 and S1R2 only");
+            if (d.radial_bh_table_size < 2) return fail(STEPS_ERR_TABLE, "radial octree correction table needs 2 bins");
+            if (!(d.Rsim > 0)) return fail(STEPS_ERR_ARG, "radial octree correction needs Rsim > 0");
+            if (d.cosmology == 1 && d.comoving == 1) p.bh_radial_size = d.radial_bh_table_size;
+        }
+    } else if (d.bh_randomize || d.radial_bh_table) {
+        return fail(STEPS_ERR_ARG, "randomisation and the radial correction need the octree (bh_theta > 0)");
     }
     if (p.precision == STEPS_PREC_MIXED && !mixed_supported(p.variant))
         return fail(STEPS_ERR_ARG, "mixed precision is implemented for R3, T3 and S1R2 (nearest z-image, CIC or TSC correction)");
@@ -355,10 +376,14 @@ int sim_create(const steps_force_desc &d, int64_t N, bool both_precisions, steps
     if (p.ext_mode == 3 &&
         (rc = s->radial.sync(d.radial_table, (size_t)d.radial_table_size, s->stream, "radial table")))
         return cleanup(rc);
+    if (p.bh_radial_size > 0 &&
+        (rc = s->radial_bh.sync(d.radial_bh_table, (size_t)d.radial_bh_table_size, s->stream, "radial octree table")))
+        return cleanup(rc);
     // the sim owns device copies: host table pointers are not kept
     s->desc.t3_table = nullptr;
     s->desc.s1r2_table = nullptr;
     s->desc.radial_table = nullptr;
+    s->desc.radial_bh_table = nullptr;
     s->dp = DeviceParticles{s->jrec.as<double4>(), s->jsoft.as<double>(), N};
     if ((e = hipMemsetAsync(s->v.ptr, 0, 3 * nb, s->stream)) != hipSuccess ||
         (e = hipMemsetAsync(s->F.ptr, 0, 3 * nb, s->stream)) != hipSuccess ||
@@ -377,7 +402,7 @@ void sim_destroy(steps_sim *s) {
             if (g) (void)hipGraphExecDestroy(g);
     s->jrec.release(); s->jsoft.release(); s->v.release(); s->F.release(); s->part.release();
     s->stage.release(); s->blockmax.release(); s->dparams.release();
-    s->t3.dev.release(); s->s1r2.dev.release(); s->radial.dev.release();
+    s->t3.dev.release(); s->s1r2.dev.release(); s->radial.dev.release(); s->radial_bh.dev.release();
     if (s->hparams) (void)hipHostFree(s->hparams);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -391,6 +416,7 @@ int sim_plan(steps_sim *s, int64_t id_min, int64_t id_max, int precision, EvalPl
     d.t3_table = s->t3.ptr();
     d.s1r2_table = s->s1r2.ptr();
     d.radial_table = s->radial.ptr();
+    d.radial_bh_table = s->radial_bh.ptr();
     d.precision = precision;
     int rc = make_plan(d, s->N, id_min, id_max, p);
     if (rc) return rc;
@@ -399,6 +425,7 @@ int sim_plan(steps_sim *s, int64_t id_min, int64_t id_max, int precision, EvalPl
     p.t3_lo = s->t3.lo;
     p.s1r2 = s->s1r2.ptr();
     p.radial = s->radial.ptr();
+    p.bh_radial = s->radial_bh.ptr();
     return STEPS_OK;
 }
 
@@ -495,7 +522,7 @@ struct OneShot {
     int device = -1;
     hipStream_t stream = nullptr;
     DevBuf x, M, soft, jrec, jsoft, part, F;
-    TableCopy t3, s1r2, radial;
+    TableCopy t3, s1r2, radial, radial_bh;
 };
 OneShot g_oneshot;
 
@@ -541,7 +568,7 @@ int steps_force_eval(const steps_force_desc *d, int64_t N, const double *x, cons
         // buffers belong to the previous device: drop them
         o.x.release(); o.M.release(); o.soft.release(); o.jrec.release(); o.jsoft.release();
         o.part.release(); o.F.release();
-        o.t3 = TableCopy{}; o.s1r2 = TableCopy{}; o.radial = TableCopy{};
+        o.t3 = TableCopy{}; o.s1r2 = TableCopy{}; o.radial = TableCopy{}; o.radial_bh = TableCopy{};
         if (o.stream) (void)hipStreamDestroy(o.stream);
         o.stream = nullptr;
         STEPS_HIP(hipStreamCreateWithFlags(&o.stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -560,10 +587,14 @@ int steps_force_eval(const steps_force_desc *d, int64_t N, const double *x, cons
         return rc;
     if (p.ext_mode == 3 && (rc = o.radial.sync(d->radial_table, (size_t)d->radial_table_size, st, "radial table")))
         return rc;
+    if (p.bh_radial_size > 0 &&
+        (rc = o.radial_bh.sync(d->radial_bh_table, (size_t)d->radial_bh_table_size, st, "radial octree table")))
+        return rc;
     p.t3 = o.t3.ptr();
     p.t3_lo = o.t3.lo;
     p.s1r2 = o.s1r2.ptr();
     p.radial = o.radial.ptr();
+    p.bh_radial = o.radial_bh.ptr();
     STEPS_HIP(hipMemcpyAsync(o.x.ptr, x, 3 * nb, hipMemcpyHostToDevice, st), "copy x");
     STEPS_HIP(hipMemcpyAsync(o.M.ptr, M, nb, hipMemcpyHostToDevice, st), "copy M");
     STEPS_HIP(hipMemcpyAsync(o.soft.ptr, soft, nb, hipMemcpyHostToDevice, st), "copy soft");
@@ -586,6 +617,15 @@ int steps_sim_create(const steps_force_desc *d, int64_t N, steps_sim **out) {
 
 int steps_sim_destroy(steps_sim *s) {
     sim_destroy(s);
+    return STEPS_OK;
+}
+
+int steps_sim_set_bh_random(steps_sim *s, const double *u) {
+    if (!s || !u) return fail(STEPS_ERR_ARG, "null argument");
+    if (!s->desc.bh_randomize) return fail(STEPS_ERR_STATE, "the simulation's octree is not randomised");
+    for (int k = 0; k < 6; ++k)
+        if (!(u[k] >= 0.0 && u[k] <= 1.0)) return fail(STEPS_ERR_ARG, "randomised octree draws must lie in [0, 1]");
+    std::memcpy(s->desc.bh_random, u, sizeof s->desc.bh_random);
     return STEPS_OK;
 }
 
@@ -743,6 +783,48 @@ int steps_s1r2_table(int32_t is_periodic, double L, double Rsim, double *table, 
 
 int steps_radial_table(double R, double Lz, int32_t size, int32_t accuracy, double *table) {
     return build_radial_table(R, Lz, size, accuracy, table);
+}
+
+// utils.cc:96-156: per call, the radial forces are summed per bin in a local
+// table, which is then added to the running one (as the reference adds each
+// iteration's FORCE_TABLE_LOC to RADIAL_BH_FORCE_TABLE).
+int steps_radial_bh_table_accumulate(int32_t topology, double Rsim, int32_t size, int64_t N, const double *x,
+                                     const double *F, double *sum, int32_t *count) {
+    if (topology != STEPS_TOPO_R3 && topology != STEPS_TOPO_S1R2)
+        return fail(STEPS_ERR_ARG, "the radial octree correction exists for R3 and S1R2 only");
+    if (size < 1 || N < 0 || !(Rsim > 0) || !sum || !count || (N > 0 && (!x || !F)))
+        return fail(STEPS_ERR_ARG, "bad radial octree table arguments");
+    std::vector<double> loc((size_t)size, 0.0);
+    std::vector<int32_t> cnt((size_t)size, 0);
+    const double w = Rsim / (double)size;
+    for (int64_t i = 0; i < N; ++i) {
+        const double *p = x + 3 * i, *f = F + 3 * i;
+        double r, fr;
+        if (topology == STEPS_TOPO_R3) {
+            r = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+            fr = (f[0] * p[0] + f[1] * p[1] + f[2] * p[2]) / r;
+        } else {
+            r = std::sqrt(p[0] * p[0] + p[1] * p[1]);
+            fr = (f[0] * p[0] + f[1] * p[1]) / r;
+        }
+        if (!std::isfinite(r)) return fail(STEPS_ERR_NONFINITE, "non-finite position in the radial octree table");
+        const double bf = std::floor(r / w);
+        const int b = bf >= (double)size ? size - 1 : (int)bf;
+        loc[(size_t)b] += fr;
+        ++cnt[(size_t)b];
+    }
+    for (int b = 0; b < size; ++b) {
+        sum[b] += loc[(size_t)b];
+        count[b] += cnt[(size_t)b];
+    }
+    return STEPS_OK;
+}
+
+// main.cc:1062-1075: the mean radial force per bin, 0 where a bin is empty
+int steps_radial_bh_table_finish(int32_t size, double *sum, const int32_t *count) {
+    if (size < 1 || !sum || !count) return fail(STEPS_ERR_ARG, "bad radial octree table arguments");
+    for (int b = 0; b < size; ++b) sum[b] = count[b] > 0 ? sum[b] / (double)count[b] : 0.0;
+    return STEPS_OK;
 }
 
 }  // extern "C"

@@ -22,6 +22,8 @@ struct Key {
     size_t off;
     bool s1r2_only;  // RADIAL_FORCE_* exist only in the PERIODIC_Z build
 };
+// The USE_BH build's keys are read whatever the build flags here: they act
+// only when steps_run evaluates the octree.
 
 #define F(field) offsetof(steps_run_params, field)
 // read_paramfile.cc:91-123, in the order the reference tests them
@@ -57,6 +59,10 @@ const Key kKeys[] = {
     {"R_SIM", KDBL, F(Rsim), false},
     {"RADIAL_FORCE_ACCURACY", KINT, F(RADIAL_FORCE_ACCURACY), true},
     {"RADIAL_FORCE_TABLE_SIZE", KINT, F(RADIAL_FORCE_TABLE_SIZE), true},
+    {"RADIAL_BH_FORCE_CORRECTION", KINT, F(RADIAL_BH_FORCE_CORRECTION), false},
+    {"GLASS_FILE_FOR_BH_FORCE_CORRECTION", KSTR, F(GLASS_FILE_FOR_BH_FORCE_CORRECTION), false},
+    {"RADIAL_BH_FORCE_TABLE_SIZE", KINT, F(RADIAL_BH_FORCE_TABLE_SIZE), false},
+    {"RADIAL_BH_FORCE_TABLE_ITERATION", KINT, F(RADIAL_BH_FORCE_TABLE_ITERATION), false},
 };
 #undef F
 
@@ -113,6 +119,10 @@ extern "C" int steps_read_params(const char *path, int32_t topology, steps_run_p
     std::memset(p, 0, sizeof *p);
     const bool s1r2 = topology == STEPS_TOPO_S1R2;
     if (s1r2) p->RADIAL_FORCE_ACCURACY = p->RADIAL_FORCE_TABLE_SIZE = 1000;  // read_paramfile.cc:116-117
+    // read_paramfile.cc:136-141
+    std::snprintf(p->GLASS_FILE_FOR_BH_FORCE_CORRECTION, sizeof p->GLASS_FILE_FOR_BH_FORCE_CORRECTION, "None");
+    p->RADIAL_BH_FORCE_TABLE_SIZE = 64;
+    p->RADIAL_BH_FORCE_TABLE_ITERATION = 4;
     char line[1024];
     while (std::fgets(line, sizeof line, f)) {
         for (const Key &k : kKeys) {
@@ -127,6 +137,14 @@ extern "C" int steps_read_params(const char *path, int32_t topology, steps_run_p
             if (!std::strcmp(k.name, "RADIAL_FORCE_ACCURACY") && p->RADIAL_FORCE_ACCURACY < 10) p->RADIAL_FORCE_ACCURACY = 10;
             if (!std::strcmp(k.name, "RADIAL_FORCE_TABLE_SIZE") && p->RADIAL_FORCE_TABLE_SIZE < 5)
                 p->RADIAL_FORCE_TABLE_SIZE = 5;
+            // read_paramfile.cc:351-378
+            if (!std::strcmp(k.name, "RADIAL_BH_FORCE_CORRECTION") && p->RADIAL_BH_FORCE_CORRECTION != 0 &&
+                p->RADIAL_BH_FORCE_CORRECTION != 1)
+                p->RADIAL_BH_FORCE_CORRECTION = 0;
+            if (!std::strcmp(k.name, "RADIAL_BH_FORCE_TABLE_SIZE") && p->RADIAL_BH_FORCE_TABLE_SIZE < 5)
+                p->RADIAL_BH_FORCE_TABLE_SIZE = 5;
+            if (!std::strcmp(k.name, "RADIAL_BH_FORCE_TABLE_ITERATION") && p->RADIAL_BH_FORCE_TABLE_ITERATION < 1)
+                p->RADIAL_BH_FORCE_TABLE_ITERATION = 1;
         }
     }
     std::fclose(f);

@@ -3,13 +3,19 @@
 //
 //   steps_run <parameter file> [--topology r3|t3|s1r2] [--precision fp64|mixed]
 //             [--interp-order 0|2|4|6|8] [--no-lookup] [--bh THETA]
-//             [--max-steps K] [--device D]
+//             [--bh-seed SEED] [--max-steps K] [--device D]
 //
-// The topology, the interpolation order, the table / no-table mode and the
-// Barnes-Hut octree with its opening angle are build flags in the reference
-// (PERIODIC, PERIODIC_Z, EWALD_INTERPOLATION_ORDER, PERIODIC_Z_NOLOOKUP,
-// USE_BH=THETA); here they are options.  The octree's radial correction
-// table (RADIAL_BH_FORCE_CORRECTION, main.cc:950-1019) is not built.
+// The topology, the interpolation order, the table / no-table mode, the
+// Barnes-Hut octree with its opening angle and its randomisation are build
+// flags in the reference (PERIODIC, PERIODIC_Z, EWALD_INTERPOLATION_ORDER,
+// PERIODIC_Z_NOLOOKUP, USE_BH=THETA, RANDOMIZE_BH=SEED); here they are
+// options.  With the octree, RADIAL_BH_FORCE_CORRECTION = 1 in R3 or S1R2
+// first builds the radial correction table from the octree forces of the
+// glass file (or the IC file) as main.cc:791-1091 does, averaged over
+// RADIAL_BH_FORCE_TABLE_ITERATION randomised evaluations, writes it to
+// OUT_DIR/radial_bh_force_table.dat and subtracts it in every evaluation.
+// With --bh-seed, rand() is seeded as main.cc:310 does and every octree
+// evaluation draws its frame from it in the reference's order.
 // The run follows main.cc:1100-1864 and step.cc:100-312:
 //   * initial conditions (ASCII or GADGET), H0-independent units, velocity
 //     rescaling (main.cc:1144-1181); equal GADGET masses for T3, the mass
@@ -73,6 +79,7 @@ struct Opts {
     int interp_order = 2;  // EWALD_INTERPOLATION_ORDER of the reference's Makefile
     int lookup = 1;
     double bh_theta = 0.0;
+    long bh_seed = -1;  // RANDOMIZE_BH
     long max_steps = -1;
     int device = 0;
 };
@@ -100,6 +107,11 @@ bool parse_opts(int argc, char **argv, Opts &o) {
         } else if (a == "--bh") {
             o.bh_theta = std::atof(val().c_str());
             if (!(o.bh_theta > 0.0)) return false;
+        } else if (a == "--bh-seed") {
+            const std::string v = val();
+            if (v.empty()) return false;
+            o.bh_seed = std::atol(v.c_str());
+            if (o.bh_seed < 0) return false;
         } else if (a == "--max-steps") {
             o.max_steps = std::atol(val().c_str());
         } else if (a == "--device") {
@@ -110,7 +122,27 @@ bool parse_opts(int argc, char **argv, Opts &o) {
             return false;
         }
     }
-    return !o.param.empty();
+    return !o.param.empty() && (o.bh_seed < 0 || o.bh_theta > 0.0);
+}
+
+// rand()'s state for the randomised octree: the state srand(seed) gives
+// (glibc's 128-byte default kind), kept apart from the process's own, which
+// the HIP runtime draws from as it starts.
+char g_draw_state[128];
+
+void seed_draws(unsigned seed) {  // main.cc:310
+    char *prev = initstate(seed, g_draw_state, sizeof g_draw_state);
+    setstate(prev);
+}
+
+// The draws of one randomised octree evaluation, rand()/RAND_MAX in the
+// order the reference's force routine makes them (forces.cc:422-430 and
+// 362-363 R3, 664-667 T3, 1027-1030 S1R2).
+void draw_frame(int topology, double *u) {
+    const int n = topology == STEPS_TOPO_R3 ? 6 : topology == STEPS_TOPO_T3 ? 3 : 4;
+    char *prev = setstate(g_draw_state);
+    for (int k = 0; k < 6; ++k) u[k] = k < n ? (double)std::rand() / (double)RAND_MAX : 0.0;
+    setstate(prev);
 }
 
 // write_ascii_snapshot's file name: t<100 t Gy>.dat or z<1000 z>.dat
@@ -131,7 +163,9 @@ int main(int argc, char **argv) {
     Opts o;
     if (!parse_opts(argc, argv, o))
         return die("usage: steps_run <parameter file> [--topology r3|t3|s1r2] [--precision fp64|mixed] "
-                   "[--interp-order 0|2|4|6|8] [--no-lookup] [--bh THETA] [--max-steps K] [--device D]", 2);
+                   "[--interp-order 0|2|4|6|8] [--no-lookup] [--bh THETA] [--bh-seed SEED] [--max-steps K] "
+                   "[--device D]", 2);
+    if (o.bh_seed >= 0) seed_draws((unsigned)o.bh_seed);
     steps_run_params P;
     if (steps_read_params(o.param.c_str(), o.topology, &P)) return die(steps_host_last_error());
     if (P.IC_FORMAT == 2) return die("IC_FORMAT 2 (HDF5) needs the HDF5 library, which this build does not link");
@@ -261,6 +295,75 @@ int main(int argc, char **argv) {
         }
     }
 
+    d.bh_randomize = o.bh_seed >= 0 ? 1 : 0;  // RANDOMIZE_BH
+
+    // ---- the radial octree correction table (main.cc:791-1091), before the
+    // run's own force evaluations, as there
+    std::vector<double> radial_bh;
+    if (o.bh_theta > 0.0 && P.RADIAL_BH_FORCE_CORRECTION == 1 && o.topology != STEPS_TOPO_T3) {
+        std::string glass = P.GLASS_FILE_FOR_BH_FORCE_CORRECTION;
+        if (glass != "None") {
+            if (std::FILE *g = std::fopen(glass.c_str(), "r")) {
+                std::fclose(g);
+            } else {
+                std::fprintf(stderr, "steps_run: the glass file %s does not exist; using the IC file\n", glass.c_str());
+                glass = "None";
+            }
+        }
+        if (glass == "None") glass = P.IC_FILE;
+        int64_t Ng = 0;
+        std::vector<double> xg, vg, Mg;
+        if (P.IC_FORMAT == 0) {
+            Ng = steps_ascii_count_particles(glass.c_str());
+            if (Ng <= 0) return die(Ng < 0 ? steps_host_last_error() : "empty glass file " + glass);
+            xg.resize(3 * Ng), vg.resize(3 * Ng), Mg.resize(Ng);
+            if (steps_read_ascii_ic(glass.c_str(), Ng, xg.data(), vg.data(), Mg.data()))
+                return die(steps_host_last_error());
+        } else {
+            double Lg = 0, ag = 0;
+            if (steps_read_gadget_ic(glass.c_str(), &Ng, nullptr, nullptr, nullptr, &Lg, &ag))
+                return die(steps_host_last_error());
+            xg.resize(3 * Ng), vg.resize(3 * Ng), Mg.resize(Ng);
+            if (steps_read_gadget_ic(glass.c_str(), &Ng, xg.data(), vg.data(), Mg.data(), &Lg, &ag))
+                return die(steps_host_last_error());
+        }
+        if (Ng != N)  // main.cc:1110-1116
+            return die("the radial octree correction's particle count (" + std::to_string(Ng) +
+                       ") differs from the initial conditions' (" + std::to_string(N) + ")");
+        if (P.H0_INDEPENDENT_UNITS != 0 && cosmo) {  // main.cc:817-830
+            for (auto &c : xg) c /= h0d;
+            for (auto &m : Mg) m /= h0d;
+        }
+        std::vector<double> softg(Ng), Fg(3 * Ng);
+        double mmin_g = 0, rho_g = 0;
+        if (steps_softening_lengths(Mg.data(), Ng, P.ParticleRadi, softg.data(), &mmin_g, &rho_g))
+            return die(steps_last_error());
+        const int iters = o.bh_seed >= 0 ? P.RADIAL_BH_FORCE_TABLE_ITERATION : 1;  // main.cc:979-983
+        const int size = P.RADIAL_BH_FORCE_TABLE_SIZE;
+        radial_bh.assign((size_t)size, 0.0);
+        std::vector<int32_t> count((size_t)size, 0);
+        for (int it = 0; it < iters; ++it) {
+            steps_force_desc dg = d;
+            if (o.bh_seed >= 0) draw_frame(o.topology, dg.bh_random);
+            if (steps_force_eval(&dg, Ng, xg.data(), Mg.data(), softg.data(), Fg.data(), 0, Ng - 1))
+                return die(std::string("radial octree correction: ") + steps_last_error());
+            if (steps_radial_bh_table_accumulate(o.topology, P.Rsim, size, Ng, xg.data(), Fg.data(), radial_bh.data(),
+                                                 count.data()))
+                return die(steps_last_error());
+        }
+        steps_radial_bh_table_finish(size, radial_bh.data(), count.data());
+        for (int b = 0; b < size; ++b)
+            if (count[(size_t)b] == 0) std::printf("steps_run: no particles in shell %d, its correction is zero\n", b);
+        const std::string tname = std::string(P.OUT_DIR) + "radial_bh_force_table.dat";
+        if (steps_write_ascii_table(tname.c_str(), 0.5 * P.Rsim / size, P.Rsim, P.Rsim / size, radial_bh.data(), size,
+                                    "# Radial BH force correction table\n# r[Mpc]    F_correction[internal units]"))
+            return die(steps_host_last_error());
+        std::printf("steps_run: radial octree correction table (%d bins, %d evaluations) written to %s\n", size, iters,
+                    tname.c_str());
+        d.radial_bh_table = radial_bh.data();
+        d.radial_bh_table_size = size;
+    }
+
     // ---- time and the output schedule (main.cc:1422-1565)
     double a = cosmo ? P.a_start : 1.0, T = 0.0, H = 0.0, a_max = P.a_max, t_next = 0.0, dT_out = 0.0;
     if (cosmo) T = C.time_of(P.a_start);
@@ -313,7 +416,13 @@ int main(int argc, char **argv) {
         return die(m);
     };
     if (steps_sim_set_state(sim, x.data(), v.data(), M.data(), soft.data())) return fail_sim("state");
-    if (steps_sim_forces(sim, 0, N - 1)) return fail_sim("initial force");
+    auto next_frame = [&]() -> int {  // a randomised octree draws afresh per evaluation
+        if (o.bh_seed < 0) return 0;
+        double u[6];
+        draw_frame(o.topology, u);
+        return steps_sim_set_bh_random(sim, u);
+    };
+    if (next_frame() || steps_sim_forces(sim, 0, N - 1)) return fail_sim("initial force");
     double errmax = 0.0;
     if (steps_sim_errmax(sim, a, H, 1.0, rho_part, &errmax)) return fail_sim("initial step length");
     double h = std::sqrt(2.0 * P.ACC_PARAM / errmax);
@@ -342,7 +451,7 @@ int main(int argc, char **argv) {
             H = C.hubble(a);
         }
         steps_step_params sp{h, a1, H1, a, H, 1.0, rho_part};
-        if (steps_sim_step(sim, &sp, &errmax)) {
+        if (next_frame() || steps_sim_step(sim, &sp, &errmax)) {
             std::fclose(log);
             return fail_sim("KDK step");
         }

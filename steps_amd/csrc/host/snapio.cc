@@ -170,4 +170,21 @@ int steps_write_ascii_snapshot(const char *path, int64_t N, const double *x, con
     return STEPS_OK;
 }
 
+// inputoutput.cc:1038-1061
+int steps_write_ascii_table(const char *path, double x_min, double x_max, double deltax, const double *values,
+                            int32_t n, const char *header) {
+    if (!path || n < 0 || (n > 0 && !values) || !header) return host_fail(STEPS_ERR_ARG, "bad arguments");
+    File f(std::fopen(path, "w"));
+    if (!f.f) return host_fail(STEPS_ERR_ARG, std::string("cannot write ") + path);
+    std::fprintf(f.f, "%s\n", header);
+    double arg = x_min;
+    for (int32_t i = 0; i < n; ++i) {
+        if (arg > x_max) break;
+        std::fprintf(f.f, "%.6f\t%.15f\n", arg, values[i]);
+        arg += deltax;
+    }
+    if (std::ferror(f.f)) return host_fail(STEPS_ERR_ARG, std::string("write error on ") + path);
+    return STEPS_OK;
+}
+
 }  // extern "C"

@@ -86,6 +86,12 @@ struct EvalPlan {
     bool bh = false;
     double bh_theta = 0, bh_particle_radius = 0;  // opening angle, ParticleRadi
     int bh_order = 2;
+    // the randomised octree (RANDOMIZE_BH) and its draws u0..u5
+    bool bh_randomize = false;
+    double bh_u[6] = {0, 0, 0, 0, 0, 0};
+    // the radial octree correction table (device), applied with the
+    // comoving external term of R3 and S1R2
+    const double *bh_radial = nullptr; int bh_radial_size = 0;
 };
 
 // The T3 Ewald evaluation deals its sources out to the eight XCDs by table

@@ -119,8 +119,9 @@ extern "C" int steps_force_eval_noalloc(const steps_force_desc *d, int64_t N, co
         s1_dev = s1r2_device_nodes(d->s1r2_nrho, d->s1r2_nz) * 2u * sizeof(double);
     }
     if (radial) rad = (size_t)d->radial_table_size * sizeof(double);
+    const size_t rbh = (size_t)p.bh_radial_size * sizeof(double);
     const size_t pieces[] = {3 * nb, nb, nb, (size_t)N * sizeof(double4), nb, 3 * (size_t)p.nrows * sizeof(double),
-                             scratch_bytes(p), t3_raw, t3_dev, s1_raw, s1_dev, rad};
+                             scratch_bytes(p), t3_raw, t3_dev, s1_raw, s1_dev, rad, rbh};
     size_t need = 0;
     for (size_t b : pieces) need += (b + 255) & ~(size_t)255;
     if (need > kNoAllocScratchBytes)
@@ -134,6 +135,7 @@ extern "C" int steps_force_eval_noalloc(const steps_force_desc *d, int64_t N, co
     double *t3r = cv.take<double>(t3_raw), *t3d = cv.take<double>(t3_dev);
     double *s1r = cv.take<double>(s1_raw), *s1d = cv.take<double>(s1_dev);
     double *radd = cv.take<double>(rad);
+    double *rbhd = cv.take<double>(rbh);
 
     const hipStream_t st = nullptr;  // the null stream: no stream is created
     if (t3) {
@@ -153,6 +155,10 @@ extern "C" int steps_force_eval_noalloc(const steps_force_desc *d, int64_t N, co
     if (radial) {
         STEPS_HIP(hipMemcpyAsync(radd, d->radial_table, rad, hipMemcpyHostToDevice, st), "copy radial table");
         p.radial = radd;
+    }
+    if (rbh) {
+        STEPS_HIP(hipMemcpyAsync(rbhd, d->radial_bh_table, rbh, hipMemcpyHostToDevice, st), "copy radial octree table");
+        p.bh_radial = rbhd;
     }
     STEPS_HIP(hipMemcpyAsync(dx, x, 3 * nb, hipMemcpyHostToDevice, st), "copy x");
     STEPS_HIP(hipMemcpyAsync(dM, M, nb, hipMemcpyHostToDevice, st), "copy M");

@@ -23,7 +23,10 @@
 // STEPS_FORCE_PRECISION=mixed selects the mixed-precision R^3 path.  Built
 // with -DUSE_BH (the reference's Barnes-Hut build, which links forces.o) the
 // same symbols evaluate the octree with the opening angle THETA and the
-// node softening of ParticleRadi, globals main.cc:61-64 defines.
+// node softening of ParticleRadi, globals main.cc:61-64 defines; with
+// -DRANDOMIZE_BH=SEED too, each call draws its frame with rand() in the
+// reference's order (main.cc:310 seeds it); once USE_RADIAL_BH_CORRECTION
+// is set, R3 and S1R2 calls subtract RADIAL_BH_FORCE_TABLE's correction.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -56,6 +59,9 @@ extern REAL rho_part;
 extern REAL M_min;
 #ifdef USE_BH
 extern REAL THETA;  // main.cc:64, set from USE_BH at main.cc:303
+extern int RADIAL_BH_FORCE_TABLE_SIZE;  // main.cc:126-133
+extern REAL *RADIAL_BH_FORCE_TABLE;
+extern bool USE_RADIAL_BH_CORRECTION;
 #endif
 #if defined(PERIODIC)
 extern REAL *T3_EWALD_FORCE_TABLE;
@@ -106,6 +112,21 @@ steps_force_desc descriptor(int topology) {
 #ifdef USE_BH
     d.bh_theta = THETA;
     d.bh_particle_radius = ParticleRadi;
+#if !defined(PERIODIC)
+    if (USE_RADIAL_BH_CORRECTION) {  // forces.cc:479-485, 1143-1147, 1180-1184
+        d.radial_bh_table = RADIAL_BH_FORCE_TABLE;
+        d.radial_bh_table_size = RADIAL_BH_FORCE_TABLE_SIZE;
+    }
+#endif
+#ifdef RANDOMIZE_BH
+    // the draws of this call, in the order the reference's force routine
+    // makes them: R3 centre x, y, z, angle, axis theta, axis z (forces.cc:
+    // 422-430, 362-363); T3 shift x, y, z (:664-667); S1R2 angle, centre x,
+    // y, z shift (:1027-1030)
+    const int draws = topology == STEPS_TOPO_R3 ? 6 : topology == STEPS_TOPO_T3 ? 3 : 4;
+    d.bh_randomize = 1;
+    for (int k = 0; k < draws; ++k) d.bh_random[k] = (double)rand() / (double)RAND_MAX;
+#endif
 #endif
 #if defined(PERIODIC)
     d.t3_table = T3_EWALD_FORCE_TABLE;
@@ -125,8 +146,29 @@ steps_force_desc descriptor(int topology) {
     return d;
 }
 
+#ifdef RANDOMIZE_BH
+// The HIP runtime draws from rand() as it starts, inside the library's first
+// call: library calls run on a scratch random() state, so that the process's
+// sequence (srand(RANDOMIZE_BH), main.cc:310) advances only by the draws the
+// reference's force routine makes.
+class ScratchRandomState {
+  public:
+    ScratchRandomState() { prev_ = initstate(1u, buf_, sizeof buf_); }
+    ~ScratchRandomState() { setstate(prev_); }
+    ScratchRandomState(const ScratchRandomState &) = delete;
+    ScratchRandomState &operator=(const ScratchRandomState &) = delete;
+
+  private:
+    char buf_[128];
+    char *prev_;
+};
+#endif
+
 void evaluate(int topology, REAL *x, REAL *F, int ID_min, int ID_max) {
     const steps_force_desc d = descriptor(topology);
+#ifdef RANDOMIZE_BH
+    ScratchRandomState guard;
+#endif
     const int rc = steps_force_eval(&d, (int64_t)N, x, M, SOFT_LENGTH, F, ID_min, ID_max);
     if (rc != STEPS_OK) {
         std::fprintf(stderr, "MPI rank %i: steps force evaluation failed (%d): %s\n", rank, rc,

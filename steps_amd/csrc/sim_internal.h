@@ -66,7 +66,7 @@ struct steps_sim {
     bool both_precisions = false;          // scratch sized for the mixed path too
     steps::DeviceParticles dp;
     steps::DevBuf jrec, jsoft, v, F, part, stage, blockmax, dparams;
-    steps::TableCopy t3, s1r2, radial;
+    steps::TableCopy t3, s1r2, radial, radial_bh;
     steps::KdkParams *hparams = nullptr;   // pinned: copied to dparams by the step sequence
     double *herrmax = nullptr;             // pinned: errmax copied back by the step sequence
     hipStream_t stream = nullptr;

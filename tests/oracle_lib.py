@@ -42,6 +42,11 @@ class OracleParams(C.Structure):
     ]
 
 
+class OracleBhOpts(C.Structure):
+    _fields_ = [("randomize", C.c_int), ("u", C.c_double * 6), ("radial_bh_table", _DP),
+                ("radial_bh_table_size", C.c_int)]
+
+
 class OracleCosmo(C.Structure):
     _fields_ = [("H0", C.c_double), ("Omega_m", C.c_double), ("Omega_lambda", C.c_double),
                 ("Omega_r", C.c_double), ("Omega_k", C.c_double)]
@@ -83,6 +88,15 @@ def lib() -> C.CDLL:
     L.oracle_forces_bh.restype = C.c_int
     L.oracle_forces_bh.argtypes = [C.POINTER(OracleParams), C.c_double, C.c_double, C.c_int, _DP, _DP, _DP, _DP,
                                    C.c_int, C.c_int]
+    L.oracle_forces_bh_ex.restype = C.c_int
+    L.oracle_forces_bh_ex.argtypes = [C.POINTER(OracleParams), C.c_double, C.c_double, C.POINTER(OracleBhOpts),
+                                      C.c_int, _DP, _DP, _DP, _DP, C.c_int, C.c_int]
+    L.oracle_bh_last_root.restype = None
+    L.oracle_bh_last_root.argtypes = [_DP]
+    L.oracle_radial_bh_correction.restype = C.c_double
+    L.oracle_radial_bh_correction.argtypes = [C.c_double, C.c_double, _DP, C.c_int]
+    L.oracle_radial_bh_table_accumulate.restype = None
+    L.oracle_radial_bh_table_accumulate.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, _DP, _DP, _DP, _IP]
     L.oracle_forces_r3_longdouble.restype = None
     L.oracle_forces_r3_longdouble.argtypes = [C.POINTER(OracleParams), C.c_int, _DP, _DP, _DP,
                                               C.POINTER(C.c_longdouble)]
@@ -206,6 +220,55 @@ def forces_bh(p: OracleParams, theta: float, particle_radius: float, x, M, h, id
     rc = lib().oracle_forces_bh(C.byref(p), theta, particle_radius, N, dp(x), dp(M), dp(h), dp(F), id_min, id_max)
     assert rc == 0, rc
     return F[: 3 * n].reshape(n, 3)
+
+
+def forces_bh_ex(p: OracleParams, theta: float, particle_radius: float, x, M, h, *, u=None, radial_bh=None,
+                 id_min=0, id_max=None) -> np.ndarray:
+    """Octree forces with the RANDOMIZE_BH draws u (six values in [0, 1], the
+    reference's order) and/or the radial correction table radial_bh."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    N = M.size
+    if id_max is None:
+        id_max = N - 1
+    n = max(0, id_max - id_min + 1)
+    o = OracleBhOpts()
+    if u is not None:
+        o.randomize = 1
+        for k, v in enumerate(list(u) + [0.0] * (6 - len(u))):
+            o.u[k] = float(v)
+    tab = None
+    if radial_bh is not None:
+        tab = np.ascontiguousarray(radial_bh, dtype=np.float64)
+        o.radial_bh_table = dp(tab)
+        o.radial_bh_table_size = tab.size
+    F = np.zeros(3 * max(n, 1))
+    rc = lib().oracle_forces_bh_ex(C.byref(p), theta, particle_radius, C.byref(o), N, dp(x), dp(M), dp(h), dp(F),
+                                   id_min, id_max)
+    assert rc == 0, rc
+    return F[: 3 * n].reshape(n, 3)
+
+
+def bh_last_root() -> np.ndarray:
+    """(centre x, y, z, side) of the last octree call's root cube."""
+    out = np.zeros(4)
+    lib().oracle_bh_last_root(dp(out))
+    return out
+
+
+def radial_bh_correction(r: float, R: float, table) -> float:
+    t = np.ascontiguousarray(table, dtype=np.float64)
+    return lib().oracle_radial_bh_correction(float(r), float(R), dp(t), t.size)
+
+
+def radial_bh_table_accumulate(topology: int, Rsim: float, x, F, table, count) -> None:
+    """Adds one evaluation's per-bin radial force sums and counts (in place)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1)
+    assert table.dtype == np.float64 and count.dtype == np.int32 and table.flags.c_contiguous
+    lib().oracle_radial_bh_table_accumulate(topology, Rsim, table.size, x.size // 3, dp(x), dp(F), dp(table),
+                                            ip(count))
 
 
 def bh_last_counts() -> tuple[int, int]:

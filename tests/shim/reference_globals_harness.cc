@@ -23,6 +23,9 @@ REAL rho_part = 0;
 REAL M_min = 0;
 #ifdef USE_BH
 REAL THETA = 0;
+int RADIAL_BH_FORCE_TABLE_SIZE = 0;
+REAL *RADIAL_BH_FORCE_TABLE = nullptr;
+bool USE_RADIAL_BH_CORRECTION = false;
 #endif
 #if defined(PERIODIC)
 REAL *T3_EWALD_FORCE_TABLE = nullptr;
@@ -66,6 +69,12 @@ void harness_set_theta(double theta) {
     THETA = theta;
 #endif
     (void)theta;
+}
+void harness_set_radial_bh(REAL *table, int size, int use) {
+#ifdef USE_BH
+    RADIAL_BH_FORCE_TABLE = table; RADIAL_BH_FORCE_TABLE_SIZE = size; USE_RADIAL_BH_CORRECTION = use != 0;
+#endif
+    (void)table; (void)size; (void)use;
 }
 int harness_variant() {
 #if defined(PERIODIC)

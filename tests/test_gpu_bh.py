@@ -6,7 +6,10 @@ accept / open decision per (row, node), so the forces agree to the rounding
 of the centres of mass and of the sums: per-row relative error <= 1e-12,
 for several opening angles, row ranges, the external-term modes and
 particle counts from 1 up; and the octree with an opening angle so small
-that no node is accepted is the direct sum."""
+that no node is accepted is the direct sum.  The randomised octree
+(RANDOMIZE_BH: rotated / shifted frame, moved root, forces turned back) and
+the radial correction table are held to the same bound against the oracle,
+in one-shot and resident evaluations with new draws per evaluation."""
 import numpy as np
 import pytest
 
@@ -109,20 +112,21 @@ def test_bh_refusals(steps):
 
 # ---- periodic topologies (forces.cc:582-772 T3, 881-1217 S1R2) ------------------
 
-def _periodic_case(steps, oracle, mode):
-    """(workload, device descriptor + keepalive, oracle params) of one periodic octree mode."""
+def _periodic_case(steps, oracle, mode, **extra):
+    """(workload, device descriptor + keepalive, oracle params) of one periodic
+    octree mode; extra goes to the device descriptor (bh_random, radial_bh_table)."""
     if mode.startswith("t3"):
         ip = 1 if mode == "t3_nearest" else 2
         w = wl.t3_lattice(n_side=16, seed=21, is_periodic=ip)
         order = {"t3_nearest": 2, "t3_ewald_o2": 2, "t3_ewald_o4": 4, "t3_ewald_o6": 6}[mode]
         T = steps.t3_table(2, w.L) if ip >= 2 else None
         d = steps.make_desc(steps.TOPO_T3, is_periodic=ip, L=w.L, t3_table=T, interp_order=order, bh_theta=0.5,
-                            bh_particle_radius=w.particle_radius)
+                            bh_particle_radius=w.particle_radius, **extra)
         p = oracle.params(1, is_periodic=ip, L=w.L, t3_table=T, interp_order=order)
         return w, d, p
     w = wl.s1r2_cylinder(n_omega=16, n_phi=16, n_z=16, seed=22, is_periodic=1 if mode == "s1r2_nearest" else 2)
     kw = dict(L=w.L, Rsim=w.Rsim, cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere)
-    bh = dict(bh_theta=0.5, bh_particle_radius=w.particle_radius)
+    bh = dict(bh_theta=0.5, bh_particle_radius=w.particle_radius, **extra)
     if mode == "s1r2_nearest":
         R = steps.radial_table(w.Rsim, 0.5 * w.L, 1000, 1000)
         d = steps.make_desc(steps.TOPO_S1R2, is_periodic=1, radial_table=R, **kw, **bh)
@@ -152,3 +156,106 @@ def test_bh_periodic_matches_oracle(steps, oracle, mode):
     assert e.max() <= 1e-12
     lo, hi = 100, 900
     assert np.array_equal(steps.force_eval(d, w.x, w.M, w.soft, id_min=lo, id_max=hi), F[lo:hi + 1])
+
+
+# ---- randomised octree (RANDOMIZE_BH) and the radial correction -----------------
+
+_U = {1: [0.83, 0.12, 0.47, 0.66, 0.29, 0.91], 2: [0.05, 0.98, 0.51, 0.37, 0.74, 0.18]}
+
+
+@pytest.mark.parametrize("mode,draw", [("r3", 1), ("r3", 2), ("t3_nearest", 1), ("t3_ewald_o4", 2),
+                                       ("s1r2_nearest", 1), ("s1r2_cic", 2), ("s1r2_images", 1)])
+def test_bh_randomised_matches_oracle(steps, oracle, mode, draw):
+    print(f"case bh randomised {mode} draw {draw}", flush=True)
+    u = _U[draw]
+    if mode == "r3":
+        w = wl.r3_compactified(N=8192, seed=31)
+        (d, keep), kw = _desc(steps, w, 0.5)
+        d.bh_randomize = 1
+        for k in range(6):
+            d.bh_random[k] = u[k]
+        p = oracle.params(0, **kw)
+    else:
+        w, (d, keep), p = _periodic_case(steps, oracle, mode, bh_random=u)
+    Fo = oracle.forces_bh_ex(p, 0.5, w.particle_radius, w.x, w.M, w.soft, u=u)
+    F = steps.force_eval(d, w.x, w.M, w.soft)
+    e = rel_err(F, Fo)
+    report(f"bh randomised {mode} N={w.N}", e)
+    # rounding of the centres of mass and sums, as in the plain octree; rows
+    # whose pair sum nearly cancels the external term reach ~1e-12
+    assert e.max() <= 5e-12 and np.percentile(e, 99) <= 1e-12
+    # and it is not the plain octree
+    assert not np.array_equal(F, oracle.forces_bh(p, 0.5, w.particle_radius, w.x, w.M, w.soft))
+
+
+@pytest.mark.parametrize("mode,randomised", [("r3", False), ("r3", True), ("s1r2_nearest", False),
+                                             ("s1r2_cic", True)])
+def test_bh_radial_correction_matches_oracle(steps, oracle, mode, randomised):
+    print(f"case bh radial correction {mode} randomised={randomised}", flush=True)
+    u = _U[1] if randomised else None
+    extra = dict(bh_random=u) if randomised else {}
+    if mode == "r3":
+        w = wl.r3_compactified(N=8192, seed=32)
+        Rsim = float(np.sqrt((w.x ** 2).sum(axis=1)).max())
+        kw = dict(cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere, Rsim=Rsim)
+        tab = (np.linspace(-1.0, 2.0, 24) ** 3) * 1e-3 * w.mass_in_unit_sphere * Rsim
+        d = steps.make_desc(steps.TOPO_R3, bh_theta=0.5, bh_particle_radius=w.particle_radius, radial_bh_table=tab,
+                            **kw, **extra)
+        p = oracle.params(0, **kw)
+    else:
+        w0 = wl.s1r2_cylinder(n_omega=16, n_phi=16, n_z=16, seed=22, is_periodic=1 if mode == "s1r2_nearest" else 2)
+        tab = np.linspace(-1.0, 2.0, 24) * 1e-2 * w0.mass_in_unit_sphere * w0.Rsim
+        w, d, p = _periodic_case(steps, oracle, mode, radial_bh_table=tab, **extra)
+    d, keep = d
+    Fo = oracle.forces_bh_ex(p, 0.5, w.particle_radius, w.x, w.M, w.soft, u=u, radial_bh=tab)
+    F = steps.force_eval(d, w.x, w.M, w.soft)
+    e = rel_err(F, Fo)
+    report(f"bh radial correction {mode} randomised={randomised}", e)
+    assert e.max() <= 5e-12 and np.percentile(e, 99) <= 1e-12
+    assert not np.array_equal(F, oracle.forces_bh_ex(p, 0.5, w.particle_radius, w.x, w.M, w.soft, u=u))
+
+
+def test_bh_randomised_resident_sim(steps, oracle):
+    print("case bh randomised resident sim", flush=True)
+    w = wl.r3_compactified(N=12000, seed=33)
+    (d, keep), kw = _desc(steps, w, 0.6)
+    p = oracle.params(0, **kw)
+    d.bh_randomize = 1
+    for k in range(6):
+        d.bh_random[k] = _U[1][k]
+    sim = steps.Sim(d, w.N)
+    sim.set_state(w.x, w.M, w.soft, w.v)
+    for draw in (1, 2, 1):  # new draws before each evaluation, as the reference's rand() per call
+        sim.set_bh_random(_U[draw])
+        sim.forces()
+        Fo = oracle.forces_bh_ex(p, 0.6, w.particle_radius, w.x, w.M, w.soft, u=_U[draw])
+        assert rel_err(sim.get_forces(), Fo).max() <= 5e-12, draw
+    with pytest.raises(_abi.StepsError) as ei:
+        sim.set_bh_random([1.5, 0, 0, 0, 0, 0])
+    assert ei.value.code == _abi.STEPS_ERR_ARG
+    sim.close()
+    (d2, keep2), _ = _desc(steps, w, 0.6)
+    plain = steps.Sim(d2, w.N)
+    with pytest.raises(_abi.StepsError) as ei:
+        plain.set_bh_random(_U[1])
+    assert ei.value.code == _abi.STEPS_ERR_STATE
+    plain.close()
+
+
+def test_bh_randomised_and_radial_refusals(steps):
+    print("case bh randomised/radial refusals", flush=True)
+    w = wl.r3_compactified(N=100, seed=18)
+    tab = np.ones(8)
+    bad = [steps.make_desc(steps.TOPO_R3, bh_random=_U[1]),  # no octree
+           steps.make_desc(steps.TOPO_R3, Rsim=10.0, radial_bh_table=tab),  # no octree
+           steps.make_desc(steps.TOPO_R3, bh_theta=0.5, bh_random=[0.5, 0.5, 1.5, 0, 0, 0]),
+           steps.make_desc(steps.TOPO_T3, is_periodic=1, L=10.0, Rsim=5.0, bh_theta=0.5, radial_bh_table=tab),
+           steps.make_desc(steps.TOPO_R3, bh_theta=0.5, radial_bh_table=tab)]  # Rsim = 0
+    for d, keep in bad:
+        with pytest.raises(_abi.StepsError) as ei:
+            steps.force_eval(d, w.x, w.M, w.soft)
+        assert ei.value.code == _abi.STEPS_ERR_ARG
+    d, keep = steps.make_desc(steps.TOPO_R3, bh_theta=0.5, Rsim=10.0, radial_bh_table=tab[:1])
+    with pytest.raises(_abi.StepsError) as ei:
+        steps.force_eval(d, w.x, w.M, w.soft)
+    assert ei.value.code == _abi.STEPS_ERR_TABLE

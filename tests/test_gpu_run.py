@@ -117,3 +117,46 @@ def test_steps_run_barnes_hut(tmp_path):
     assert np.allclose(s_t[:, :3], s_d[:, :3], rtol=0, atol=1e-9 * np.abs(s_d[:, :3]).max())
     assert np.allclose(s_t[:, 3:6], s_d[:, 3:6], rtol=1e-6, atol=1e-9 * np.abs(s_d[:, 3:6]).max())
     assert log_h.shape[0] >= 2 and np.allclose(s_h[:, :3], s_d[:, :3], rtol=0, atol=1e-4 * np.abs(s_d[:, :3]).max())
+
+
+def test_steps_run_randomised_octree_radial_table(tmp_path, oracle, steps):
+    """--bh 0.5 --bh-seed 5 with RADIAL_BH_FORCE_CORRECTION 1 (the reference's
+    USE_BH + RANDOMIZE_BH build, main.cc:791-1091): the table written to
+    OUT_DIR/radial_bh_force_table.dat is the per-shell mean radial octree
+    force of the IC (GLASS_FILE None) over RADIAL_BH_FORCE_TABLE_ITERATION
+    evaluations, each with its own draws of rand() seeded by the seed -- the
+    oracle's table from the same draws; the run then completes."""
+    print("case steps_run R3 randomised octree + radial table N=2048", flush=True)
+    w = wl.r3_compactified(N=2048, seed=24)
+    Rsim = float(np.sqrt((w.x ** 2).sum(axis=1)).max()) * 1.01
+    par, out = _write_run(tmp_path, w, 1.0 / 32, 0.0318, 0.0125, "bh_rand")
+    with open(par, "a") as f:
+        f.write(f"R_SIM\t{Rsim!r}\nRADIAL_BH_FORCE_CORRECTION\t1\nGLASS_FILE_FOR_BH_FORCE_CORRECTION\tNone\n"
+                "RADIAL_BH_FORCE_TABLE_SIZE\t16\nRADIAL_BH_FORCE_TABLE_ITERATION\t3\n")
+    exe = ROOT / "steps_amd" / "lib" / "steps_run"
+    r = subprocess.run([str(exe), str(par), "--bh", "0.5", "--bh-seed", "5", "--max-steps", "4"], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-400:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    got = np.loadtxt(out / "radial_bh_force_table.dat", comments="#")
+    assert got.shape == (16, 2)
+    assert np.allclose(got[:, 0], np.round((np.arange(16) + 0.5) * Rsim / 16, 6), rtol=0, atol=1e-9)
+    # the oracle's table from the same draws
+    libc = C.CDLL("libc.so.6")
+    libc.rand.restype = C.c_int
+    libc.srand.argtypes = [C.c_uint]
+    libc.srand(5)
+    H0 = float(repr(wl.H0 * UNIT_V)) / UNIT_V
+    misph = 4.0 * math.pi * (3.0 * H0 * H0 / (8.0 * math.pi)) * wl.OMEGA_M / 3.0
+    soft = steps.softening_lengths(w.M, w.particle_radius)[0]
+    p = oracle.params(0, cosmology=1, comoving=1, mass_in_unit_sphere=misph, Rsim=Rsim)
+    acc, cnt = np.zeros(16), np.zeros(16, dtype=np.int32)
+    for _ in range(3):
+        u = [libc.rand() / 2147483647 for _ in range(6)]
+        F = oracle.forces_bh_ex(p, 0.5, w.particle_radius, w.x, w.M, soft, u=u)
+        oracle.radial_bh_table_accumulate(0, Rsim, w.x, F, acc, cnt)
+    want = np.where(cnt > 0, acc / np.maximum(cnt, 1), 0.0)
+    assert (cnt > 0).sum() >= 12
+    assert np.allclose(got[:, 1], want, rtol=1e-10, atol=1e-15 + 1e-12 * np.abs(want).max())
+    log = np.loadtxt(out / "Logfile.dat")
+    assert log.shape[0] >= 2

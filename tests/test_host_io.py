@@ -27,7 +27,9 @@ class RunParams(C.Structure):
                 ("FIRST_T_OUT", C.c_double), ("IC_FORMAT", C.c_int32), ("OUTPUT_TIME_VARIABLE", C.c_int32),
                 ("OUT_LST", C.c_char * 1024), ("H_OUT", C.c_double), ("ParticleRadi", C.c_double),
                 ("MIN_REDSHIFT", C.c_double), ("REDSHIFT_CONE", C.c_int32), ("H0_INDEPENDENT_UNITS", C.c_int32),
-                ("Rsim", C.c_double), ("RADIAL_FORCE_ACCURACY", C.c_int32), ("RADIAL_FORCE_TABLE_SIZE", C.c_int32)]
+                ("Rsim", C.c_double), ("RADIAL_FORCE_ACCURACY", C.c_int32), ("RADIAL_FORCE_TABLE_SIZE", C.c_int32),
+                ("RADIAL_BH_FORCE_CORRECTION", C.c_int32), ("GLASS_FILE_FOR_BH_FORCE_CORRECTION", C.c_char * 1024),
+                ("RADIAL_BH_FORCE_TABLE_SIZE", C.c_int32), ("RADIAL_BH_FORCE_TABLE_ITERATION", C.c_int32)]
 
 
 @pytest.fixture(scope="module")
@@ -77,6 +79,12 @@ def test_example_cylindrical_file(host):
     assert p.h_max == pytest.approx(0.03125 / UNIT_T, rel=1e-15)
     assert (p.ACC_PARAM, p.ParticleRadi, p.MIN_REDSHIFT) == (0.0125, 0.108326, 0.0003012504)
     assert (p.FIRST_T_OUT, p.H_OUT, p.REDSHIFT_CONE, p.H0_INDEPENDENT_UNITS) == (1.0, 1.0, 0, 0)
+    # the octree build's keys (read_paramfile.cc:130-141, 348-379)
+    assert (p.RADIAL_BH_FORCE_CORRECTION, p.RADIAL_BH_FORCE_TABLE_SIZE, p.RADIAL_BH_FORCE_TABLE_ITERATION) == (1, 128, 16)
+    assert p.GLASS_FILE_FOR_BH_FORCE_CORRECTION == b"../../StePS_IC/examples/Glass_cylindrical_L100_D500.hdf5"
+    q = read(host, GOLD / "LCDM_SP_1860_com_VOI100.param", R3)
+    assert (q.RADIAL_BH_FORCE_CORRECTION, q.RADIAL_BH_FORCE_TABLE_SIZE, q.RADIAL_BH_FORCE_TABLE_ITERATION) == (1, 64, 7)
+    assert q.GLASS_FILE_FOR_BH_FORCE_CORRECTION == b"None"
 
 
 def test_example_files_parse_in_every_build(host):
@@ -120,11 +128,33 @@ def test_strstr_semantics_and_clamps(host, tmp_path):
     assert p.N_snapshot == 12 and p.ACC_PARAM == 16.0 and p.REDSHIFT_CONE == 8
     s = read(host, f, S1R2)
     assert s.IS_PERIODIC == 9 and s.RADIAL_FORCE_TABLE_SIZE == 5 and s.RADIAL_FORCE_ACCURACY == 10
+    # octree keys: defaults, then the clamps
+    assert (p.RADIAL_BH_FORCE_CORRECTION, p.RADIAL_BH_FORCE_TABLE_SIZE, p.RADIAL_BH_FORCE_TABLE_ITERATION) == (0, 64, 4)
+    assert p.GLASS_FILE_FOR_BH_FORCE_CORRECTION == b"None"
+    g = tmp_path / "bh.param"
+    g.write_text("RADIAL_BH_FORCE_CORRECTION 2\nRADIAL_BH_FORCE_TABLE_SIZE 3\nRADIAL_BH_FORCE_TABLE_ITERATION 0\n"
+                 "GLASS_FILE_FOR_BH_FORCE_CORRECTION glass.dat\n")
+    b = read(host, g, R3)
+    assert (b.RADIAL_BH_FORCE_CORRECTION, b.RADIAL_BH_FORCE_TABLE_SIZE, b.RADIAL_BH_FORCE_TABLE_ITERATION) == (0, 5, 1)
+    assert b.GLASS_FILE_FOR_BH_FORCE_CORRECTION == b"glass.dat"
     # H0-independent units convert L and PARTICLE_RADII with cosmology on
     f.write_text("COSMOLOGY 1\nHubbleConstant 67.66\nL_BOX 100\nPARTICLE_RADII 0.5\nH_INDEPENDENT_UNITS 1\n")
     q = read(host, f, T3)
     assert q.L == pytest.approx(100 / 0.6766, rel=1e-14) and q.ParticleRadi == pytest.approx(0.5 / 0.6766, rel=1e-14)
     assert host.steps_read_params(str(tmp_path / "missing.param").encode(), R3, C.byref(RunParams())) < 0
+
+
+def test_ascii_table_writer(host, tmp_path):
+    # save_function_to_ascii_table (inputoutput.cc:1038-1061): header line, then
+    # "%.6f\t%.15f" rows while the argument does not exceed x_max
+    dp = C.POINTER(C.c_double)
+    host.steps_write_ascii_table.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_double, dp, C.c_int32,
+                                             C.c_char_p]
+    v = np.array([0.1, -2.5, 3.25, 7.0])
+    f = tmp_path / "table.dat"
+    assert host.steps_write_ascii_table(str(f).encode(), 0.5, 3.0, 1.0, ptr(v), 4, b"# h1\n# h2") == 0
+    assert f.read_text() == ("# h1\n# h2\n0.500000\t0.100000000000000\n1.500000\t-2.500000000000000\n"
+                             "2.500000\t3.250000000000000\n")
 
 
 def test_ascii_ic_and_snapshot_round_trip(host, tmp_path):

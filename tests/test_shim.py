@@ -167,3 +167,87 @@ def test_shim_barnes_hut_builds(steps, oracle):
                           t.x, t.M, t.soft)
     report("shim T3 octree", rel_err(F, Fo))
     assert rel_err(F, Fo).max() <= 1e-12
+
+
+def shim_randomised_r3_errors():
+    """Three calls of the randomised R3 shim after srand(7), the third with the
+    radial correction; the largest per-row relative error of each against the
+    oracle with the draws rand() gives after srand(7).  Run in a process of
+    its own by the test below: the HIP runtime draws from rand() as it starts,
+    which must not shift the draws."""
+    import oracle_lib as oracle
+    libc = C.CDLL("libc.so.6")
+    libc.rand.restype = C.c_int
+    libc.srand.argtypes = [C.c_uint]
+    w = wl.r3_compactified(N=4096, seed=79)
+    Rsim = float(np.sqrt((w.x ** 2).sum(axis=1)).max())
+    tab = np.linspace(-2.0, 3.0, 16) * 1e-3 * w.mass_in_unit_sphere * Rsim
+    lib = shim("r3_bh_rand")
+    lib.harness_set_theta.argtypes = [C.c_double]
+    lib.harness_set_radial_bh.argtypes = [DP, C.c_int, C.c_int]
+    M, h, x = w.M.copy(), w.soft.copy(), w.x.reshape(-1).copy()
+    lib.harness_set_particles(w.N, p(M), p(h))
+    lib.harness_set_scalars(0, 1, 1, 0.0, Rsim, w.mass_in_unit_sphere, wl.H0, wl.OMEGA_L)
+    lib.harness_set_softening(w.particle_radius, float(w.M.min()))
+    lib.harness_set_theta(0.5)
+    libc.srand(7)
+    u = [[libc.rand() / 2147483647 for _ in range(6)] for _ in range(3)]  # glibc RAND_MAX
+    libc.srand(7)
+    op = oracle.params(0, cosmology=1, comoving=1, mass_in_unit_sphere=w.mass_in_unit_sphere, Rsim=Rsim)
+    errs = []
+    for k in range(3):
+        lib.harness_set_radial_bh(p(tab), tab.size, 1 if k == 2 else 0)
+        F = np.zeros(3 * w.N)
+        assert lib.harness_call(p(x), p(F), 0, w.N - 1) == 0
+        Fo = oracle.forces_bh_ex(op, 0.5, w.particle_radius, w.x, w.M, w.soft, u=u[k],
+                                 radial_bh=tab if k == 2 else None)
+        errs.append(float(rel_err(F, Fo).max()))
+    assert np.array_equal(x, w.x.reshape(-1))  # positions are left as they were
+    return errs
+
+
+@pytest.mark.gpu
+def test_shim_randomised_octree_and_radial_correction(steps, oracle):
+    """-DUSE_BH -DRANDOMIZE_BH: each call of the reference's force routine
+    draws its frame with rand() in the reference's order (six draws in R3,
+    four in S1R2); the harness seeds rand() as main.cc:310 does.  Once
+    USE_RADIAL_BH_CORRECTION is set, the table's correction is subtracted."""
+    import os
+    import subprocess
+    import sys
+    # a fresh process, whose first library call starts the HIP runtime
+    here = Path(__file__).resolve().parent
+    env = dict(os.environ, PYTHONPATH=f"{here.parent}{os.pathsep}{here}")
+    r = subprocess.run([sys.executable, "-c", "import test_shim; print(test_shim.shim_randomised_r3_errors())"],
+                       cwd=str(here), env=env, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    errs = [float(v) for v in r.stdout.strip().splitlines()[-1].strip("[]").split(",")]
+    report("shim randomised octree R3 (fresh process), max per call", np.array(errs))
+    assert max(errs) <= 5e-12
+    libc = C.CDLL("libc.so.6")
+    libc.rand.restype = C.c_int
+    libc.srand.argtypes = [C.c_uint]
+    rand_max = 2147483647
+    # S1R2 with the CIC table: four draws per call
+    c = wl.s1r2_cylinder(n_omega=12, n_phi=12, n_z=12, seed=6, is_periodic=2)
+    T, shape = oracle.s1r2_table(2, c.L, c.Rsim)
+    lib = shim("s1r2_bh_rand")
+    lib.harness_set_theta.argtypes = [C.c_double]
+    M, h, x = c.M.copy(), c.soft.copy(), c.x.reshape(-1).copy()
+    lib.harness_set_particles(c.N, p(M), p(h))
+    lib.harness_set_scalars(2, 1, 1, c.L, c.Rsim, c.mass_in_unit_sphere, 0.0, 0.0)
+    lib.harness_set_tables(None, 0, p(T), shape[0], shape[1], None, 0)
+    lib.harness_set_softening(c.particle_radius, float(c.M.min()))
+    lib.harness_set_theta(0.5)
+    libc.srand(11)
+    u = [libc.rand() / rand_max for _ in range(4)]
+    libc.srand(11)
+    F = np.zeros(3 * c.N)
+    assert lib.harness_call(p(x), p(F), 0, c.N - 1) == 0
+    Fo = oracle.forces_bh_ex(oracle.params(2, is_periodic=2, L=c.L, Rsim=c.Rsim, cosmology=1, comoving=1,
+                                           mass_in_unit_sphere=c.mass_in_unit_sphere, s1r2_table=T,
+                                           s1r2_shape=shape, interp_order=2),
+                             0.5, c.particle_radius, c.x, c.M, c.soft, u=u)
+    report("shim randomised octree S1R2 CIC", rel_err(F, Fo))
+    assert rel_err(F, Fo).max() <= 5e-12

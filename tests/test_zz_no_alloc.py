@@ -37,6 +37,10 @@ def cases(steps, oracle):
         out.append((f"r3 prec={prec}", d, keep, r3a, r3b))
     d, keep = steps.make_desc(steps.TOPO_R3, bh_theta=0.5, bh_particle_radius=r3a.particle_radius, **kw)
     out.append(("r3 barnes-hut", d, keep, r3a, r3b))
+    rtab = np.linspace(-1.0, 1.0, 32) * 1e-3 * r3a.mass_in_unit_sphere
+    d, keep = steps.make_desc(steps.TOPO_R3, bh_theta=0.5, bh_particle_radius=r3a.particle_radius, Rsim=50.0,
+                              bh_random=[0.3, 0.6, 0.2, 0.9, 0.4, 0.7], radial_bh_table=rtab, **kw)
+    out.append(("r3 barnes-hut randomised, radial correction", d, keep, r3a, r3b))
     ta, tb = wl.t3_lattice(n_side=16, seed=1001), wl.t3_lattice(n_side=10, seed=1002)
     T3 = steps.t3_table(2, ta.L)
     d, keep = steps.make_desc(steps.TOPO_T3, is_periodic=2, L=ta.L, t3_table=T3)
