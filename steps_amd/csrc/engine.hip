@@ -197,6 +197,8 @@ int make_plan(const steps_force_desc &d, int64_t N, int64_t id_min, int64_t id_m
                 return fail(STEPS_ERR_TABLE, "S1R2 Ewald table missing (or fewer than 2 rho nodes)");
             if (s1r2_device_nodes(d.s1r2_nrho, d.s1r2_nz) >= ((size_t)1 << 28))
                 return fail(STEPS_ERR_TABLE, "S1R2 table too large (2^28 nodes at most)");
+            if (d.s1r2_nz + 3 >= (1 << 24) || d.s1r2_nrho + 2 >= (1 << 24))  // 24-bit row offsets in the kernels
+                return fail(STEPS_ERR_TABLE, "S1R2 table dimensions must stay below 2^24 - 3 nodes");
             p.variant = d.interp_order == 0 ? PV_S1R2_NGP
                         : d.interp_order == 2 ? PV_S1R2_CIC
                                               : PV_S1R2_TSC;  // forces_cuda.cu:449-451 fallback

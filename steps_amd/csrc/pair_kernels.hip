@@ -321,8 +321,11 @@ __device__ __forceinline__ void t3_interp(const double *__restrict__ T, const KS
 // (periodic ghosts), each node (D_rho, D_z) as 16 bytes.  Node (ir, iz) sits
 // at ((ir + 1) * (Nz + 3) + iz + 1) * 16 bytes; s1r2_row(T, k, ir) + iz + 1
 // points at it.
+// Row offsets use the full-rate 24-bit multiply (v_mul_u32_u24; the 32-bit
+// v_mul_lo_u32 issues at a quarter of the rate): make_plan keeps Nrho + 2 and
+// Nz + 3 below 2^24, and the 2^28-node cap keeps the products in 32 bits.
 __device__ __forceinline__ const double2 *s1r2_row(const double *__restrict__ T, const KScalars &k, int ir) {
-    return (const double2 *)((const char *)T + (unsigned)(ir + 1) * (unsigned)(k.nz + 3) * 16u);
+    return (const double2 *)((const char *)T + __umul24((unsigned)(ir + 1), (unsigned)(k.nz + 3)) * 16u);
 }
 
 // A z index in [lo_ok, hi_ok] needs no wrap (the ghost rows cover it); any
@@ -466,6 +469,14 @@ struct S1R2Win {
 // it, else from the global table (the same values either way).  MS: the
 // source record carries -1.5 m instead of m (R3 and nearest-image variants;
 // the FP64 kernel's source copies, mr3_scaled).
+// One nearest-image wrap, d - L sign(d) when |d| > L/2 (forces_cuda.cu:
+// 622-627, 833-836; positions in [0, L) need no second one), as a multiply
+// by 0 or 1 in the FMA rather than two 64-bit selects: fma(-1, c, d) is the
+// subtraction d - c, fma(-0, c, d) is d.
+__device__ __forceinline__ double wrap_half(double d, const KScalars &k) {
+    return fma(fabs(d) > k.halfL ? -1.0 : 0.0, copysign(k.L, d), d);
+}
+
 template <int V, bool WIN = false, bool MS = false>
 __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__restrict__ table,
                                          const double *__restrict__ soft, const int64_t j,
@@ -481,9 +492,9 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
         fz = fma(s, dz, fz);
     } else if (V == PV_T3_NEAREST || V == PV_T3_EWALD) {
         // forces_cuda.cu:622-627: d - L*d/|d| when |d| > L/2 (d/|d| is exactly +-1)
-        if (fabs(dx) > k.halfL) dx -= copysign(k.L, dx);
-        if (fabs(dy) > k.halfL) dy -= copysign(k.L, dy);
-        if (fabs(dz) > k.halfL) dz -= copysign(k.L, dz);
+        dx = wrap_half(dx, k);
+        dy = wrap_half(dy, k);
+        dz = wrap_half(dz, k);
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
         if (V == PV_T3_NEAREST) {
             const double s = MS ? pair_s_tile(r2, B, hi, pj.w, soft, j, k) : pj.w * pair_w_tile(r2, B, hi, soft, j, k);
@@ -516,8 +527,7 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
         fz += sz;
     } else {
         // forces_cuda.cu:833-836: nearest z-image only
-        if (dz > k.halfL) dz -= k.L;
-        else if (dz < -k.halfL) dz += k.L;
+        dz = wrap_half(dz, k);
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
         if (V == PV_S1R2_NEAREST) {
             const double s = MS ? pair_s_tile(r2, B, hi, pj.w, soft, j, k) : pj.w * pair_w_tile(r2, B, hi, soft, j, k);
@@ -538,7 +548,8 @@ __device__ __forceinline__ void pair_f64(const KScalars &k, const double *__rest
             }
             if (in) {
                 const unsigned nz3 = (unsigned)(k.nz + 3);
-                s1r2_gather<ORD>([&](int ir) { return sw.win + (unsigned)(ir - sw.w0) * nz3; }, k, rho, uz, Dr, Dzv);
+                s1r2_gather<ORD>([&](int ir) { return sw.win + __umul24((unsigned)(ir - sw.w0), nz3); }, k, rho, uz, Dr,
+                                 Dzv);
             } else {
                 s1r2_gather<ORD>([&](int ir) { return s1r2_row(table, k, ir); }, k, rho, uz, Dr, Dzv);
             }
@@ -949,7 +960,7 @@ constexpr int s1r2_order(int V) { return V == PV_S1R2_CIC ? 2 : 4; }
 
 // FP32 copy of the device layout (s1r2_row): same rows, 8-byte nodes.
 __device__ __forceinline__ const float2 *s1r2_row32(const float2 *__restrict__ T, const KScalars &k, int ir) {
-    return (const float2 *)((const char *)T + (unsigned)(ir + 1) * (unsigned)(k.nz + 3) * 8u);
+    return (const float2 *)((const char *)T + __umul24((unsigned)(ir + 1), (unsigned)(k.nz + 3)) * 8u);
 }
 
 // s1r2_interp in FP32 (CIC, TSC): returns q = D_rho / rho and D_z.
@@ -1019,7 +1030,7 @@ __device__ __forceinline__ void s1r2_interp_f32(const float2 *__restrict__ T, co
     }
     if (in) {
         const unsigned nz3 = (unsigned)(k.nz + 3);
-        gather([&](int ir) { return sw.win + (unsigned)(ir - sw.w0) * nz3; });
+        gather([&](int ir) { return sw.win + __umul24((unsigned)(ir - sw.w0), nz3); });
     } else {
         gather([&](int ir) { return s1r2_row32(T, k, ir); });
     }
