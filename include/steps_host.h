@@ -78,6 +78,9 @@ typedef struct steps_run_params {
     char GLASS_FILE_FOR_BH_FORCE_CORRECTION[STEPS_PATH_MAX];
     int32_t RADIAL_BH_FORCE_TABLE_SIZE;
     int32_t RADIAL_BH_FORCE_TABLE_ITERATION;
+    double w0, wa;                  /* COSMOPARAM 1 and 2 builds */
+    char EXPANSION_FILE[STEPS_PATH_MAX];  /* COSMOPARAM -1 build */
+    int32_t INTERPOLATION_ORDER;    /* COSMOPARAM -1: 1..3, anything else reads as 1 */
 } steps_run_params;
 
 const char *steps_host_last_error(void);
@@ -109,6 +112,31 @@ int steps_read_gadget_ic(const char *path, int64_t *N, double *x, double *v, dou
  * OUTPUT_TIME_VARIABLE; values are returned in file order.  With out == NULL
  * only the count is returned (>= 0). */
 int64_t steps_read_output_list(const char *path, double *out, int64_t cap);
+
+/* The expansion history (friedmann_solver.cc) of the COSMOPARAM builds:
+ * 0 LCDM, 1 wCDM (w0), 2 w0-wa (w = w0 + wa (1 - a)), -1 a tabulated history
+ * `table` of `rows` rows (t, a, H) in internal units, interpolated with
+ * `order` 1, 2 or 3 and a cursor `index` the calls move (expansion_index).
+ * Omega_k = 1 - Omega_m - Omega_lambda - Omega_r (main.cc:1261). */
+typedef struct steps_expansion {
+    int32_t cosmoparam;
+    double H0, Omega_m, Omega_r, Omega_lambda, Omega_k, w0, wa;
+    const double *table;
+    int64_t rows;
+    int32_t order;
+    int64_t index;
+} steps_expansion;
+/* CALCULATE_Hubble_param */
+double steps_hubble(const steps_expansion *e, double a);
+/* friedmann_solver_step: a after one RK4 step of length h from a0 (the
+ * tabulated history: a at the time T at the end of the step) */
+double steps_friedmann_step(steps_expansion *e, double a0, double h, double T);
+/* friedmann_solver_start: the cosmic time of a_start (RK4 step h backwards
+ * to the big bang, halved until converged) */
+double steps_friedmann_start(steps_expansion *e, double a_start, double h);
+/* read_expansion_history: rows (t Gy, a, H km/s/Mpc) into table [3 rows] in
+ * internal units; table == NULL returns the row count */
+int64_t steps_read_expansion_history(const char *path, double *table, int64_t cap);
 
 /* save_function_to_ascii_table (inputoutput.cc:1038-1061): the header line,
  * then "%.6f\t%.15f" rows of (x_min + k deltax, values[k]) while the

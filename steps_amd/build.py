@@ -94,7 +94,8 @@ def build_host(verbose: bool = False) -> None:
     common = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", f"-I{host}"]
     lib = LIBDIR / "libsteps_host.so"
     tmp = LIBDIR / "libsteps_host.so.tmp"
-    _run([*common, "-fPIC", "-shared", str(host / "params.cc"), str(host / "snapio.cc"), "-o", str(tmp)], verbose)
+    _run([*common, "-fPIC", "-shared", str(host / "params.cc"), str(host / "snapio.cc"), str(host / "cosmo.cc"),
+          "-o", str(tmp)], verbose)
     os.replace(tmp, lib)
     exe = LIBDIR / "steps_run"
     tmp = LIBDIR / "steps_run.tmp"

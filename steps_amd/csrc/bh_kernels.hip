@@ -10,14 +10,29 @@
 // centre), so the tree's internal nodes are exactly the cubes that hold two
 // or more particles, at every level, and its leaves the particles.  Here the
 // same tree is built in parallel:
-//   * each particle's 21 octant digits are computed with the reference's own
-//     centre arithmetic (c +- side/4, side/2 per level) into a 63-bit key;
-//   * the keys are sorted; adjacent sorted particles share lcp[p] leading
-//     digits, so the internal nodes that start at particle p are the cubes of
+//   * each particle's 42 octant digits are computed with the reference's own
+//     centre arithmetic (c +- side/4, side/2 per level) into two 63-bit keys;
+//   * the particles are sorted by the first key (LSD radix), then each run
+//     of equal first keys (particles closer than 2^-21 of the root side,
+//     rare) by the second: a stable sort by the 126-bit key; adjacent sorted
+//     particles share lcp[p]
+//     leading digits, so the cubes that start at particle p are those of
 //     levels lcp[p-1]+1 .. lcp[p] and, followed by p's leaf, they list the
 //     tree in pre-order with children in octant order (the reference's loop
-//     order); a node of level l starting at p ends at the first b >= p with
-//     lcp[b] < l (binary lifting over a range-minimum table of lcp);
+//     order); a cube of level l starting at p ends at the last b >= p whose
+//     key shares l digits with p's (an exponential, then binary, search over
+//     the sorted keys: 2 log2 of the cube's particle count steps);
+//   * a cube whose particles all lie in one octant (a chain: one non-empty
+//     child, the same particles a level down) becomes one item with its
+//     deepest cube: the cubes starting at p are then only the levels lcp[p],
+//     lcp[e1], lcp[e2], ... above lcp[p-1], e_k the end of the k-th (each
+//     level the value of lcp that ends the one below).  The chain's cubes
+//     share particles, mass, centre of mass and softening, and their sides
+//     halve exactly, so side / dist < theta holds at some cube of the chain
+//     exactly when it holds at the deepest: the reference takes the chain's
+//     particles as one interaction or opens the deepest cube, as the item
+//     does.  Every item then has two or more children, and the N leaves have
+//     at most N - 1 items above them;
 //   * every item gets a skip link: the item after its subtree;
 //   * node masses and centres of mass are summed bottom-up, level by level.
 // The traversal makes the reference's decision per (row, node): a leaf, or a
@@ -29,12 +44,11 @@
 // (bh_force_kernel), so the set of interactions is the reference's and only
 // the rounding of the centres of mass and of the sums differs.
 //
-// Particles closer than 2^-21 of the root side share one level-21 node whose
+// Particles closer than 2^-42 of the root side share one level-42 node whose
 // children are their leaves (the reference splits further, which changes the
-// force only for rows within a few 2^-21 root sides of such a group).
-// At each of the 22 levels the cubes holding two or more particles are
-// disjoint, so there are at most 22 * floor(N/2) internal nodes: the item
-// arrays are sized for N + 11 N and cannot overflow.
+// force only for rows within a few 2^-42 root sides of such a group; at
+// 2^-21, 22 of the compactified R^3 particles at N = 2^21 would share).
+// The item arrays hold 2N items and cannot overflow.
 #include "internal.h"
 
 #include <algorithm>
@@ -43,18 +57,12 @@
 namespace steps {
 namespace {
 
-constexpr int kLevels = 21;  // octree levels below the root resolved by a 63-bit key
+constexpr int kKeyLevels = 21;             // octant digits in one 63-bit key
+constexpr int kLevels = 2 * kKeyLevels;  // octree levels below the root resolved by the two keys
 constexpr int kRedBlocks = 256;
 constexpr int kScanItems = 1024;  // counts per workgroup of the item scan
 
 size_t a256(size_t b) { return (b + 255) & ~(size_t)255; }
-
-// range-minimum table rows: spans 2^0 .. 2^(rows-1) >= n
-int table_rows(int64_t n) {
-    int r = 1;
-    while (((int64_t)1 << (r - 1)) < n) ++r;
-    return r;
-}
 
 int64_t scan_blocks(int64_t n) { return std::max<int64_t>(1, (n + kScanItems - 1) / kScanItems); }
 
@@ -209,75 +217,156 @@ __global__ __launch_bounds__(256) void bh_reduce_final(const double *__restrict_
     }
 }
 
-// The 21 octant digits of each particle, most significant first: at each
+// The 42 octant digits of each particle, most significant first: at each
 // level the reference's test x > cx (bit 0), y > cy (bit 1), z > cz (bit 2)
 // against the cube centre, then the child's centre c +- side/4 and side/2
 // (forces.cc:121-128, 147-155), in the same floating-point operations.
+// Levels 1-21 go into the first key (its high and low halves k0, k1),
+// 22-42 into the second (k2).
 __global__ __launch_bounds__(256) void bh_keys_kernel(const double4 *__restrict__ rec, int64_t n, BhFrame fr,
-                                                      const double *__restrict__ red, uint32_t *__restrict__ khi,
-                                                      uint32_t *__restrict__ klo) {
+                                                      const double *__restrict__ red, uint32_t *__restrict__ k0,
+                                                      uint32_t *__restrict__ k1, uint64_t *__restrict__ k2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double4 p = bh_to_frame(fr, rec[i]);
     double cx = red[2], cy = red[3], cz = red[4], side = red[0];
-    uint64_t key = 0;
-    for (int l = 0; l < kLevels; ++l) {
-        const bool bx = p.x > cx, by = p.y > cy, bz = p.z > cz;
-        key = (key << 3) | (bx ? 1u : 0u) | (by ? 2u : 0u) | (bz ? 4u : 0u);
-        const double off = side / 4;
-        cx = cx + (bx ? off : -off);
-        cy = cy + (by ? off : -off);
-        cz = cz + (bz ? off : -off);
-        side = side / 2;
+    uint64_t key[2] = {0, 0};
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+        for (int l = 0; l < kKeyLevels; ++l) {
+            const bool bx = p.x > cx, by = p.y > cy, bz = p.z > cz;
+            key[h] = (key[h] << 3) | (bx ? 1u : 0u) | (by ? 2u : 0u) | (bz ? 4u : 0u);
+            const double off = side / 4;
+            cx = cx + (bx ? off : -off);
+            cy = cy + (by ? off : -off);
+            cz = cz + (bz ? off : -off);
+            side = side / 2;
+        }
+    k0[i] = (uint32_t)(key[0] >> 32);
+    k1[i] = (uint32_t)key[0];
+    k2[i] = key[1];
+}
+
+// The sort by the first key completed to a stable sort by both: a particle
+// whose first key is shared moves to the start of its run plus its rank in
+// the run by (second key, position), and its record is written there again
+// (the gather placed the run's records in first-key order).  A run of s
+// particles costs s^2 comparisons (all N particles inside one 2^-21 cube:
+// ~10^-1 s at N = 2^21).
+__global__ __launch_bounds__(256) void bh_refine_kernel(const double4 *__restrict__ rec, BhFrame fr,
+                                                        const double *__restrict__ soft,
+                                                        const uint64_t *__restrict__ k2,
+                                                        const uint64_t *__restrict__ skey,
+                                                        const uint32_t *__restrict__ perm, int64_t n,
+                                                        double4 *__restrict__ srec, double *__restrict__ ssoft,
+                                                        uint64_t *__restrict__ skey2, uint32_t *__restrict__ sidx) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint64_t k = skey[p];
+    const bool shared = (p > 0 && skey[p - 1] == k) || (p + 1 < n && skey[p + 1] == k);
+    if (!shared) return;
+    int64_t s = p, e = p;
+    while (s > 0 && skey[s - 1] == k) --s;
+    while (e + 1 < n && skey[e + 1] == k) ++e;
+    const uint32_t me = perm[p];
+    const uint64_t m = k2[me];
+    int64_t rank = 0;
+    for (int64_t q = s; q <= e; ++q) {
+        const uint64_t o = k2[perm[q]];
+        rank += (o < m || (o == m && q < p)) ? 1 : 0;
     }
-    khi[i] = (uint32_t)(key >> 32);
-    klo[i] = (uint32_t)key;
+    const int64_t d = s + rank;
+    srec[d] = bh_to_frame(fr, rec[me]);
+    ssoft[d] = soft[me];
+    skey2[d] = m;
+    sidx[d] = me;
 }
 
 // sorted records, in the octree's frame
 __global__ __launch_bounds__(256) void bh_gather_kernel(const double4 *__restrict__ rec, BhFrame fr,
                                                         const double *__restrict__ soft,
-                                                        const uint32_t *__restrict__ khi,
-                                                        const uint32_t *__restrict__ klo,
+                                                        const uint32_t *__restrict__ k0,
+                                                        const uint32_t *__restrict__ k1,
+                                                        const uint64_t *__restrict__ k2,
                                                         const uint32_t *__restrict__ perm, int64_t n,
                                                         double4 *__restrict__ srec, double *__restrict__ ssoft,
-                                                        uint64_t *__restrict__ skey, uint32_t *__restrict__ sidx) {
+                                                        uint64_t *__restrict__ skey, uint64_t *__restrict__ skey2,
+                                                        uint32_t *__restrict__ sidx) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = perm[i];
     srec[i] = bh_to_frame(fr, rec[s]);
     ssoft[i] = soft[s];
-    skey[i] = ((uint64_t)khi[s] << 32) | klo[s];
+    skey[i] = ((uint64_t)k0[s] << 32) | k1[s];
+    skey2[i] = k2[s];
     sidx[i] = s;
 }
 
 // lcp[p]: the leading octant digits sorted particles p and p+1 share
-// (0..21); -1 after the last particle.  Row 0 of the range-minimum table.
-__global__ __launch_bounds__(256) void bh_lcp_kernel(const uint64_t *__restrict__ skey, int64_t n,
+// (0..42); -1 after the last particle.
+__global__ __launch_bounds__(256) void bh_lcp_kernel(const uint64_t *__restrict__ skey,
+                                                     const uint64_t *__restrict__ skey2, int64_t n,
                                                      int8_t *__restrict__ lcp) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     int v = -1;
     if (p + 1 < n) {
-        const uint64_t x = skey[p] ^ skey[p + 1];
-        v = (x == 0) ? kLevels : (__clzll((long long)x) - 1) / 3;  // keys use bits 0..62
+        const uint64_t x = skey[p] ^ skey[p + 1];  // keys use bits 0..62
+        if (x != 0) {
+            v = (__clzll((long long)x) - 1) / 3;
+        } else {
+            const uint64_t x2 = skey2[p] ^ skey2[p + 1];
+            v = kKeyLevels + ((x2 == 0) ? kKeyLevels : (__clzll((long long)x2) - 1) / 3);
+        }
     }
     lcp[p] = (int8_t)v;
 }
 
-// row k of the range-minimum table: min(lcp[j .. j + 2^k - 1]), the range
-// clamped at the last particle (whose lcp is -1)
-__global__ __launch_bounds__(256) void bh_table_kernel(int8_t *__restrict__ tab, int64_t n, int k) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int8_t *prev = tab + (size_t)(k - 1) * n;
-    const int64_t j2 = min(j + ((int64_t)1 << (k - 1)), n - 1);
-    tab[(size_t)k * n + j] = min(prev[j], prev[j2]);
+// The leading octant digits sorted particles a and b share (0..42).
+__device__ __forceinline__ int bh_shared_digits(const uint64_t *__restrict__ skey,
+                                                const uint64_t *__restrict__ skey2, int64_t a, int64_t b) {
+    const uint64_t x = skey[a] ^ skey[b];  // keys use bits 0..62
+    if (x != 0) return (__clzll((long long)x) - 1) / 3;
+    const uint64_t x2 = skey2[a] ^ skey2[b];
+    return kKeyLevels + ((x2 == 0) ? kKeyLevels : (__clzll((long long)x2) - 1) / 3);
+}
+
+// The last particle of the cube of level l that starts at p, searched from a
+// particle b >= p known to lie in it: the shared digits with p do not grow
+// along the sorted keys, so an exponential search past b, then a binary
+// search back, finds the last particle sharing l digits with p.
+__device__ __forceinline__ int64_t bh_cube_end(const uint64_t *__restrict__ skey, const uint64_t *__restrict__ skey2,
+                                               int64_t n, int64_t p, int64_t b, int l) {
+    int64_t step = 1;
+    while (b + step < n && bh_shared_digits(skey, skey2, p, b + step) >= l) {
+        b += step;
+        step <<= 1;
+    }
+    for (step >>= 1; step > 0; step >>= 1)
+        if (b + step < n && bh_shared_digits(skey, skey2, p, b + step) >= l) b += step;
+    return b;
+}
+
+// Items that start at sorted particle p: the leaf, and one per level of
+// lcp[p], lcp[e1], lcp[e2], ... above lcp[p-1] (e_k the end of the k-th;
+// the levels strictly decrease, so at most 43 steps).
+__device__ __forceinline__ uint32_t bh_items_at(const int8_t *__restrict__ lcp, const uint64_t *__restrict__ skey,
+                                                const uint64_t *__restrict__ skey2, int64_t n, int64_t p) {
+    const int lp = (p > 0) ? lcp[p - 1] : -1;
+    uint32_t c = 1;
+    int64_t b = p;
+    for (int v = lcp[p]; v > lp; v = lcp[b]) {
+        ++c;
+        b = bh_cube_end(skey, skey2, n, p, b, v);
+    }
+    return c;
 }
 
 // Items (internal nodes + the leaf) that start at each sorted particle, and
 // their exclusive scan within each workgroup of kScanItems particles.
-__global__ __launch_bounds__(256) void bh_count_scan_kernel(const int8_t *__restrict__ lcp, int64_t n,
+__global__ __launch_bounds__(256) void bh_count_scan_kernel(const int8_t *__restrict__ lcp,
+                                                            const uint64_t *__restrict__ skey,
+                                                            const uint64_t *__restrict__ skey2, int64_t n,
                                                             uint32_t *__restrict__ base,
                                                             uint32_t *__restrict__ bsum) {
     __shared__ uint32_t s[256];
@@ -288,11 +377,7 @@ __global__ __launch_bounds__(256) void bh_count_scan_kernel(const int8_t *__rest
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
         const int64_t p = p0 + q;
-        c[q] = 0;
-        if (p < n) {
-            const int lp = (p > 0) ? lcp[p - 1] : -1, lc = lcp[p];
-            c[q] = (uint32_t)(max(0, lc - lp) + 1);
-        }
+        c[q] = (p < n) ? bh_items_at(lcp, skey, skey2, n, p) : 0u;
         sum += c[q];
     }
     s[t] = sum;
@@ -357,9 +442,12 @@ struct __align__(64) BhNode {
 };
 static_assert(sizeof(BhNode) == 64, "one 64-byte record per item");
 
-// Every item of the pre-order list: the internal nodes starting at p, then
-// p's leaf.  level[] holds an internal node's level, -1 for a leaf.
-__global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__ tab, int rows,
+// Every item of the pre-order list: the internal nodes starting at p
+// (shallowest first), then p's leaf.  level[] holds an internal node's level
+// (its deepest cube), -1 for a leaf.
+__global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__ lcp,
+                                                      const uint64_t *__restrict__ skey,
+                                                      const uint64_t *__restrict__ skey2,
                                                       const uint32_t *__restrict__ base,
                                                       const double4 *__restrict__ srec,
                                                       const uint32_t *__restrict__ sidx, int64_t n,
@@ -369,22 +457,20 @@ __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int total = info[0];
-    const int8_t *lcp = tab;
-    const int lp = (p > 0) ? lcp[p - 1] : -1, lc = lcp[p];
+    const int lp = (p > 0) ? lcp[p - 1] : -1;
     const double root = red[0], mmin = red[1];
-    int item = (int)base[p];
-    for (int l = lp + 1; l <= lc; ++l, ++item) {
-        // the node's last particle: the first j >= p with lcp[j] < l
-        int64_t j = p;
-        for (int k = rows - 1; k >= 0; --k)
-            if (tab[(size_t)k * n + j] >= l) j = min(j + ((int64_t)1 << k), n - 1);
+    const int leaf = ((p + 1 < n) ? (int)base[p + 1] : total) - 1;
+    int item = leaf - 1;  // the deepest internal node starting at p, then up
+    int64_t b = p;
+    for (int v = lcp[p]; v > lp; v = lcp[b], --item) {
+        b = bh_cube_end(skey, skey2, n, p, b, v);  // the node's last particle
         double side = root;
-        for (int q = 0; q < l; ++q) side = side / 2;  // forces.cc:155, level by level
+        for (int q = 0; q < v; ++q) side = side / 2;  // forces.cc:155, level by level
         BhNode &o = nodes[item];
         o.side = side;
-        o.next = (j + 1 < n) ? (int)base[j + 1] : total;
+        o.next = (b + 1 < n) ? (int)base[b + 1] : total;
         o.pidx = -1;
-        level[item] = (int8_t)l;
+        level[item] = (int8_t)v;
     }
     const double4 r = srec[p];
     BhNode o;
@@ -397,8 +483,8 @@ __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__
     o.next = (p + 1 < n) ? (int)base[p + 1] : total;
     o.pidx = (int)sidx[p];
     o.pad[0] = o.pad[1] = 0;
-    nodes[item] = o;
-    level[item] = -1;
+    nodes[leaf] = o;
+    level[leaf] = -1;
 }
 
 // The internal nodes of one level: mass and centre of mass from their direct
@@ -813,12 +899,14 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const double4 *__restrict
 
 struct BhView {
     double *red;  // kRedBlocks partials (max radius, min m), then root side, M_min, root centre
-    uint32_t *khi, *klo, *base, *bsum;
+    uint32_t *k0, *k1;  // high and low halves of the first key
+    uint64_t *k2;       // the second key
+    uint32_t *base, *bsum;
     double4 *srec;
     double *ssoft;
-    uint64_t *skey;
+    uint64_t *skey, *skey2;
     uint32_t *sidx;
-    int8_t *tab;
+    int8_t *lcp;
     int *info;
     BhNode *nodes;
     int8_t *level;
@@ -829,15 +917,17 @@ struct BhView {
 template <class Take> void bh_pieces(int64_t n, Take &&take) {
     const size_t nn = (size_t)std::max<int64_t>(n, 1), cap = (size_t)bh_item_capacity(n);
     take((size_t)(2 * kRedBlocks + 5) * 8);  // red
-    take(nn * 4);                             // khi
-    take(nn * 4);                             // klo
+    take(nn * 4);                             // k0
+    take(nn * 4);                             // k1
+    take(nn * 8);                             // k2
     take(nn * 4);                             // base
     take((size_t)scan_blocks(n) * 4);         // bsum
     take(nn * 32);                            // srec
     take(nn * 8);                             // ssoft
     take(nn * 8);                             // skey
+    take(nn * 8);                             // skey2
     take(nn * 4);                             // sidx
-    take(nn * (size_t)table_rows(n));         // tab
+    take(nn);                                 // lcp
     take(16);                                 // info
     take(cap * sizeof(BhNode));               // nodes
     take(cap);                                // level
@@ -846,7 +936,7 @@ template <class Take> void bh_pieces(int64_t n, Take &&take) {
 
 BhView bh_view(void *scratch, int64_t n) {
     char *base = (char *)scratch;
-    char *ptr[14];
+    char *ptr[16];
     int k = 0;
     size_t off = 0;
     bh_pieces(n, [&](size_t b) {
@@ -855,19 +945,21 @@ BhView bh_view(void *scratch, int64_t n) {
     });
     BhView v;
     v.red = (double *)ptr[0];
-    v.khi = (uint32_t *)ptr[1];
-    v.klo = (uint32_t *)ptr[2];
-    v.base = (uint32_t *)ptr[3];
-    v.bsum = (uint32_t *)ptr[4];
-    v.srec = (double4 *)ptr[5];
-    v.ssoft = (double *)ptr[6];
-    v.skey = (uint64_t *)ptr[7];
-    v.sidx = (uint32_t *)ptr[8];
-    v.tab = (int8_t *)ptr[9];
-    v.info = (int *)ptr[10];
-    v.nodes = (BhNode *)ptr[11];
-    v.level = (int8_t *)ptr[12];
-    v.sort = ptr[13];
+    v.k0 = (uint32_t *)ptr[1];
+    v.k1 = (uint32_t *)ptr[2];
+    v.k2 = (uint64_t *)ptr[3];
+    v.base = (uint32_t *)ptr[4];
+    v.bsum = (uint32_t *)ptr[5];
+    v.srec = (double4 *)ptr[6];
+    v.ssoft = (double *)ptr[7];
+    v.skey = (uint64_t *)ptr[8];
+    v.skey2 = (uint64_t *)ptr[9];
+    v.sidx = (uint32_t *)ptr[10];
+    v.lcp = (int8_t *)ptr[11];
+    v.info = (int *)ptr[12];
+    v.nodes = (BhNode *)ptr[13];
+    v.level = (int8_t *)ptr[14];
+    v.sort = ptr[15];
     return v;
 }
 
@@ -921,7 +1013,7 @@ BhFrame bh_frame(const EvalPlan &p, int topology) {
 
 int64_t bh_item_capacity(int64_t n) {
     const int64_t nn = std::max<int64_t>(n, 1);
-    return nn + (int64_t)(kLevels + 1) * (nn / 2) + 64;
+    return 2 * nn + 64;  // N leaves and at most N - 1 internal nodes
 }
 
 size_t bh_scratch_bytes(int64_t n) {
@@ -938,7 +1030,6 @@ hipError_t launch_bh_forces(const EvalPlan &p, const DeviceParticles &dp, void *
     const BhView v = bh_view(scratch, n);
     const unsigned ng = (unsigned)((n + 255) / 256);
     const int nb = (int)std::min<int64_t>(kRedBlocks, (n + 255) / 256);
-    const int rows = table_rows(n);
     hipError_t e;
     double *red = v.red + 2 * kRedBlocks;  // root side, M_min, root centre
     const int topology = (p.variant == PV_R3)                                     ? STEPS_TOPO_R3
@@ -947,20 +1038,22 @@ hipError_t launch_bh_forces(const EvalPlan &p, const DeviceParticles &dp, void *
     const BhFrame fr = bh_frame(p, topology);
     hipLaunchKernelGGL(bh_reduce_partial, dim3(nb), dim3(256), 0, st, dp.jrec, n, topology == STEPS_TOPO_S1R2, v.red);
     hipLaunchKernelGGL(bh_reduce_final, dim3(1), dim3(256), 0, st, v.red, nb, topology, p.L, fr, red);
-    hipLaunchKernelGGL(bh_keys_kernel, dim3(ng), dim3(256), 0, st, dp.jrec, n, fr, red, v.khi, v.klo);
+    hipLaunchKernelGGL(bh_keys_kernel, dim3(ng), dim3(256), 0, st, dp.jrec, n, fr, red, v.k0, v.k1, v.k2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const uint32_t *perm = nullptr;
-    if ((e = launch_sort_u64(v.khi, v.klo, n, v.sort, &perm, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(bh_gather_kernel, dim3(ng), dim3(256), 0, st, dp.jrec, fr, dp.soft, v.khi, v.klo, perm, n, v.srec,
-                       v.ssoft, v.skey, v.sidx);
-    hipLaunchKernelGGL(bh_lcp_kernel, dim3(ng), dim3(256), 0, st, v.skey, n, v.tab);
-    for (int k = 1; k < rows; ++k) hipLaunchKernelGGL(bh_table_kernel, dim3(ng), dim3(256), 0, st, v.tab, n, k);
+    const uint32_t *perm1 = nullptr;
+    if ((e = launch_sort_u64(v.k0, v.k1, n, v.sort, &perm1, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(bh_gather_kernel, dim3(ng), dim3(256), 0, st, dp.jrec, fr, dp.soft, v.k0, v.k1, v.k2, perm1, n,
+                       v.srec, v.ssoft, v.skey, v.skey2, v.sidx);
+    hipLaunchKernelGGL(bh_refine_kernel, dim3(ng), dim3(256), 0, st, dp.jrec, fr, dp.soft, v.k2, v.skey, perm1, n,
+                       v.srec, v.ssoft, v.skey2, v.sidx);
+    hipLaunchKernelGGL(bh_lcp_kernel, dim3(ng), dim3(256), 0, st, v.skey, v.skey2, n, v.lcp);
     const int64_t sb = scan_blocks(n);
-    hipLaunchKernelGGL(bh_count_scan_kernel, dim3((unsigned)sb), dim3(256), 0, st, v.tab, n, v.base, v.bsum);
+    hipLaunchKernelGGL(bh_count_scan_kernel, dim3((unsigned)sb), dim3(256), 0, st, v.lcp, v.skey, v.skey2, n, v.base,
+                       v.bsum);
     hipLaunchKernelGGL(bh_scan_top_kernel, dim3(1), dim3(1024), 0, st, v.bsum, sb, v.info);
     hipLaunchKernelGGL(bh_scan_add_kernel, dim3(ng), dim3(256), 0, st, v.base, n, v.bsum);
-    hipLaunchKernelGGL(bh_fill_kernel, dim3(ng), dim3(256), 0, st, v.tab, rows, v.base, v.srec, v.sidx, n, red,
-                       p.bh_particle_radius, v.info, v.nodes, v.level);
+    hipLaunchKernelGGL(bh_fill_kernel, dim3(ng), dim3(256), 0, st, v.lcp, v.skey, v.skey2, v.base, v.srec, v.sidx, n,
+                       red, p.bh_particle_radius, v.info, v.nodes, v.level);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // the items of one level are few next to the grid: a grid-stride loop over
     // the items the tree has (info[0], known only on the device)

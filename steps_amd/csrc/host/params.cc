@@ -63,6 +63,12 @@ const Key kKeys[] = {
     {"GLASS_FILE_FOR_BH_FORCE_CORRECTION", KSTR, F(GLASS_FILE_FOR_BH_FORCE_CORRECTION), false},
     {"RADIAL_BH_FORCE_TABLE_SIZE", KINT, F(RADIAL_BH_FORCE_TABLE_SIZE), false},
     {"RADIAL_BH_FORCE_TABLE_ITERATION", KINT, F(RADIAL_BH_FORCE_TABLE_ITERATION), false},
+    // the COSMOPARAM builds' keys (read_paramfile.cc:115-121), read here whatever
+    // the expansion history steps_run is given
+    {"w0", KDBL, F(w0), false},
+    {"wa", KDBL, F(wa), false},
+    {"EXPANSION_FILE", KSTR, F(EXPANSION_FILE), false},
+    {"INTERPOLATION_ORDER", KINT, F(INTERPOLATION_ORDER), false},
 };
 #undef F
 
@@ -148,6 +154,8 @@ extern "C" int steps_read_params(const char *path, int32_t topology, steps_run_p
         }
     }
     std::fclose(f);
+    // read_paramfile.cc:627-631
+    if (p->INTERPOLATION_ORDER < 1 || p->INTERPOLATION_ORDER > 3) p->INTERPOLATION_ORDER = 1;
     // OUT_DIR ends with a '/' (read_paramfile.cc:205-210 appends one when the
     // value lacks it)
     const size_t n = std::strlen(p->OUT_DIR);

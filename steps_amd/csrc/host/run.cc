@@ -3,13 +3,15 @@
 //
 //   steps_run <parameter file> [--topology r3|t3|s1r2] [--precision fp64|mixed]
 //             [--interp-order 0|2|4|6|8] [--no-lookup] [--bh THETA]
-//             [--bh-seed SEED] [--max-steps K] [--device D]
+//             [--bh-seed SEED] [--cosmo-param 0|1|2|-1] [--max-steps K] [--device D]
 //
 // The topology, the interpolation order, the table / no-table mode, the
 // Barnes-Hut octree with its opening angle and its randomisation are build
 // flags in the reference (PERIODIC, PERIODIC_Z, EWALD_INTERPOLATION_ORDER,
 // PERIODIC_Z_NOLOOKUP, USE_BH=THETA, RANDOMIZE_BH=SEED); here they are
-// options.  With the octree, RADIAL_BH_FORCE_CORRECTION = 1 in R3 or S1R2
+// options, and so is the expansion history (COSMOPARAM: 0 LCDM, 1 wCDM with
+// w0, 2 w0-wa, -1 the tabulated EXPANSION_FILE with INTERPOLATION_ORDER;
+// cosmo.cc).  With the octree, RADIAL_BH_FORCE_CORRECTION = 1 in R3 or S1R2
 // first builds the radial correction table from the octree forces of the
 // glass file (or the IC file) as main.cc:791-1091 does, averaged over
 // RADIAL_BH_FORCE_TABLE_ITERATION randomised evaluations, writes it to
@@ -23,14 +25,14 @@
 //   * softening lengths (utils.cc:59-82), the tables of the mode
 //     (main.cc:404-726, 1256-1302), the initial time from the Friedmann
 //     equation, the output schedule (main.cc:1422-1565);
+//   * the initial time from the expansion history (friedmann_solver.cc:32-95);
 //   * initial force, initial step length (step.cc:35-98), then steps until
 //     a (or T) reaches a_max: a device KDK step, the scale factor advanced by
 //     RK4 in between its two kicks, snapshots at output times or redshifts,
 //     the next step length sqrt(2 ACC_PARAM / errmax) clamped to
 //     [STEP_MIN, STEP_MAX] and cut to land on the next output time.
 // Outputs are ASCII snapshots (OUTPUT_FORMAT 0) and a log of the steps.
-// Not provided: HDF5 input/output, redshift-cone output, tabulated or
-// w0/wa expansion histories (COSMOPARAM != 0), glass making.
+// Not provided: HDF5 input/output, redshift-cone output, glass making.
 #include "host_common.h"
 
 #include <algorithm>
@@ -46,32 +48,6 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846264338327950288;
 
-struct Cosmo {
-    double H0, Om, Or, OL, Ok;
-    // H(a) of the Friedmann equation
-    double hubble(double a) const {
-        const double e = Om / (a * a * a) + Or / (a * a * a * a) + OL + Ok / (a * a);
-        return H0 * std::sqrt(std::fabs(e));
-    }
-    // one classical fourth-order Runge-Kutta step of da/dt = a H(a)
-    double rk4(double a, double h) const {
-        auto f = [&](double b) { return b * hubble(b); };
-        const double k1 = f(a), k2 = f(a + 0.5 * h * k1), k3 = f(a + 0.5 * h * k2), k4 = f(a + h * k3);
-        return a + h * (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0;
-    }
-    // cosmic time at scale factor a: integral of da / (a H(a)) from 0, with
-    // a = s^2 (the integrand 2 / (s H(s^2)) vanishes smoothly at s = 0 in
-    // matter or radiation domination), composite Simpson over 4096 panels
-    double time_of(double a) const {
-        const int n = 4096;
-        const double s1 = std::sqrt(a), ds = s1 / n;
-        auto g = [&](double s) { return s > 0 ? 2.0 / (s * hubble(s * s)) : 0.0; };
-        double acc = g(0.0) + g(s1);
-        for (int i = 1; i < n; ++i) acc += g(i * ds) * ((i & 1) ? 4.0 : 2.0);
-        return acc * ds / 3.0;
-    }
-};
-
 struct Opts {
     std::string param;
     int topology = STEPS_TOPO_R3;
@@ -80,6 +56,7 @@ struct Opts {
     int lookup = 1;
     double bh_theta = 0.0;
     long bh_seed = -1;  // RANDOMIZE_BH
+    int cosmoparam = 0;  // COSMOPARAM (Template-LinuxGCC-Makefile:44)
     long max_steps = -1;
     int device = 0;
 };
@@ -112,6 +89,10 @@ bool parse_opts(int argc, char **argv, Opts &o) {
             if (v.empty()) return false;
             o.bh_seed = std::atol(v.c_str());
             if (o.bh_seed < 0) return false;
+        } else if (a == "--cosmo-param") {
+            const std::string v = val();
+            if (v != "0" && v != "1" && v != "2" && v != "-1") return false;
+            o.cosmoparam = std::atoi(v.c_str());
         } else if (a == "--max-steps") {
             o.max_steps = std::atol(val().c_str());
         } else if (a == "--device") {
@@ -163,8 +144,8 @@ int main(int argc, char **argv) {
     Opts o;
     if (!parse_opts(argc, argv, o))
         return die("usage: steps_run <parameter file> [--topology r3|t3|s1r2] [--precision fp64|mixed] "
-                   "[--interp-order 0|2|4|6|8] [--no-lookup] [--bh THETA] [--bh-seed SEED] [--max-steps K] "
-                   "[--device D]", 2);
+                   "[--interp-order 0|2|4|6|8] [--no-lookup] [--bh THETA] [--bh-seed SEED] "
+                   "[--cosmo-param 0|1|2|-1] [--max-steps K] [--device D]", 2);
     if (o.bh_seed >= 0) seed_draws((unsigned)o.bh_seed);
     steps_run_params P;
     if (steps_read_params(o.param.c_str(), o.topology, &P)) return die(steps_host_last_error());
@@ -208,7 +189,27 @@ int main(int argc, char **argv) {
         for (auto &c : v) c /= STEPS_UNIT_V;
 
     // ---- cosmology, masses (main.cc:1256-1369)
-    Cosmo C{P.H0, P.Omega_m, P.Omega_r, P.Omega_lambda, 1.0 - P.Omega_m - P.Omega_lambda - P.Omega_r};
+    steps_expansion E{};
+    E.cosmoparam = o.cosmoparam;
+    E.H0 = P.H0;
+    E.Omega_m = P.Omega_m;
+    E.Omega_r = P.Omega_r;
+    E.Omega_lambda = P.Omega_lambda;
+    E.Omega_k = 1. - P.Omega_m - P.Omega_lambda - P.Omega_r;  // main.cc:1261
+    E.w0 = P.w0;
+    E.wa = P.wa;
+    E.order = P.INTERPOLATION_ORDER;
+    std::vector<double> expansion;
+    if (cosmo && o.cosmoparam == -1) {  // main.cc:1436-1446
+        const int64_t rows = steps_read_expansion_history(P.EXPANSION_FILE, nullptr, 0);
+        if (rows < 0) return die(std::string("the expansion history file ") + P.EXPANSION_FILE + " does not exist");
+        if (rows < 4) return die("the expansion history needs at least 4 rows");
+        expansion.resize(3 * (size_t)rows);
+        if (steps_read_expansion_history(P.EXPANSION_FILE, expansion.data(), rows) != rows)
+            return die(steps_host_last_error());
+        E.table = expansion.data();
+        E.rows = rows;
+    }
     const double rho_crit = 3.0 * P.H0 * P.H0 / (8.0 * kPi);
     double misph = 0.0;
     if (comoving) {
@@ -366,8 +367,8 @@ int main(int argc, char **argv) {
 
     // ---- time and the output schedule (main.cc:1422-1565)
     double a = cosmo ? P.a_start : 1.0, T = 0.0, H = 0.0, a_max = P.a_max, t_next = 0.0, dT_out = 0.0;
-    if (cosmo) T = C.time_of(P.a_start);
-    if (comoving) H = C.hubble(a);
+    if (cosmo) T = steps_friedmann_start(&E, P.a_start, P.h_min * 0.05);  // main.cc:1454
+    if (comoving) H = steps_hubble(&E, a);
     if (cosmo && !comoving) {
         a = 1.0;
         a_max = P.a_max / STEPS_UNIT_T;  // a_max is the final time in Gy here (main.cc:1551)
@@ -446,9 +447,9 @@ int main(int argc, char **argv) {
         T += h;
         // step.cc:128-269: kick (a, H), drift, force, a <- RK4, kick (a', H')
         const double a1 = a, H1 = H;
-        if (comoving) {
-            a = C.rk4(a, h);
-            H = C.hubble(a);
+        if (comoving) {  // step.cc:233-240
+            a = steps_friedmann_step(&E, a, h, T);
+            H = steps_hubble(&E, a);
         }
         steps_step_params sp{h, a1, H1, a, H, 1.0, rho_part};
         if (next_frame() || steps_sim_step(sim, &sp, &errmax)) {

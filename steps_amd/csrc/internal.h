@@ -155,6 +155,10 @@ const uint32_t *spatial_order_keys(const void *scratch, int64_t n);
 size_t sort_u64_scratch_bytes(int64_t n);
 hipError_t launch_sort_u64(const uint32_t *hi, const uint32_t *lo, int64_t n, void *scratch, const uint32_t **perm,
                            hipStream_t stream);
+// The same for keys of nwords 32-bit words, words[0] the most significant
+// (the same scratch size: one word is sorted at a time).
+hipError_t launch_sort_words(const uint32_t *const *words, int nwords, int64_t n, void *scratch, const uint32_t **perm,
+                             hipStream_t stream);
 
 // Barnes-Hut octree forces (bh_kernels.hip): tree build and traversal for
 // rows id_min .. id_min+nrows-1 into F (device AoS, external term included);

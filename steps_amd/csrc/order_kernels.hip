@@ -337,9 +337,10 @@ size_t sort_u64_scratch_bytes(int64_t n) {
            a256((size_t)scan_chunks((int64_t)256 * sort_blocks(n)) * 4) + 256;
 }
 
-hipError_t launch_sort_u64(const uint32_t *hi, const uint32_t *lo, int64_t n, void *scratch, const uint32_t **perm,
-                           hipStream_t st) {
+hipError_t launch_sort_words(const uint32_t *const *words, int nwords, int64_t n, void *scratch, const uint32_t **perm,
+                             hipStream_t st) {
     if (n <= 0) return hipSuccess;
+    if (nwords < 1) return hipErrorInvalidValue;
     const size_t nn = (size_t)n;
     char *p = (char *)scratch;
     uint32_t *key[2], *val[2];
@@ -350,20 +351,28 @@ hipError_t launch_sort_u64(const uint32_t *hi, const uint32_t *lo, int64_t n, vo
     uint32_t *counts = (uint32_t *)(p + 4 * a256(nn * 4));
     uint32_t *bsum = (uint32_t *)(p + 4 * a256(nn * 4) + a256((size_t)256 * sort_blocks(n) * 4));
     const unsigned ng = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(iota_copy_kernel, dim3(ng), dim3(256), 0, st, lo, n, key[0], val[0]);
+    hipLaunchKernelGGL(iota_copy_kernel, dim3(ng), dim3(256), 0, st, words[nwords - 1], n, key[0], val[0]);
     hipError_t e;
     if ((e = hipGetLastError()) != hipSuccess) return e;
     int cur = 0;
     for (int shift = 0; shift < 32; shift += 8)
         if ((e = radix_pass(key, val, cur, n, shift, counts, bsum, st)) != hipSuccess) return e;
-    // the high halves, in the order the low-half passes left: a stable sort
-    // by them completes the sort by (hi, lo)
-    hipLaunchKernelGGL(gather_u32_kernel, dim3(ng), dim3(256), 0, st, hi, val[cur], n, key[cur]);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    for (int shift = 0; shift < 32; shift += 8)
-        if ((e = radix_pass(key, val, cur, n, shift, counts, bsum, st)) != hipSuccess) return e;
+    // each more significant word, in the order the passes so far left: a
+    // stable sort by it completes the sort by the words up to it
+    for (int w = nwords - 2; w >= 0; --w) {
+        hipLaunchKernelGGL(gather_u32_kernel, dim3(ng), dim3(256), 0, st, words[w], val[cur], n, key[cur]);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        for (int shift = 0; shift < 32; shift += 8)
+            if ((e = radix_pass(key, val, cur, n, shift, counts, bsum, st)) != hipSuccess) return e;
+    }
     *perm = val[cur];
     return hipSuccess;
+}
+
+hipError_t launch_sort_u64(const uint32_t *hi, const uint32_t *lo, int64_t n, void *scratch, const uint32_t **perm,
+                           hipStream_t st) {
+    const uint32_t *words[2] = {hi, lo};
+    return launch_sort_words(words, 2, n, scratch, perm, st);
 }
 
 size_t order_scratch_bytes(int64_t n) {

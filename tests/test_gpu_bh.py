@@ -259,3 +259,31 @@ def test_bh_randomised_and_radial_refusals(steps):
     with pytest.raises(_abi.StepsError) as ei:
         steps.force_eval(d, w.x, w.M, w.soft)
     assert ei.value.code == _abi.STEPS_ERR_TABLE
+
+
+@pytest.mark.parametrize("randomised", [False, True])
+def test_bh_full_size_sampled_rows(steps, oracle, randomised):
+    """The headline workload (N = 2^21, theta 0.5): the device octree over all
+    rows against the oracle's pointer octree on three row ranges (the oracle
+    builds the whole tree for each range)."""
+    print(f"case bh full size N=2^21 randomised={randomised}", flush=True)
+    w = wl.r3_compactified(N=2 ** 21, seed=1003)
+    (d, keep), kw = _desc(steps, w, 0.5)
+    u = _U[2] if randomised else None
+    if randomised:
+        d.bh_randomize = 1
+        for k in range(6):
+            d.bh_random[k] = u[k]
+    sim = steps.Sim(d, w.N)
+    sim.set_state(w.x, w.M, w.soft, w.v)
+    sim.forces()
+    F = sim.get_forces()
+    sim.close()
+    p = oracle.params(0, **kw)
+    errs = []
+    for lo in (0, w.N // 2 - 256, w.N - 512):
+        Fo = oracle.forces_bh_ex(p, 0.5, w.particle_radius, w.x, w.M, w.soft, u=u, id_min=lo, id_max=lo + 511)
+        errs.append(rel_err(F[lo:lo + 512], Fo))
+    e = np.concatenate(errs)
+    report(f"bh full size randomised={randomised} (1536 rows)", e)
+    assert e.max() <= 5e-12 and np.median(e) <= 1e-14

@@ -160,3 +160,45 @@ def test_steps_run_randomised_octree_radial_table(tmp_path, oracle, steps):
     assert np.allclose(got[:, 1], want, rtol=1e-10, atol=1e-15 + 1e-12 * np.abs(want).max())
     log = np.loadtxt(out / "Logfile.dat")
     assert log.shape[0] >= 2
+
+
+@pytest.mark.parametrize("cosmoparam,extra", [(1, "w0\t-0.9\n"), (2, "w0\t-0.9\nwa\t0.3\n")])
+def test_steps_run_dark_energy_models(tmp_path, cosmoparam, extra):
+    """--cosmo-param 1 / 2 (the reference's COSMOPARAM builds, friedmann_solver.cc:
+    174-333): the logged scale factors and Hubble parameters are the host
+    library's wCDM / w0-wa expansion history replayed over the logged steps,
+    from its start time of a_start."""
+    print(f"case steps_run R3 comoving N=2048 COSMOPARAM={cosmoparam}", flush=True)
+    import test_host_cosmo as hc
+    w = wl.r3_compactified(N=2048, seed=25)
+    par, out = _write_run(tmp_path, w, 1.0 / 32, 0.0318, 0.0125, f"de{cosmoparam}")
+    with open(par, "a") as f:
+        f.write(extra)
+    exe = ROOT / "steps_amd" / "lib" / "steps_run"
+    r = subprocess.run([str(exe), str(par), "--cosmo-param", str(cosmoparam), "--max-steps", "12"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    log = np.loadtxt(out / "Logfile.dat")
+    host = C.CDLL(str(ROOT / "steps_amd" / "lib" / "libsteps_host.so"))
+    E = C.POINTER(hc.Expansion)
+    host.steps_hubble.restype = C.c_double
+    host.steps_hubble.argtypes = [E, C.c_double]
+    host.steps_friedmann_step.restype = C.c_double
+    host.steps_friedmann_step.argtypes = [E, C.c_double, C.c_double, C.c_double]
+    host.steps_friedmann_start.restype = C.c_double
+    host.steps_friedmann_start.argtypes = [E, C.c_double, C.c_double]
+    H0 = float(repr(wl.H0 * UNIT_V)) / UNIT_V
+    e = hc.model(cosmoparam, H0=H0, Om=wl.OMEGA_M, OL=wl.OMEGA_L, w0=-0.9, wa=0.3 if cosmoparam == 2 else 0.0)
+    T = host.steps_friedmann_start(C.byref(e), 1.0 / 32, 0.00002 / UNIT_T * 0.05)
+    a = 1.0 / 32
+    for row in log:
+        h = row[5]
+        T += h
+        a = host.steps_friedmann_step(C.byref(e), a, h, T)
+        assert row[1] == pytest.approx(T, rel=1e-13)
+        assert row[2] == pytest.approx(a, rel=1e-13)
+        assert row[4] == pytest.approx(host.steps_hubble(C.byref(e), a), rel=1e-13)
+    # and it is not LCDM
+    lc = hc.model(0, H0=H0, Om=wl.OMEGA_M, OL=wl.OMEGA_L)
+    assert abs(host.steps_hubble(C.byref(lc), log[-1, 2]) - log[-1, 4]) > 1e-9 * log[-1, 4]

@@ -29,7 +29,9 @@ class RunParams(C.Structure):
                 ("MIN_REDSHIFT", C.c_double), ("REDSHIFT_CONE", C.c_int32), ("H0_INDEPENDENT_UNITS", C.c_int32),
                 ("Rsim", C.c_double), ("RADIAL_FORCE_ACCURACY", C.c_int32), ("RADIAL_FORCE_TABLE_SIZE", C.c_int32),
                 ("RADIAL_BH_FORCE_CORRECTION", C.c_int32), ("GLASS_FILE_FOR_BH_FORCE_CORRECTION", C.c_char * 1024),
-                ("RADIAL_BH_FORCE_TABLE_SIZE", C.c_int32), ("RADIAL_BH_FORCE_TABLE_ITERATION", C.c_int32)]
+                ("RADIAL_BH_FORCE_TABLE_SIZE", C.c_int32), ("RADIAL_BH_FORCE_TABLE_ITERATION", C.c_int32),
+                ("w0", C.c_double), ("wa", C.c_double), ("EXPANSION_FILE", C.c_char * 1024),
+                ("INTERPOLATION_ORDER", C.c_int32)]
 
 
 @pytest.fixture(scope="module")
@@ -137,6 +139,13 @@ def test_strstr_semantics_and_clamps(host, tmp_path):
     b = read(host, g, R3)
     assert (b.RADIAL_BH_FORCE_CORRECTION, b.RADIAL_BH_FORCE_TABLE_SIZE, b.RADIAL_BH_FORCE_TABLE_ITERATION) == (0, 5, 1)
     assert b.GLASS_FILE_FOR_BH_FORCE_CORRECTION == b"glass.dat"
+    # the COSMOPARAM keys; INTERPOLATION_ORDER outside 1..3 reads as 1 (read_paramfile.cc:627-631)
+    assert p.INTERPOLATION_ORDER == 1 and p.w0 == 0.0 and p.wa == 0.0
+    g.write_text("w0 -0.9\nwa 0.25\nEXPANSION_FILE hist.dat\nINTERPOLATION_ORDER 3\n")
+    c = read(host, g, R3)
+    assert (c.w0, c.wa, c.EXPANSION_FILE, c.INTERPOLATION_ORDER) == (-0.9, 0.25, b"hist.dat", 3)
+    g.write_text("INTERPOLATION_ORDER 7\n")
+    assert read(host, g, R3).INTERPOLATION_ORDER == 1
     # H0-independent units convert L and PARTICLE_RADII with cosmology on
     f.write_text("COSMOLOGY 1\nHubbleConstant 67.66\nL_BOX 100\nPARTICLE_RADII 0.5\nH_INDEPENDENT_UNITS 1\n")
     q = read(host, f, T3)
