@@ -60,6 +60,53 @@ def test_bh_tiny_theta_is_the_direct_sum(steps):
     assert e.max() <= 1e-11
 
 
+def test_bh_close_pairs_resolve_below_level_21(steps, oracle):
+    """150 pairs placed 10^-12 .. 10^-6.5 of the root side apart (below the
+    2^-21 cube a single 63-bit key resolves; 2^-40 at the closest): the device
+    splits them level by level as the reference's pointer tree does, so every
+    row agrees with the oracle."""
+    print("case bh N=4096 with 150 close pairs", flush=True)
+    w = wl.r3_compactified(N=4096, seed=18)
+    x = w.x.copy()
+    rng = np.random.default_rng(18)
+    far = int(np.argmax(np.abs(x).max(axis=1)))  # keeps the root cube
+    pick = rng.choice(np.delete(np.arange(w.N), far), 300, replace=False)
+    a, b = pick[:150], pick[150:]
+    root = 2.00002 * np.abs(x).max()
+    sep = root * 10.0 ** rng.uniform(-12.0, -6.5, 150)
+    u = rng.normal(size=(150, 3))
+    x[b] = x[a] + u / np.linalg.norm(u, axis=1, keepdims=True) * sep[:, None]
+    assert 2.00002 * np.abs(x).max() == root
+    (d, keep), kw = _desc(steps, w, 0.5)
+    Fo = oracle.forces_bh(oracle.params(0, **kw), 0.5, w.particle_radius, x, w.M, w.soft)
+    F = steps.force_eval(d, x, w.M, w.soft)
+    e = rel_err(F, Fo)
+    report("bh close pairs", e)
+    assert e.max() <= 1e-12
+
+
+def test_bh_coincident_particles(steps):
+    """Exact duplicates (the reference's insertion would recurse without end;
+    the device keeps particles that share all 42 digits as leaves of one
+    level-42 node): forces stay finite, and with an opening angle too small
+    to accept any node they are the direct sum, whose softened kernel gives
+    a coincident pair no force."""
+    print("case bh N=3000 with coincident particles", flush=True)
+    w = wl.r3_compactified(N=3000, seed=19)
+    x = w.x.copy()
+    x[1500:1540] = x[100:140]  # 40 duplicated particles
+    x[1600:1610] = x[200]  # 11 particles at one point
+    (d, keep), kw = _desc(steps, w, 1e-12)
+    dd, keep2 = steps.make_desc(steps.TOPO_R3, **kw)
+    F, Fd = steps.force_eval(d, x, w.M, w.soft), steps.force_eval(dd, x, w.M, w.soft)
+    assert np.isfinite(F).all() and np.isfinite(Fd).all()
+    e = rel_err(F, Fd)
+    report("bh coincident, tiny theta vs direct", e)
+    assert e.max() <= 1e-11
+    (d5, keep5), kw5 = _desc(steps, w, 0.5)
+    assert np.isfinite(steps.force_eval(d5, x, w.M, w.soft)).all()
+
+
 def test_bh_small_counts(steps, oracle):
     for N in (1, 2, 3, 9, 64, 65, 257):
         print(f"case bh small N={N}", flush=True)
