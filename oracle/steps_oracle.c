@@ -1187,6 +1187,32 @@ static void orc_bh_force(orc_bh_ctx *c, int node, int i, double *f)
     }
 }
 
+/* Node masses and centres of mass summed from the children, bottom-up: the
+ * running averages of forces.cc:164-170 replaced by sums over the same tree
+ * (oracle_bh_opts.com_from_children). */
+static void orc_bh_com_from_children(orc_bh_tree *t, int node)
+{
+    orc_bh_node *q = &t->n[node];
+    if (q->pidx >= 0) return;
+    double sm = 0, sx = 0, sy = 0, sz = 0;
+    for (int j = 0; j < 8; ++j) {
+        const int c = q->child[j];
+        if (c < 0) continue;
+        orc_bh_com_from_children(t, c);
+        const orc_bh_node *k = &t->n[c];
+        sx += k->mass * k->com[0];
+        sy += k->mass * k->com[1];
+        sz += k->mass * k->com[2];
+        sm += k->mass;
+    }
+    if (sm > 0) {
+        q->mass = sm;
+        q->com[0] = sx / sm;
+        q->com[1] = sy / sm;
+        q->com[2] = sz / sm;
+    }
+}
+
 static long long orc_bh_counts[2];
 static double orc_bh_root[4];
 
@@ -1376,6 +1402,7 @@ int oracle_forces_bh_ex(const oracle_params *p, double theta, double particle_ra
     orc_bh_tree t = {NULL, 0, 0};
     int rc = orc_bh_new(&t, c0[0], c0[1], c0[2], side) < 0 ? -1 : 0;
     for (int i = 0; i < N && !rc; ++i) rc = orc_bh_insert(&t, 0, x, M, i, 0);
+    if (!rc && o && o->com_from_children) orc_bh_com_from_children(&t, 0);
     if (rc) {
         free(t.n);
         free(x);

@@ -40,7 +40,12 @@
  *                          cluster taken at its centre of mass; error falls
  *                          with theta; identity draws = plain octree bit for
  *                          bit; root cubes by the placement rules; averaging
- *                          over draws shrinks the error (tests/test_oracle_bh*.py)
+ *                          over draws shrinks the error (tests/test_oracle_bh*.py);
+ *                          with a huge opening angle (every row takes the
+ *                          root) a row's force is the closed form at the
+ *                          extended-precision centre of mass, for the
+ *                          running averages and for com_from_children
+ *                          (tests/test_oracle_bh_com.py)
  *   radial_bh correction   linear data between bin centres come back, the
  *                          origin line below bin 1's centre; the table is the
  *                          binned mean radial force; the corrected force is
@@ -212,6 +217,12 @@ typedef struct oracle_bh_opts {
     double u[6];
     const double *radial_bh_table;
     int radial_bh_table_size;
+    /* 0: the reference's centres of mass, running averages updated at each
+     * insertion (forces.cc:164-170); 1: after the insertions, each internal
+     * node's mass and centre of mass summed from its children's, bottom-up
+     * (the same tree and walk; a check that isolates the rounding of the
+     * running averages, not the reference's arithmetic) */
+    int com_from_children;
 } oracle_bh_opts;
 /* oracle_forces_bh with those options (o may be NULL): the positions are
  * moved into the random frame (R3 forces.cc:324-356, 420-434; T3 :662-685;

@@ -44,7 +44,7 @@ class OracleParams(C.Structure):
 
 class OracleBhOpts(C.Structure):
     _fields_ = [("randomize", C.c_int), ("u", C.c_double * 6), ("radial_bh_table", _DP),
-                ("radial_bh_table_size", C.c_int)]
+                ("radial_bh_table_size", C.c_int), ("com_from_children", C.c_int)]
 
 
 class OracleCosmo(C.Structure):
@@ -223,9 +223,11 @@ def forces_bh(p: OracleParams, theta: float, particle_radius: float, x, M, h, id
 
 
 def forces_bh_ex(p: OracleParams, theta: float, particle_radius: float, x, M, h, *, u=None, radial_bh=None,
-                 id_min=0, id_max=None) -> np.ndarray:
+                 id_min=0, id_max=None, com_from_children=False) -> np.ndarray:
     """Octree forces with the RANDOMIZE_BH draws u (six values in [0, 1], the
-    reference's order) and/or the radial correction table radial_bh."""
+    reference's order) and/or the radial correction table radial_bh;
+    com_from_children sums node centres of mass from the children instead of
+    the reference's running averages (same tree, same walk)."""
     x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
     M = np.ascontiguousarray(M, dtype=np.float64)
     h = np.ascontiguousarray(h, dtype=np.float64)
@@ -234,6 +236,7 @@ def forces_bh_ex(p: OracleParams, theta: float, particle_radius: float, x, M, h,
         id_max = N - 1
     n = max(0, id_max - id_min + 1)
     o = OracleBhOpts()
+    o.com_from_children = 1 if com_from_children else 0
     if u is not None:
         o.randomize = 1
         for k, v in enumerate(list(u) + [0.0] * (6 - len(u))):
