@@ -202,3 +202,63 @@ def test_steps_run_dark_energy_models(tmp_path, cosmoparam, extra):
     # and it is not LCDM
     lc = hc.model(0, H0=H0, Om=wl.OMEGA_M, OL=wl.OMEGA_L)
     assert abs(host.steps_hubble(C.byref(lc), log[-1, 2]) - log[-1, 4]) > 1e-9 * log[-1, 4]
+
+
+def test_steps_run_tabulated_expansion_history(tmp_path):
+    """--cosmo-param -1 (friedmann_solver.cc:336-466): the run reads
+    EXPANSION_FILE (t [Gy], a, H [km/s/Mpc]) and takes a and H by cubic
+    interpolation (INTERPOLATION_ORDER 3) at each step's end time, from the
+    start time of the row before the first past a_start; the log is the host
+    library's replay of the same table over the logged steps."""
+    print("case steps_run R3 comoving N=2048 tabulated expansion history", flush=True)
+    import test_host_cosmo as hc
+    host = C.CDLL(str(ROOT / "steps_amd" / "lib" / "libsteps_host.so"))
+    E = C.POINTER(hc.Expansion)
+    host.steps_hubble.restype = C.c_double
+    host.steps_hubble.argtypes = [E, C.c_double]
+    host.steps_friedmann_step.restype = C.c_double
+    host.steps_friedmann_step.argtypes = [E, C.c_double, C.c_double, C.c_double]
+    host.steps_friedmann_start.restype = C.c_double
+    host.steps_friedmann_start.argtypes = [E, C.c_double, C.c_double]
+    host.steps_read_expansion_history.restype = C.c_int64
+    host.steps_read_expansion_history.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int64]
+    H0 = float(repr(wl.H0 * UNIT_V)) / UNIT_V
+    lc = hc.model(0, H0=H0, Om=wl.OMEGA_M, OL=wl.OMEGA_L)
+    # an LCDM history from a = 0.02 on a fine grid, written in the file's units
+    a, t, h = 0.02, host.steps_friedmann_start(C.byref(lc), 0.02, 1e-6), 2e-5
+    rows = []
+    while a < 0.05:
+        rows.append((t * UNIT_T, a, host.steps_hubble(C.byref(lc), a) * UNIT_V))
+        t += h
+        a = host.steps_friedmann_step(C.byref(lc), a, h, t)
+    hist = tmp_path / "expansion.dat"
+    hist.write_text("".join("%.17g %.17g %.17g\n" % r for r in rows))
+    w = wl.r3_compactified(N=2048, seed=26)
+    par, out = _write_run(tmp_path, w, 1.0 / 32, 0.0318, 0.0125, "tab")
+    with open(par, "a") as f:
+        f.write(f"EXPANSION_FILE\t{hist}\nINTERPOLATION_ORDER\t3\n")
+    exe = ROOT / "steps_amd" / "lib" / "steps_run"
+    r = subprocess.run([str(exe), str(par), "--cosmo-param", "-1", "--max-steps", "12"], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    log = np.loadtxt(out / "Logfile.dat")
+    n = host.steps_read_expansion_history(str(hist).encode(), None, 0)
+    assert n == len(rows)
+    tab = np.zeros(3 * n)
+    assert host.steps_read_expansion_history(str(hist).encode(), tab.ctypes.data_as(C.POINTER(C.c_double)), n) == n
+    e = hc.model(-1, H0=H0, Om=wl.OMEGA_M, OL=wl.OMEGA_L)
+    e.table = tab.ctypes.data_as(C.POINTER(C.c_double))
+    e.rows = n
+    e.order = 3
+    T = host.steps_friedmann_start(C.byref(e), 1.0 / 32, 0.00002 / UNIT_T * 0.05)
+    assert tab[3 * (e.index - 1) + 1] <= 1.0 / 32 < tab[3 * e.index + 1]
+    aa = 1.0 / 32
+    for row in log:
+        T += row[5]
+        aa = host.steps_friedmann_step(C.byref(e), aa, row[5], T)
+        assert row[1] == pytest.approx(T, rel=1e-13)
+        assert row[2] == pytest.approx(aa, rel=1e-13)
+        assert row[4] == pytest.approx(host.steps_hubble(C.byref(e), aa), rel=1e-13)
+    # the tabulated run follows the LCDM history it was sampled from
+    assert log[-1, 2] == pytest.approx(host.steps_friedmann_step(C.byref(lc), log[-2, 2], log[-1, 5], 0.0), rel=1e-6)
