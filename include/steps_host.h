@@ -126,16 +126,20 @@ typedef struct steps_expansion {
     int32_t order;
     int64_t index;
 } steps_expansion;
-/* CALCULATE_Hubble_param */
+/* CALCULATE_Hubble_param (friedmann_solver.cc:161-164 LCDM, 237-240 wCDM,
+ * 322-325 w0-wa, 381-415 tabulated) */
 double steps_hubble(const steps_expansion *e, double a);
-/* friedmann_solver_step: a after one RK4 step of length h from a0 (the
+/* friedmann_solver_step (friedmann_solver.cc:100-159, 176-235, 255-320;
+ * tabulated 343-379): a after one RK4 step of length h from a0 (the
  * tabulated history: a at the time T at the end of the step) */
 double steps_friedmann_step(steps_expansion *e, double a0, double h, double T);
-/* friedmann_solver_start: the cosmic time of a_start (RK4 step h backwards
- * to the big bang, halved until converged) */
+/* friedmann_solver_start (friedmann_solver.cc:33-95; tabulated 438-467):
+ * the cosmic time of a_start (RK4 step h backwards to the big bang, halved
+ * until converged) */
 double steps_friedmann_start(steps_expansion *e, double a_start, double h);
-/* read_expansion_history: rows (t Gy, a, H km/s/Mpc) into table [3 rows] in
- * internal units; table == NULL returns the row count */
+/* read_expansion_history (inputoutput.cc:64-97): rows (t Gy, a, H km/s/Mpc)
+ * into table [3 rows] in internal units; table == NULL returns the row
+ * count */
 int64_t steps_read_expansion_history(const char *path, double *table, int64_t cap);
 
 /* save_function_to_ascii_table (inputoutput.cc:1038-1061): the header line,

@@ -78,7 +78,7 @@ double tab_interp(const steps_expansion *e, int64_t i, int cx, int cy, double X)
 
 extern "C" {
 
-// friedmann_solver.cc:160-163, 236-239, 317-320, 376-407
+// friedmann_solver.cc:161-164, 237-240, 322-325, 381-415
 double steps_hubble(const steps_expansion *e, double a) {
     if (e->cosmoparam == -1) {
         int64_t i = e->index - 2;
@@ -100,7 +100,7 @@ double steps_hubble(const steps_expansion *e, double a) {
 // integrate sqrt|E^2| and run backwards once every stage is negative (a
 // recollapse; also when H0 <= 0).  The tabulated history interpolates a at
 // the time T at the end of the step from the rows around T - h
-// (friedmann_solver.cc:341-373), and moves the history's cursor.
+// (friedmann_solver.cc:343-379), and moves the history's cursor.
 double steps_friedmann_step(steps_expansion *e, double a0, double h, double T) {
     if (e->cosmoparam == -1) {
         int64_t i = e->index - 2;
@@ -157,7 +157,7 @@ double steps_friedmann_step(steps_expansion *e, double a0, double h, double T) {
 // with h halved until two estimates agree to 1e-10 (at most 20 times).  The
 // tabulated history takes the time of the row before the first one past
 // a_start: the reference's interpolation cases fall through to that default
-// (friedmann_solver.cc:433-466).
+// (friedmann_solver.cc:438-467).
 double steps_friedmann_start(steps_expansion *e, double a_start, double h) {
     if (e->cosmoparam == -1) {
         int64_t i = 0;
