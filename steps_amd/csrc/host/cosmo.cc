@@ -53,7 +53,7 @@ double tab(const steps_expansion *e, int64_t r, int c) { return e->table[3 * r +
 
 // Interpolation of column cy against column cx around row i (the first row
 // whose cx value passes the argument), order 1, 2 or 3 with the reference's
-// index clamps (friedmann_solver.cc:343-372, 376-407).
+// index clamps (friedmann_solver.cc:343-379, 381-415).
 double tab_interp(const steps_expansion *e, int64_t i, int cx, int cy, double X) {
     const int64_t n = e->rows;
     switch (e->order) {
@@ -150,7 +150,7 @@ double steps_friedmann_step(steps_expansion *e, double a0, double h, double T) {
     return collapse == 0 ? b + K : b - K;
 }
 
-// friedmann_solver.cc:32-95: the time since the big bang at a_start, by
+// friedmann_solver.cc:33-95: the time since the big bang at a_start, by
 // integrating backwards from a_start with step h until a <= 0, extrapolating
 // the last two positive steps to a = 0 as a power law (a ~ t^(2/3) when
 // matter dominates the last step, t^(1/2) under radiation), and repeating
