@@ -438,8 +438,8 @@ __global__ __launch_bounds__(256) void bh_scan_add_kernel(uint32_t *__restrict__
 struct __align__(64) BhNode {
     double x, y, z, m;
     double side, msoft;
+    double side2;  // side^2, for the R3 opening test side^2 < theta^2 r^2
     int next, pidx;
-    int pad[2];
 };
 static_assert(sizeof(BhNode) == 64, "one 64-byte record per item");
 
@@ -469,6 +469,7 @@ __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__
         for (int q = 0; q < v; ++q) side = side / 2;  // forces.cc:155, level by level
         BhNode &o = nodes[item];
         o.side = side;
+        o.side2 = side * side;
         o.next = (b + 1 < n) ? (int)base[b + 1] : total;
         o.pidx = -1;
         level[item] = (int8_t)v;
@@ -480,10 +481,10 @@ __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__
     o.z = r.z;
     o.m = r.w;
     o.side = 0.0;
+    o.side2 = 0.0;
     o.msoft = cbrt(r.w / mmin) * pr;
     o.next = (p + 1 < n) ? (int)base[p + 1] : total;
     o.pidx = (int)sidx[p];
-    o.pad[0] = o.pad[1] = 0;
     nodes[leaf] = o;
     level[leaf] = -1;
 }
@@ -524,6 +525,8 @@ __device__ __forceinline__ double bh_soft(double r, double beta) {
     return 32.0 / b6 * r3 + -38.4 / b5 * r2 + 32.0 / (3.0 * b3);
 }
 
+// (The periodic walks.  The R3 walk needs no 1/r: it tests side^2 < theta^2 r^2
+// and r^2 >= beta^2 and forms m r^-3 with one correction of the seed.)
 // y = 1/r and r from r^2: the v_rsq_f64 seed y0 (relative error |d| <= 5.2e-8
 // on gfx950) refined to second order, y0 (1 + e/2 + 3e^2/8) with
 // e = 1 - r^2 y0^2, relative error below 1e-21 before rounding, so y and r
@@ -777,6 +780,7 @@ __device__ __forceinline__ void bh_walk(const BhNode *__restrict__ nodes, int to
                                         double yi, double zi, double hi, double theta, double L, double cut,
                                         const BhTables &tb, double *f) {
     int wait = mine ? 0 : INT32_MAX;  // the next item this lane takes part in
+    const double theta2 = theta * theta;
     int it = 0;
     while (it < total) {
         const BhNode q = nodes[it];
@@ -793,31 +797,55 @@ __device__ __forceinline__ void bh_walk(const BhNode *__restrict__ nodes, int to
                 if (MODE >= BH_T3_NEAR && MODE <= BH_S1R2_TABLE)  // forces.cc:600-601, 941-942
                     if (fabs(dz) > 0.5 * L) dz -= copysign(L, dz);
                 const double r2 = dx * dx + dy * dy + dz * dz;
-                double y, dist;
-                bh_rsq(r2, y, dist);
-                if (q.pidx >= 0 || q.side * y < theta) {
-                    wait = q.next;
-                    if (MODE != BH_S1R2_IMAGES || fabs(dz) <= cut) {  // forces.cc:915
+                if constexpr (MODE == BH_R3) {
+                    // side / r < theta as side^2 < theta^2 r^2 and r >= beta as r^2 >= beta^2 (the
+                    // same decisions to rounding), and m r^-3 from the v_rsq_f64 seed y with one
+                    // correction, y^3 (1 + 1.5 e), e = 1 - r^2 y^2: within 2.1e-14 (one-sided)
+                    if (q.pidx >= 0 || q.side2 < theta2 * r2) {
+                        wait = q.next;
                         const double beta = q.msoft + hi;
-                        const double w = dist >= beta ? (y * y) * y : bh_soft(r2 > 0.0 ? dist : 0.0, beta);
-                        if constexpr (MODE == BH_T3_EWALD || MODE == BH_S1R2_TABLE) {  // forces.cc:606-610, 947-951
-                            double Dx, Dy, Dz;
-                            if constexpr (MODE == BH_T3_EWALD)
-                                bh_t3_correction<ORD>(tb, L, dx, dy, dz, Dx, Dy, Dz);
-                            else
-                                bh_s1r2_correction<ORD>(tb, L, dx, dy, dz, Dx, Dy, Dz);
-                            f[0] += q.m * (w * dx - Dx);
-                            f[1] += q.m * (w * dy - Dy);
-                            f[2] += q.m * (w * dz - Dz);
+                        double wm;
+                        if (r2 >= beta * beta) {
+                            const double y0 = __builtin_amdgcn_rsq(r2);
+                            const double t = y0 * y0;
+                            const double e = fma(-r2, t, 1.0);
+                            wm = (t * (y0 * q.m)) * fma(e, 1.5, 1.0);
                         } else {
-                            const double wm = q.m * w;
-                            f[0] += wm * dx;
-                            f[1] += wm * dy;
-                            f[2] += wm * dz;
+                            wm = q.m * bh_soft(sqrt(r2), beta);
                         }
+                        f[0] += wm * dx;
+                        f[1] += wm * dy;
+                        f[2] += wm * dz;
+                    } else {
+                        open = true;  // its first child follows in pre-order
                     }
                 } else {
-                    open = true;  // its first child follows in pre-order
+                    double y, dist;
+                    bh_rsq(r2, y, dist);
+                    if (q.pidx >= 0 || q.side * y < theta) {
+                        wait = q.next;
+                        if (MODE != BH_S1R2_IMAGES || fabs(dz) <= cut) {  // forces.cc:915
+                            const double beta = q.msoft + hi;
+                            const double w = dist >= beta ? (y * y) * y : bh_soft(r2 > 0.0 ? dist : 0.0, beta);
+                            if constexpr (MODE == BH_T3_EWALD || MODE == BH_S1R2_TABLE) {  // forces.cc:606-610, 947-951
+                                double Dx, Dy, Dz;
+                                if constexpr (MODE == BH_T3_EWALD)
+                                    bh_t3_correction<ORD>(tb, L, dx, dy, dz, Dx, Dy, Dz);
+                                else
+                                    bh_s1r2_correction<ORD>(tb, L, dx, dy, dz, Dx, Dy, Dz);
+                                f[0] += q.m * (w * dx - Dx);
+                                f[1] += q.m * (w * dy - Dy);
+                                f[2] += q.m * (w * dz - Dz);
+                            } else {
+                                const double wm = q.m * w;
+                                f[0] += wm * dx;
+                                f[1] += wm * dy;
+                                f[2] += wm * dz;
+                            }
+                        }
+                    } else {
+                        open = true;  // its first child follows in pre-order
+                    }
                 }
             }
         }
