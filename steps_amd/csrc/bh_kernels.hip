@@ -431,10 +431,13 @@ __global__ __launch_bounds__(256) void bh_scan_add_kernel(uint32_t *__restrict__
 }
 
 // One item of the pre-order list, 64 bytes (one scalar load of 16 dwords):
-// centre of mass and mass; the cube's side (0 for a leaf); the mass part of
-// the softening length, cbrt(mass / M_min) * ParticleRadi; the skip link
-// (the item after the subtree) and the particle (input index) of a leaf, or
-// -1 for an internal node.
+// centre of mass and mass; the cube's side (0 for a leaf) and its square; the
+// mass part of the softening length, cbrt(mass / M_min) * ParticleRadi; the
+// skip link (the item after the subtree) and the particle (input index) of a
+// leaf, -1 for an internal node, kBhMassless for an item of zero mass (which
+// every row skips: an integer test on the scalar unit, where a test of the
+// mass would take a vector compare per item).
+constexpr int kBhMassless = -2;
 struct __align__(64) BhNode {
     double x, y, z, m;
     double side, msoft;
@@ -484,7 +487,7 @@ __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__
     o.side2 = 0.0;
     o.msoft = cbrt(r.w / mmin) * pr;
     o.next = (p + 1 < n) ? (int)base[p + 1] : total;
-    o.pidx = (int)sidx[p];
+    o.pidx = (r.w == 0.0) ? kBhMassless : (int)sidx[p];
     nodes[leaf] = o;
     level[leaf] = -1;
 }
@@ -501,6 +504,7 @@ __global__ __launch_bounds__(256) void bh_level_kernel(int lev, const int *__res
         double sx = 0, sy = 0, sz = 0, sm = 0;
         for (int c = it + 1; c < end; c = nodes[c].next) {
             const BhNode &q = nodes[c];
+            if (q.m == 0.0) continue;  // a massless child adds nothing (and has no centre of mass)
             sx += q.m * q.x;
             sy += q.m * q.y;
             sz += q.m * q.z;
@@ -512,6 +516,7 @@ __global__ __launch_bounds__(256) void bh_level_kernel(int lev, const int *__res
         o.z = sz / sm;
         o.m = sm;
         o.msoft = cbrt(sm / red[1]) * pr;
+        if (sm == 0.0) o.pidx = kBhMassless;
     }
 }
 
@@ -786,7 +791,7 @@ __device__ __forceinline__ void bh_walk(const BhNode *__restrict__ nodes, int to
         const BhNode q = nodes[it];
         bool open = false;
         if (wait <= it) {
-            if (q.pidx == i || q.m == 0.0) {  // the row's own leaf; a massless node
+            if (q.pidx == i || q.pidx == kBhMassless) {  // the row's own leaf; a massless item
                 wait = q.next;
             } else {
                 double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
