@@ -431,7 +431,7 @@ __global__ __launch_bounds__(256) void bh_scan_add_kernel(uint32_t *__restrict__
 }
 
 // One item of the pre-order list, 64 bytes (one scalar load of 16 dwords):
-// centre of mass and mass; the cube's side (0 for a leaf) and its square; the
+// centre of mass and mass; the cube's side (0 for a leaf) and side^2/theta^2; the
 // mass part of the softening length, cbrt(mass / M_min) * ParticleRadi; the
 // skip link (the item after the subtree) and the particle (input index) of a
 // leaf, -1 for an internal node, kBhMassless for an item of zero mass (which
@@ -441,7 +441,7 @@ constexpr int kBhMassless = -2;
 struct __align__(64) BhNode {
     double x, y, z, m;
     double side, msoft;
-    double side2;  // side^2, for the R3 opening test side^2 < theta^2 r^2
+    double open2;  // side^2 / theta^2: the R3 walk opens the cube at r^2 <= open2
     int next, pidx;
 };
 static_assert(sizeof(BhNode) == 64, "one 64-byte record per item");
@@ -456,8 +456,8 @@ __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__
                                                       const double4 *__restrict__ srec,
                                                       const uint32_t *__restrict__ sidx, int64_t n,
                                                       const double *__restrict__ red, double pr,
-                                                      const int *__restrict__ info, BhNode *__restrict__ nodes,
-                                                      int8_t *__restrict__ level) {
+                                                      double inv_theta2, const int *__restrict__ info,
+                                                      BhNode *__restrict__ nodes, int8_t *__restrict__ level) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int total = info[0];
@@ -472,7 +472,7 @@ __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__
         for (int q = 0; q < v; ++q) side = side / 2;  // forces.cc:155, level by level
         BhNode &o = nodes[item];
         o.side = side;
-        o.side2 = side * side;
+        o.open2 = (side * side) * inv_theta2;
         o.next = (b + 1 < n) ? (int)base[b + 1] : total;
         o.pidx = -1;
         level[item] = (int8_t)v;
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void bh_fill_kernel(const int8_t *__restrict__
     o.z = r.z;
     o.m = r.w;
     o.side = 0.0;
-    o.side2 = 0.0;
+    o.open2 = 0.0;
     o.msoft = cbrt(r.w / mmin) * pr;
     o.next = (p + 1 < n) ? (int)base[p + 1] : total;
     o.pidx = (r.w == 0.0) ? kBhMassless : (int)sidx[p];
@@ -530,7 +530,7 @@ __device__ __forceinline__ double bh_soft(double r, double beta) {
     return 32.0 / b6 * r3 + -38.4 / b5 * r2 + 32.0 / (3.0 * b3);
 }
 
-// (The periodic walks.  The R3 walk needs no 1/r: it tests side^2 < theta^2 r^2
+// (The periodic walks.  The R3 walk needs no 1/r: it tests side^2 / theta^2 < r^2
 // and r^2 >= beta^2 and forms m r^-3 with one correction of the seed.)
 // y = 1/r and r from r^2: the v_rsq_f64 seed y0 (relative error |d| <= 5.2e-8
 // on gfx950) refined to second order, y0 (1 + e/2 + 3e^2/8) with
@@ -785,7 +785,6 @@ __device__ __forceinline__ void bh_walk(const BhNode *__restrict__ nodes, int to
                                         double yi, double zi, double hi, double theta, double L, double cut,
                                         const BhTables &tb, double *f) {
     int wait = mine ? 0 : INT32_MAX;  // the next item this lane takes part in
-    const double theta2 = theta * theta;
     int it = 0;
     while (it < total) {
         const BhNode q = nodes[it];
@@ -803,10 +802,10 @@ __device__ __forceinline__ void bh_walk(const BhNode *__restrict__ nodes, int to
                     if (fabs(dz) > 0.5 * L) dz -= copysign(L, dz);
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 if constexpr (MODE == BH_R3) {
-                    // side / r < theta as side^2 < theta^2 r^2 and r >= beta as r^2 >= beta^2 (the
+                    // side / r < theta as side^2 / theta^2 < r^2 and r >= beta as r^2 >= beta^2 (the
                     // same decisions to rounding), and m r^-3 from the v_rsq_f64 seed y with one
                     // correction, y^3 (1 + 1.5 e), e = 1 - r^2 y^2: within 2.1e-14 (one-sided)
-                    if (q.pidx >= 0 || q.side2 < theta2 * r2) {
+                    if (q.pidx >= 0 || q.open2 < r2) {
                         wait = q.next;
                         const double beta = q.msoft + hi;
                         double wm;
@@ -1087,7 +1086,7 @@ hipError_t launch_bh_forces(const EvalPlan &p, const DeviceParticles &dp, void *
     hipLaunchKernelGGL(bh_scan_top_kernel, dim3(1), dim3(1024), 0, st, v.bsum, sb, v.info);
     hipLaunchKernelGGL(bh_scan_add_kernel, dim3(ng), dim3(256), 0, st, v.base, n, v.bsum);
     hipLaunchKernelGGL(bh_fill_kernel, dim3(ng), dim3(256), 0, st, v.lcp, v.skey, v.skey2, v.base, v.srec, v.sidx, n,
-                       red, p.bh_particle_radius, v.info, v.nodes, v.level);
+                       red, p.bh_particle_radius, 1.0 / (p.bh_theta * p.bh_theta), v.info, v.nodes, v.level);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // the items of one level are few next to the grid: a grid-stride loop over
     // the items the tree has (info[0], known only on the device)
