@@ -786,6 +786,38 @@ __device__ __forceinline__ void bh_walk(const BhNode *__restrict__ nodes, int to
                                         const BhTables &tb, double *f) {
     int wait = mine ? 0 : INT32_MAX;  // the next item this lane takes part in
     int it = 0;
+    if constexpr (MODE == BH_R3) {
+        // The R3 walk with selects in place of branches: the item's test and
+        // interaction are formed for every lane and kept where the lane takes
+        // part (the scalar unit, not the vector unit, bounds this walk, and
+        // each branch costs it exec-mask work); only the softened branch stays.
+        // side / r < theta as side^2 / theta^2 < r^2, r >= beta as r^2 >= beta^2
+        // (the same decisions to rounding), and m r^-3 from the v_rsq_f64 seed y
+        // with one correction, y^3 (1 + 1.5 e), e = 1 - r^2 y^2: within 2.1e-14
+        // (one-sided).
+        while (it < total) {
+            const BhNode q = nodes[it];
+            const bool active = wait <= it && q.pidx != i && q.pidx != kBhMassless;
+            const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const bool take = q.pidx >= 0 || q.open2 < r2;
+            const double beta = q.msoft + hi;
+            const double y0 = __builtin_amdgcn_rsq(r2);
+            const double t = y0 * y0;
+            const double e = fma(-r2, t, 1.0);
+            double wm = (t * (y0 * q.m)) * fma(e, 1.5, 1.0);
+            const bool soft = active && take && !(r2 >= beta * beta);
+            if (soft) wm = q.m * bh_soft(sqrt(r2), beta);
+            wm = (active && take) ? wm : 0.0;
+            f[0] = fma(wm, dx, f[0]);
+            f[1] = fma(wm, dy, f[1]);
+            f[2] = fma(wm, dz, f[2]);
+            const bool open = active && !take;
+            if (wait <= it && !open) wait = q.next;  // took or skipped it: wait for its skip link
+            it = __ballot(open) ? it + 1 : q.next;
+        }
+        return;
+    }
     while (it < total) {
         const BhNode q = nodes[it];
         bool open = false;
