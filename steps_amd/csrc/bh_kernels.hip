@@ -833,55 +833,31 @@ __device__ __forceinline__ void bh_walk(const BhNode *__restrict__ nodes, int to
                 if (MODE >= BH_T3_NEAR && MODE <= BH_S1R2_TABLE)  // forces.cc:600-601, 941-942
                     if (fabs(dz) > 0.5 * L) dz -= copysign(L, dz);
                 const double r2 = dx * dx + dy * dy + dz * dz;
-                if constexpr (MODE == BH_R3) {
-                    // side / r < theta as side^2 / theta^2 < r^2 and r >= beta as r^2 >= beta^2 (the
-                    // same decisions to rounding), and m r^-3 from the v_rsq_f64 seed y with one
-                    // correction, y^3 (1 + 1.5 e), e = 1 - r^2 y^2: within 2.1e-14 (one-sided)
-                    if (q.pidx >= 0 || q.open2 < r2) {
-                        wait = q.next;
+                double y, dist;
+                bh_rsq(r2, y, dist);
+                if (q.pidx >= 0 || q.side * y < theta) {
+                    wait = q.next;
+                    if (MODE != BH_S1R2_IMAGES || fabs(dz) <= cut) {  // forces.cc:915
                         const double beta = q.msoft + hi;
-                        double wm;
-                        if (r2 >= beta * beta) {
-                            const double y0 = __builtin_amdgcn_rsq(r2);
-                            const double t = y0 * y0;
-                            const double e = fma(-r2, t, 1.0);
-                            wm = (t * (y0 * q.m)) * fma(e, 1.5, 1.0);
+                        const double w = dist >= beta ? (y * y) * y : bh_soft(r2 > 0.0 ? dist : 0.0, beta);
+                        if constexpr (MODE == BH_T3_EWALD || MODE == BH_S1R2_TABLE) {  // forces.cc:606-610, 947-951
+                            double Dx, Dy, Dz;
+                            if constexpr (MODE == BH_T3_EWALD)
+                                bh_t3_correction<ORD>(tb, L, dx, dy, dz, Dx, Dy, Dz);
+                            else
+                                bh_s1r2_correction<ORD>(tb, L, dx, dy, dz, Dx, Dy, Dz);
+                            f[0] += q.m * (w * dx - Dx);
+                            f[1] += q.m * (w * dy - Dy);
+                            f[2] += q.m * (w * dz - Dz);
                         } else {
-                            wm = q.m * bh_soft(sqrt(r2), beta);
+                            const double wm = q.m * w;
+                            f[0] += wm * dx;
+                            f[1] += wm * dy;
+                            f[2] += wm * dz;
                         }
-                        f[0] += wm * dx;
-                        f[1] += wm * dy;
-                        f[2] += wm * dz;
-                    } else {
-                        open = true;  // its first child follows in pre-order
                     }
                 } else {
-                    double y, dist;
-                    bh_rsq(r2, y, dist);
-                    if (q.pidx >= 0 || q.side * y < theta) {
-                        wait = q.next;
-                        if (MODE != BH_S1R2_IMAGES || fabs(dz) <= cut) {  // forces.cc:915
-                            const double beta = q.msoft + hi;
-                            const double w = dist >= beta ? (y * y) * y : bh_soft(r2 > 0.0 ? dist : 0.0, beta);
-                            if constexpr (MODE == BH_T3_EWALD || MODE == BH_S1R2_TABLE) {  // forces.cc:606-610, 947-951
-                                double Dx, Dy, Dz;
-                                if constexpr (MODE == BH_T3_EWALD)
-                                    bh_t3_correction<ORD>(tb, L, dx, dy, dz, Dx, Dy, Dz);
-                                else
-                                    bh_s1r2_correction<ORD>(tb, L, dx, dy, dz, Dx, Dy, Dz);
-                                f[0] += q.m * (w * dx - Dx);
-                                f[1] += q.m * (w * dy - Dy);
-                                f[2] += q.m * (w * dz - Dz);
-                            } else {
-                                const double wm = q.m * w;
-                                f[0] += wm * dx;
-                                f[1] += wm * dy;
-                                f[2] += wm * dz;
-                            }
-                        }
-                    } else {
-                        open = true;  // its first child follows in pre-order
-                    }
+                    open = true;  // its first child follows in pre-order
                 }
             }
         }
