@@ -818,6 +818,38 @@ __device__ __forceinline__ void bh_walk(const BhNode *__restrict__ nodes, int to
         }
         return;
     }
+    if constexpr (MODE == BH_T3_NEAR || MODE == BH_S1R2_NEAR) {
+        // The same select walk for the nearest-image modes (2.5-3% faster at
+        // the 64^3 lattice and the N = 524288 cylinder); the second-order 1/r
+        // stays.  The table modes keep their branches: there the select walk
+        // made every lane gather table taps (S1R2 CIC 12.3 against 11.0 ms).
+        while (it < total) {
+            const BhNode q = nodes[it];
+            const bool active = wait <= it && q.pidx != i && q.pidx != kBhMassless;
+            double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+            if (MODE == BH_T3_NEAR) {  // forces.cc:594-599
+                if (fabs(dx) > 0.5 * L) dx -= copysign(L, dx);
+                if (fabs(dy) > 0.5 * L) dy -= copysign(L, dy);
+            }
+            if (fabs(dz) > 0.5 * L) dz -= copysign(L, dz);  // forces.cc:600-601, 941-942
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            double y, dist;
+            bh_rsq(r2, y, dist);
+            const bool take = q.pidx >= 0 || q.side * y < theta;
+            const bool use = active && take;
+            const double beta = q.msoft + hi;
+            double w = (y * y) * y;
+            if (use && !(dist >= beta)) w = bh_soft(r2 > 0.0 ? dist : 0.0, beta);
+            const double wm = use ? q.m * w : 0.0;  // selected: an unused lane's w may be NaN
+            f[0] += wm * dx;
+            f[1] += wm * dy;
+            f[2] += wm * dz;
+            const bool open = active && !take;
+            if (wait <= it && !open) wait = q.next;  // took or skipped it: wait for its skip link
+            it = __ballot(open) ? it + 1 : q.next;
+        }
+        return;
+    }
     while (it < total) {
         const BhNode q = nodes[it];
         bool open = false;
